@@ -1,0 +1,187 @@
+// capi.cpp -- the extern "C" boundary.  Every entry point catches all C++
+// exceptions and returns a status; nothing here exits the process (the
+// reference's CK macro exits, src/spgemm_engine.cuh:18).
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "engine.hpp"
+
+namespace appt {
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& m) { g_last_error = m; }
+}  // namespace appt
+
+using appt::Engine;
+using appt::Error;
+
+struct appt_engine {
+  std::unique_ptr<Engine> impl;
+};
+
+template <typename F>
+static appt_status guard(F&& f) {
+  try {
+    f();
+    return APPT_OK;
+  } catch (const Error& e) {
+    appt::set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    appt::set_last_error("host allocation failed");
+    return APPT_ERR_OOM;
+  } catch (const std::exception& e) {
+    appt::set_last_error(e.what());
+    return APPT_ERR_HIP;
+  } catch (...) {
+    appt::set_last_error("unknown error");
+    return APPT_ERR_HIP;
+  }
+}
+
+extern "C" {
+
+const char* appt_last_error(void) { return appt::g_last_error.c_str(); }
+
+appt_status appt_op_from_task(const char* task, appt_op* out) {
+  if (!task || !out) return APPT_ERR_INVALID;
+  if (std::strcmp(task, "AA") == 0) { *out = APPT_OP_AA; return APPT_OK; }
+  if (std::strcmp(task, "AAt") == 0) { *out = APPT_OP_AAT; return APPT_OK; }
+  if (std::strcmp(task, "AAtS") == 0) { *out = APPT_OP_AATS; return APPT_OK; }
+  appt::set_last_error(std::string("unknown task ") + task);
+  return APPT_ERR_INVALID;
+}
+
+appt_status appt_engine_create(const appt_options* opt, appt_engine** out) {
+  if (!out) return APPT_ERR_INVALID;
+  *out = nullptr;
+  return guard([&] {
+    auto* e = new appt_engine;
+    try {
+      e->impl.reset(new Engine(opt));
+    } catch (...) {
+      delete e;
+      throw;
+    }
+    *out = e;
+  });
+}
+
+appt_status appt_engine_destroy(appt_engine* eng) {
+  if (!eng) return APPT_OK;
+  return guard([&] { delete eng; });
+}
+
+appt_status appt_engine_set_timing(appt_engine* eng, int enable) {
+  if (!eng) return APPT_ERR_INVALID;
+  eng->impl->set_timing(enable != 0);
+  return APPT_OK;
+}
+
+appt_status appt_engine_phase_times(const appt_engine* eng, appt_phase_times* out) {
+  if (!eng || !out) return APPT_ERR_INVALID;
+  *out = eng->impl->times();
+  return APPT_OK;
+}
+
+appt_status appt_engine_trim(appt_engine* eng) {
+  if (!eng) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->pool().trim(); });
+}
+
+int64_t appt_engine_device_bytes(const appt_engine* eng) { return eng ? eng->impl->pool().allocated() : 0; }
+
+appt_status appt_spgemm_compute(appt_engine* eng, const appt_csr_view* A, appt_op op, appt_csr_result* C) {
+  if (!eng || !A || !C) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->compute(*A, op, C); });
+}
+
+appt_status appt_csr_release(appt_engine* eng, appt_csr_result* C) {
+  if (!eng || !C) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->release(C); });
+}
+
+appt_status appt_engine_run_timed(appt_engine* eng, const char* task, const appt_csr_view* A, int32_t reps,
+                                  appt_eng_result* out) {
+  if (!eng || !A || !out) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->run_timed(task, *A, reps, out); });
+}
+
+appt_status appt_spgemm_run_timed(const char* task, const int32_t* dOff, const int32_t* dCol, const double* dVal,
+                                  int32_t n, int64_t nnz, int32_t reps, appt_eng_result* out) {
+  if (!out) return APPT_ERR_INVALID;
+  return guard([&] {
+    // one engine per host thread, created on first use (the reference
+    // allocates per call, src/spgemm_engine.cuh:136-181, outside its timing)
+    static thread_local std::unique_ptr<Engine> eng;
+    if (!eng) {
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      appt_options o{};
+      o.device = dev;
+      eng.reset(new Engine(&o));
+    }
+    appt_csr_view A{n, nnz, dOff, dCol, dVal};
+    eng->run_timed(task, A, reps, out);
+  });
+}
+
+appt_status appt_dispatch_descriptors(appt_engine* eng, const appt_csr_view* A, appt_op op, appt_descriptors* out) {
+  if (!eng || !A || !out) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->descriptors(*A, op, out); });
+}
+
+appt_status appt_device_upload(appt_engine* eng, const appt_host_csr* h, appt_csr_view* v) {
+  if (!eng || !h || !v) return APPT_ERR_INVALID;
+  return guard([&] {
+    APPT_HIP(hipSetDevice(eng->impl->device()));
+    int32_t *rp = nullptr, *c = nullptr;
+    double* x = nullptr;
+    APPT_HIP(hipMalloc(&rp, sizeof(int32_t) * ((size_t)h->n + 1)));
+    APPT_HIP(hipMalloc(&c, sizeof(int32_t) * (size_t)(h->nnz > 0 ? h->nnz : 1)));
+    APPT_HIP(hipMalloc(&x, sizeof(double) * (size_t)(h->nnz > 0 ? h->nnz : 1)));
+    APPT_HIP(hipMemcpy(rp, h->rowptr, sizeof(int32_t) * ((size_t)h->n + 1), hipMemcpyHostToDevice));
+    if (h->nnz > 0) {
+      APPT_HIP(hipMemcpy(c, h->col, sizeof(int32_t) * (size_t)h->nnz, hipMemcpyHostToDevice));
+      APPT_HIP(hipMemcpy(x, h->val, sizeof(double) * (size_t)h->nnz, hipMemcpyHostToDevice));
+    }
+    v->n = h->n;
+    v->nnz = h->nnz;
+    v->rowptr = rp;
+    v->col = c;
+    v->val = x;
+  });
+}
+
+appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v) {
+  if (!eng || !v) return APPT_ERR_INVALID;
+  return guard([&] {
+    APPT_HIP(hipSetDevice(eng->impl->device()));
+    (void)hipFree((void*)v->rowptr);
+    (void)hipFree((void*)v->col);
+    (void)hipFree((void*)v->val);
+    std::memset(v, 0, sizeof(*v));
+  });
+}
+
+appt_status appt_csr_download(appt_engine* eng, const appt_csr_result* C, int64_t* rowptr, int32_t* col,
+                              double* val) {
+  if (!eng || !C) return APPT_ERR_INVALID;
+  return guard([&] {
+    APPT_HIP(hipSetDevice(eng->impl->device()));
+    const size_t n1 = (size_t)C->n + 1;
+    if (rowptr) {
+      if (C->rowptr_is_64) {
+        APPT_HIP(hipMemcpy(rowptr, C->rowptr, sizeof(int64_t) * n1, hipMemcpyDeviceToHost));
+      } else {
+        std::unique_ptr<int32_t[]> t(new int32_t[n1]);
+        APPT_HIP(hipMemcpy(t.get(), C->rowptr, sizeof(int32_t) * n1, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n1; i++) rowptr[i] = t[i];
+      }
+    }
+    if (col && C->nnz) APPT_HIP(hipMemcpy(col, C->col, sizeof(int32_t) * (size_t)C->nnz, hipMemcpyDeviceToHost));
+    if (val && C->nnz) APPT_HIP(hipMemcpy(val, C->val, sizeof(double) * (size_t)C->nnz, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// kernels.hpp -- host-side launchers of the gfx950 SpGEMM kernels.
+//
+// Row bins (one kernel configuration each):
+//   0                 empty output row
+//   1 .. 24           LDS group bins: 1 + (cls*2 + mode)*4 + gidx
+//                       cls  : products (symbolic) / row nnz (numeric) class
+//                       mode : 0 dense bitmap window, 1 open-addressing hash
+//                       gidx : lanes per row G = 8, 16, 32, 64
+//   25 / 26           one workgroup per row, bitmap / hash in LDS
+//   27                heavy rows: one workgroup per row, bitmap window in a
+//                     global scratch slot
+//   28                heavy rows whose span is too wide for a bitmap slot
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace appt {
+
+constexpr int kNumCls = 3;
+constexpr int kNumBins = 29;
+constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinGlBitmap = 27, kBinGlHash = 28;
+constexpr int kStatFields = 3;  // count, max metric, max words
+
+struct BinParams {
+  int32_t ncls;
+  int32_t cls_max[kNumCls];       // metric bound of class c
+  int32_t cls_bm_words[kNumCls];  // dense window if words(span) <= this
+  int32_t wg_max;                 // metric bound of the workgroup bin
+  int32_t wg_bm_words;
+  int32_t gl_bm_words;            // global bitmap if words <= this
+  int32_t glen_max[3];            // mean B-row length bounds for G = 8, 16, 32
+};
+
+inline int bin_group(int bin) { return 8 << ((bin - 1) % 4); }
+inline int bin_cls(int bin) { return (bin - 1) / 8; }
+inline bool bin_is_bitmap(int bin) { return ((bin - 1) / 4) % 2 == 0; }
+
+// Symbolic hash table slots for a row with f products (np2l((4f)/3+1) of the
+// reference sizing rule); numeric hash slots for a row with c outputs.
+inline int64_t np2_host(int64_t x) { if (x < 2) return 2; int64_t p = 2; while (p < x) p <<= 1; return p; }
+inline int64_t sym_slots(int64_t f) { return np2_host((f * 4) / 3 + 1); }
+inline int64_t num_slots(int64_t c) { return np2_host((c * 3) / 2 + 1); }
+
+// ---- launchers (all asynchronous on `s`) ----
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* rmin, int32_t* rmax,
+                     hipStream_t s);
+void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
+void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
+                              int32_t* cursor, int32_t* bc, double* bv, hipStream_t s);
+// Exclusive scan of n int32 counts into out64[0..n] (out64[n] = total); out32
+// (optional) receives the same values truncated to int32.  `tmp` needs
+// scan_tmp_elems(n) int64.
+int64_t scan_tmp_elems(int64_t n);
+void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, int64_t* tmp,
+                 hipStream_t s);
+void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int32_t* bp,
+                    const int32_t* bmin, const int32_t* bmax, int upper, const BinParams& P,
+                    int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin, int32_t* stats,
+                    unsigned long long* total_products, hipStream_t s);
+void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
+                   const int32_t* rowlo, const int32_t* rowhi, const BinParams& P, uint8_t* bin,
+                   int32_t* stats, hipStream_t s);
+void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s);
+
+struct RowArgs {
+  const int32_t* rows;   // row ids of this bin
+  int32_t nrows;
+  const int32_t* ap; const int32_t* ac; const double* av;
+  const int32_t* bp; const int32_t* bc; const double* bv;
+  const int64_t* rowf; const int32_t* rowlo; const int32_t* rowhi;
+  int upper;
+};
+
+// Symbolic: rowcnt[i] = distinct output columns of row i.
+// lds_row_words: u32 words of LDS per row (bitmap words or hash slots).
+void launch_symbolic_bin(int bin, const RowArgs& a, int32_t lds_row_words, int32_t* rowcnt,
+                         uint32_t* gscratch, int32_t gslots, hipStream_t s);
+// Numeric: writes sorted col/val of each row at rowoff[i].
+// lds_row_bytes: LDS bytes per row; words_max / c_max: bin maxima.
+void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_max,
+                        const int64_t* rowoff, int32_t* ccol, double* cval, uint32_t* gscratch,
+                        int32_t gslots, hipStream_t s);
+// Bytes of LDS one row of a bin needs (0 for global bins).
+int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max);
+int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max);
+
+// AAtS mirror helpers.
+void launch_lower_count(int32_t n, const int64_t* uoff, const int32_t* ucol, int32_t* low,
+                        hipStream_t s);
+void launch_add_counts(int32_t n, const int32_t* a, const int64_t* boff, int32_t* out, hipStream_t s);
+void launch_mirror_scatter(int32_t n, const int64_t* uoff, const int32_t* ucol, const double* uval,
+                           const int64_t* coff, const int32_t* low, int32_t* cursor, int32_t* ccol,
+                           double* cval, hipStream_t s);
+// Sort the segments [off[i], off[i] + len[i]) of (col, val) by col; keys in a
+// segment are distinct.  Segments longer than seg_cap_lds use a global
+// bitmap rank over [kmin[i], kmax[i]] in gscratch.
+void launch_segment_sort(int32_t n, const int64_t* off, const int32_t* len, int32_t* col, double* val,
+                         int32_t max_len, hipStream_t s);
+
+void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s);
+void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s);
+
+}  // namespace appt

@@ -1,0 +1,254 @@
+"""Python mirror of the reference's engine interface over the C ABI.
+
+Reference interface (Wendy-Hamlet/appt_spgemm):
+  run_ours(task, dOff, dCol, dVal, n, nnz, reps) -> EngResult
+      src/spgemm_engine.cuh:132-133, EngResult :129, tasks "AA" | "AAt" | "AAtS" :134
+  load_bin / write_bin (CSR1)            src/spgemm_engine.cuh:22-32, src/export_bin.py:20-29
+  dispatcher descriptors                  bench/dispatch.cu:26-38
+
+All compute runs in ``libappt_spgemm.so`` on the GPU; this module only moves
+host arrays and handles.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._abi import (OPS, CsrResult, CsrView, Descriptors, EngResult, HostCsr, Options, PhaseTimes,
+                   Tuning, check, lib)
+
+
+# ---------------------------------------------------------------------------
+# host matrices
+# ---------------------------------------------------------------------------
+@dataclass
+class HostMatrix:
+    """Square CSR matrix in host memory (int32 indices, fp64 values)."""
+    n: int
+    rowptr: np.ndarray
+    col: np.ndarray
+    val: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col.shape[0])
+
+    def _as_c(self) -> HostCsr:
+        self.rowptr = np.ascontiguousarray(self.rowptr, dtype=np.int32)
+        self.col = np.ascontiguousarray(self.col, dtype=np.int32)
+        self.val = np.ascontiguousarray(self.val, dtype=np.float64)
+        return HostCsr(self.n, self.nnz, self.rowptr.ctypes.data_as(C.POINTER(C.c_int32)),
+                       self.col.ctypes.data_as(C.POINTER(C.c_int32)),
+                       self.val.ctypes.data_as(C.POINTER(C.c_double)))
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.val, self.col, self.rowptr), shape=(self.n, self.n))
+
+
+def _take(h: HostCsr, meta: dict) -> HostMatrix:
+    """Copy a library-owned host CSR into numpy and free it."""
+    n, nnz = int(h.n), int(h.nnz)
+    rp = np.ctypeslib.as_array(h.rowptr, shape=(n + 1,)).copy()
+    col = np.ctypeslib.as_array(h.col, shape=(max(nnz, 1),))[:nnz].copy()
+    val = np.ctypeslib.as_array(h.val, shape=(max(nnz, 1),))[:nnz].copy()
+    lib().appt_host_csr_free(C.byref(h))
+    return HostMatrix(n, rp, col, val, meta)
+
+
+def laplace2d(g: int) -> HostMatrix:
+    h = HostCsr()
+    check(lib().appt_gen_laplace2d(g, C.byref(h)), "appt_gen_laplace2d")
+    return _take(h, {"gen": "laplace2d", "g": g})
+
+
+def stencil27(g: int, drop_p: float = 0.0, seed: int = 0) -> HostMatrix:
+    h = HostCsr()
+    check(lib().appt_gen_stencil27(g, drop_p, seed, C.byref(h)), "appt_gen_stencil27")
+    return _take(h, {"gen": "stencil27", "g": g, "drop_p": drop_p, "seed": seed})
+
+
+def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19, c: float = 0.19,
+         seed: int = 1) -> HostMatrix:
+    h = HostCsr()
+    check(lib().appt_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(h)), "appt_gen_rmat")
+    return _take(h, {"gen": "rmat", "scale": scale, "edge_factor": edge_factor, "abc": (a, b, c),
+                     "seed": seed})
+
+
+def banded(n: int, bw: int, fill: float = 0.7, seed: int = 0) -> HostMatrix:
+    h = HostCsr()
+    check(lib().appt_gen_banded(n, bw, fill, seed, C.byref(h)), "appt_gen_banded")
+    return _take(h, {"gen": "banded", "n": n, "bw": bw, "fill": fill, "seed": seed})
+
+
+def erdos_renyi(n: int, mean_deg: float, seed: int = 0) -> HostMatrix:
+    h = HostCsr()
+    check(lib().appt_gen_erdos_renyi(n, mean_deg, seed, C.byref(h)), "appt_gen_erdos_renyi")
+    return _take(h, {"gen": "erdos_renyi", "n": n, "mean_deg": mean_deg, "seed": seed})
+
+
+def csr1_read(path: str) -> HostMatrix:
+    """load_bin (src/spgemm_engine.cuh:22-32) with structural validation."""
+    h = HostCsr()
+    check(lib().appt_csr1_read(str(path).encode(), C.byref(h)), "appt_csr1_read")
+    return _take(h, {"path": str(path)})
+
+
+def csr1_write(path: str, m: HostMatrix) -> None:
+    """write_bin (src/export_bin.py:20-29).  Writes rows as given."""
+    h = m._as_c()
+    check(lib().appt_csr1_write(str(path).encode(), C.byref(h)), "appt_csr1_write")
+
+
+# ---------------------------------------------------------------------------
+# device engine
+# ---------------------------------------------------------------------------
+class DeviceMatrix:
+    """Device CSR uploaded through the engine (owned; freed by ``free``)."""
+
+    def __init__(self, engine: "Engine", view: CsrView):
+        self._engine = engine
+        self.view = view
+
+    @property
+    def n(self) -> int:
+        return int(self.view.n)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.view.nnz)
+
+    def free(self) -> None:
+        if self.view.rowptr:
+            check(lib().appt_device_free_view(self._engine.handle, C.byref(self.view)), "free_view")
+
+
+class DeviceResult:
+    """Device CSR result; buffers belong to the engine until ``release``."""
+
+    def __init__(self, engine: "Engine", res: CsrResult):
+        self._engine = engine
+        self.res = res
+
+    @property
+    def n(self) -> int:
+        return int(self.res.n)
+
+    @property
+    def nnz(self) -> int:
+        return int(self.res.nnz)
+
+    def download(self) -> HostMatrix:
+        n, nnz = self.n, self.nnz
+        rp = np.empty(n + 1, dtype=np.int64)
+        col = np.empty(max(nnz, 1), dtype=np.int32)
+        val = np.empty(max(nnz, 1), dtype=np.float64)
+        check(lib().appt_csr_download(self._engine.handle, C.byref(self.res),
+                                      rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                      col.ctypes.data_as(C.POINTER(C.c_int32)),
+                                      val.ctypes.data_as(C.POINTER(C.c_double))), "csr_download")
+        return HostMatrix(n, rp, col[:nnz], val[:nnz])
+
+    def release(self) -> None:
+        if self.res._priv:
+            check(lib().appt_csr_release(self._engine.handle, C.byref(self.res)), "csr_release")
+
+
+class Engine:
+    """One engine per host thread; owns a HIP stream and a device memory pool."""
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0, tuning: dict | None = None,
+                 stream: int | None = None, timing: bool = False):
+        opt = Options()
+        opt.device = device
+        opt.stream = stream
+        opt.hbm_budget_bytes = hbm_budget_bytes
+        if tuning:
+            t = Tuning()
+            for k, v in tuning.items():
+                if k in ("f_class_max", "group_for_len"):
+                    arr = getattr(t, k)
+                    for i, x in enumerate(v):
+                        arr[i] = int(x)
+                else:
+                    setattr(t, k, int(v))
+            opt.tuning = t
+        h = C.c_void_p()
+        check(lib().appt_engine_create(C.byref(opt), C.byref(h)), "appt_engine_create")
+        self.handle = h
+        if timing:
+            self.set_timing(True)
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().appt_engine_destroy(self.handle), "appt_engine_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().appt_engine_set_timing(self.handle, 1 if on else 0), "set_timing")
+
+    def phase_times(self) -> dict:
+        t = PhaseTimes()
+        check(lib().appt_engine_phase_times(self.handle, C.byref(t)), "phase_times")
+        return {f: getattr(t, f) for f, _ in PhaseTimes._fields_}
+
+    def device_bytes(self) -> int:
+        return int(lib().appt_engine_device_bytes(self.handle))
+
+    def trim(self) -> None:
+        check(lib().appt_engine_trim(self.handle), "trim")
+
+    def upload(self, m: HostMatrix) -> DeviceMatrix:
+        v = CsrView()
+        h = m._as_c()
+        check(lib().appt_device_upload(self.handle, C.byref(h), C.byref(v)), "device_upload")
+        return DeviceMatrix(self, v)
+
+    def compute(self, A: DeviceMatrix, op: str = "AA") -> DeviceResult:
+        r = CsrResult()
+        check(lib().appt_spgemm_compute(self.handle, C.byref(A.view), OPS[op], C.byref(r)),
+              f"appt_spgemm_compute({op})")
+        return DeviceResult(self, r)
+
+    def multiply(self, m: HostMatrix, op: str = "AA") -> HostMatrix:
+        """Upload, compute, download, release: C = op(m) as a host matrix."""
+        A = self.upload(m)
+        try:
+            R = self.compute(A, op)
+            try:
+                return R.download()
+            finally:
+                R.release()
+        finally:
+            A.free()
+
+    def run_timed(self, task: str, A: DeviceMatrix, reps: int) -> EngResult:
+        out = EngResult()
+        check(lib().appt_engine_run_timed(self.handle, task.encode(), C.byref(A.view), reps,
+                                          C.byref(out)), "run_timed")
+        return out
+
+    def descriptors(self, A: DeviceMatrix, op: str = "AA") -> dict:
+        d = Descriptors()
+        check(lib().appt_dispatch_descriptors(self.handle, C.byref(A.view), OPS[op], C.byref(d)),
+              "dispatch_descriptors")
+        return {f: getattr(d, f) for f, _ in Descriptors._fields_}
+
+
+def run_ours(task: str, dOff: int, dCol: int, dVal: int, n: int, nnz: int, reps: int) -> EngResult:
+    """run_ours(task, dOff, dCol, dVal, n, nnz, reps) of src/spgemm_engine.cuh:132-133:
+    device pointers in, EngResult {ms, nnz, s, as, sq} out (mean ms over reps)."""
+    out = EngResult()
+    check(lib().appt_spgemm_run_timed(task.encode(), dOff, dCol, dVal, n, nnz, reps, C.byref(out)),
+          f"run_ours({task})")
+    return out

@@ -1,0 +1,212 @@
+/*
+ * appt_spgemm.h -- C ABI of the MI355X (gfx950) SpGEMM engine.
+ *
+ * Computes C = A*A, C = A*A^T, or C = A*A^T via the symmetric upper-triangle
+ * Gram kernel, for square CSR matrices with int32 indices and fp64 values,
+ * using structural semantics: an entry whose contributions cancel to 0.0 is
+ * kept, columns of every output row are strictly ascending.
+ *
+ * Plain pointers and sizes only; every function returns an appt_status and
+ * never exits the process.  All "device" pointers are HIP device memory on the
+ * engine's device.  One engine per host thread (thread-compatible, not
+ * thread-safe); engines on different threads are independent.
+ *
+ * Reference interface each entry point replaces is cited as file:line of
+ * Wendy-Hamlet/appt_spgemm.
+ */
+#ifndef APPT_SPGEMM_H
+#define APPT_SPGEMM_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference prints and exit(1)s on every error
+ * (src/spgemm_engine.cuh:18 CK macro); this ABI returns them instead. */
+typedef enum appt_status {
+  APPT_OK = 0,
+  APPT_ERR_INVALID = 1,      /* bad argument / malformed CSR / bad task string */
+  APPT_ERR_HIP = 2,          /* a HIP runtime call failed */
+  APPT_ERR_OOM = 3,          /* device allocation failed or budget exceeded */
+  APPT_ERR_ADMISSION = 4,    /* job rejected by the HBM budget admission */
+  APPT_ERR_IO = 5,           /* CSR1 file could not be read/written */
+  APPT_ERR_UNSUPPORTED = 6,  /* shape outside what the kernels support */
+  APPT_ERR_OVERFLOW = 7,     /* a size does not fit the requested index width */
+  APPT_ERR_NO_DEVICE = 8     /* no HIP device visible */
+} appt_status;
+
+/* Operation.  Task strings "AA" | "AAt" | "AAtS" of src/spgemm_engine.cuh:134. */
+typedef enum appt_op {
+  APPT_OP_AA = 0,    /* C = A*A */
+  APPT_OP_AAT = 1,   /* C = A*A^T, B = A^T materialised */
+  APPT_OP_AATS = 2   /* C = A*A^T, upper triangle computed then mirrored */
+} appt_op;
+
+/* Device CSR operand (src/spgemm_engine.cuh:132 dOff/dCol/dVal, n, nnz).
+ * Rows need not be sorted; duplicate columns inside a row are merged by
+ * summation.  The engine only reads these arrays. */
+typedef struct appt_csr_view {
+  int32_t n;                /* rows == columns */
+  int64_t nnz;              /* < 2^31 */
+  const int32_t* rowptr;    /* device, n+1 */
+  const int32_t* col;       /* device, nnz */
+  const double* val;        /* device, nnz */
+} appt_csr_view;
+
+/* Device CSR result.  rowptr is int32 when nnz < 2^31, else int64
+ * (rowptr_is_64 = 1).  The buffers belong to the engine until
+ * appt_csr_release(). */
+typedef struct appt_csr_result {
+  int32_t n;
+  int64_t nnz;
+  int32_t rowptr_is_64;
+  void* rowptr;             /* device, n+1 entries of int32 or int64 */
+  int32_t* col;             /* device, nnz */
+  double* val;              /* device, nnz */
+  void* _priv;              /* engine bookkeeping; do not touch */
+} appt_csr_result;
+
+/* struct EngResult { float ms; int64_t nnz; double s, as, sq; }
+ * (src/spgemm_engine.cuh:129), same field order and types. */
+typedef struct appt_eng_result {
+  float ms;
+  int64_t nnz;
+  double s, as, sq;
+} appt_eng_result;
+
+/* Bin thresholds and launch shapes.  Zero-filled means "built-in defaults". */
+#define APPT_MAX_CLASSES 8
+typedef struct appt_tuning {
+  int32_t f_class_max[APPT_MAX_CLASSES]; /* products upper bound of LDS classes, ascending, 0-terminated */
+  int32_t wg_f_max;           /* rows with products <= this (and above the classes) use a workgroup per row */
+  int32_t lds_budget_bytes;   /* LDS bytes a workgroup may take */
+  int32_t bitmap_words_factor;/* use the dense bitmap window when words(span) <= factor * table slots / 8 */
+  int32_t group_for_len[4];   /* lanes per row for mean B-row length <= 8,16,32,inf (0 = default) */
+  int32_t reserved[8];
+} appt_tuning;
+
+typedef struct appt_options {
+  int32_t device;             /* HIP device ordinal */
+  void* stream;               /* hipStream_t to run on; NULL = engine creates its own default-priority stream */
+  int64_t hbm_budget_bytes;   /* cap on engine-allocated device memory; 0 = 90% of free memory at create */
+  appt_tuning tuning;
+} appt_options;
+
+typedef struct appt_engine appt_engine;
+
+/* Per-phase device times of the last compute call, in milliseconds
+ * (measured with HIP events when timing is enabled). */
+typedef struct appt_phase_times {
+  float transpose_ms, prepass_ms, symbolic_ms, scan_ms, numeric_ms, mirror_ms, total_ms;
+  int64_t products;            /* sum over A's nonzeros of nnz(B row k): 2*products = FLOPs */
+  int64_t nnz_c;
+  int32_t n_bins_used;
+  int32_t sync_points;         /* device->host synchronisations in the call */
+} appt_phase_times;
+
+/* Structural descriptors of src/fetch_and_prep.py:37-48 and bench/dispatch.cu:26-37
+ * plus the products / span summary the internal dispatcher bins rows by. */
+typedef struct appt_descriptors {
+  int32_t n;
+  int64_t nnz;
+  double deg_mean, deg_cv;     /* row-degree mean and std/mean (ddof=0) */
+  int32_t deg_max;
+  double fill_aa;              /* sum_p deg(col_p) / nnz       (bench/dispatch.cu:31-33) */
+  double fill_aat;             /* sum_k colcount_k^2 / nnz     (bench/dispatch.cu:34-36) */
+  double psym;                 /* nnz(P & P^T) / nnz(P)        (src/fetch_and_prep.py:42-46) */
+  int64_t products_aa, products_aat;
+  int32_t max_row_products_aa; /* clamped to int32 range */
+  int32_t bandwidth;           /* max |i - j| over stored entries */
+  int32_t pick_ours;           /* decision rule of bench/dispatch.cu:14-17 on (deg_cv, fill, nnz) */
+  int32_t plan_bins;           /* number of non-empty row bins the engine would launch */
+} appt_descriptors;
+
+/* ---- engine lifetime ------------------------------------------------------ */
+appt_status appt_engine_create(const appt_options* opt, appt_engine** out);
+appt_status appt_engine_destroy(appt_engine* eng);
+/* Enable per-phase HIP-event timing (adds event records, no extra syncs). */
+appt_status appt_engine_set_timing(appt_engine* eng, int enable);
+appt_status appt_engine_phase_times(const appt_engine* eng, appt_phase_times* out);
+/* Release cached device memory held by the engine's pool. */
+appt_status appt_engine_trim(appt_engine* eng);
+/* Bytes currently allocated from HIP by the engine (pool + live results). */
+int64_t appt_engine_device_bytes(const appt_engine* eng);
+/* Last error text of this engine (or of the thread's last engine-less call). */
+const char* appt_last_error(void);
+
+/* ---- the hot path ---------------------------------------------------------- */
+/* C = op(A).  Replaces run_ours()'s pipeline (src/spgemm_engine.cuh:132-210)
+ * and, unlike it, returns C (the reference frees C, :203-209).  Synchronous
+ * with respect to the host on return (C is complete). */
+appt_status appt_spgemm_compute(appt_engine* eng, const appt_csr_view* A, appt_op op,
+                                appt_csr_result* C);
+appt_status appt_csr_release(appt_engine* eng, appt_csr_result* C);
+
+/* run_ours(task, dOff, dCol, dVal, n, nnz, reps) (src/spgemm_engine.cuh:132-133):
+ * one checked call (checksums from the sorted values on the host, :194-196),
+ * then `reps` timed calls; ms is the mean per call over the timed loop
+ * (:199-202).  Timed scope: transpose, pre-pass, symbolic, scans, numeric,
+ * mirror. */
+appt_status appt_spgemm_run_timed(const char* task, const int32_t* dOff, const int32_t* dCol,
+                                  const double* dVal, int32_t n, int64_t nnz, int32_t reps,
+                                  appt_eng_result* out);
+/* Same on a caller-owned engine (keeps its pool and tuning). */
+appt_status appt_engine_run_timed(appt_engine* eng, const char* task, const appt_csr_view* A,
+                                  int32_t reps, appt_eng_result* out);
+
+/* Descriptors + plan (bench/dispatch.cu:26-38), computed on the GPU. */
+appt_status appt_dispatch_descriptors(appt_engine* eng, const appt_csr_view* A, appt_op op,
+                                      appt_descriptors* out);
+
+/* Parse "AA" | "AAt" | "AAtS" (src/spgemm_engine.cuh:134). */
+appt_status appt_op_from_task(const char* task, appt_op* out);
+
+/* ---- host CSR, CSR1 files and synthetic generators --------------------------- */
+/* Host CSR owned by the library (malloc'd); free with appt_host_csr_free. */
+typedef struct appt_host_csr {
+  int32_t n;
+  int64_t nnz;
+  int32_t* rowptr;   /* n+1 */
+  int32_t* col;      /* nnz */
+  double* val;       /* nnz */
+} appt_host_csr;
+
+void appt_host_csr_free(appt_host_csr* m);
+
+/* CSR1 layout of src/export_bin.py:6-8 (reader src/spgemm_engine.cuh:22-32):
+ * "CSR1", int32 n, int32 pad, int64 nnz, int32 indptr[n+1], int32 indices[nnz],
+ * f64 data[nnz], little-endian. */
+appt_status appt_csr1_read(const char* path, appt_host_csr* out);
+appt_status appt_csr1_write(const char* path, const appt_host_csr* m);
+
+/* 2D 5-point Laplacian on a g x g grid, 4 on the diagonal, -1 off it. */
+appt_status appt_gen_laplace2d(int32_t g, appt_host_csr* out);
+/* 27-point stencil on a g^3 grid, index x + g*y + g*g*z, values U(-1,1) from
+ * seed; drop_p > 0 drops each off-diagonal independently with probability
+ * drop_p (second random stream of the same seed). */
+appt_status appt_gen_stencil27(int32_t g, double drop_p, uint64_t seed, appt_host_csr* out);
+/* R-MAT: 2^scale rows, edge_factor * 2^scale sampled edges with quadrant
+ * probabilities (a,b,c, d = 1-a-b-c), duplicates merged, values U(0,1). */
+appt_status appt_gen_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                          uint64_t seed, appt_host_csr* out);
+/* Random banded: half-bandwidth bw, each in-band off-diagonal kept with
+ * probability fill, diagonal always kept, values U(-1,1). */
+appt_status appt_gen_banded(int32_t n, int32_t bw, double fill, uint64_t seed, appt_host_csr* out);
+/* Erdos-Renyi: round(mean_deg * n) uniform (i,j) pairs, duplicates merged,
+ * values U(-1,1). */
+appt_status appt_gen_erdos_renyi(int32_t n, double mean_deg, uint64_t seed, appt_host_csr* out);
+
+/* ---- host <-> device helpers (for callers without their own allocator) ------- */
+appt_status appt_device_upload(appt_engine* eng, const appt_host_csr* h, appt_csr_view* out_dev);
+appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v);
+/* Copy a device result to caller-provided host arrays (rowptr as int64). */
+appt_status appt_csr_download(appt_engine* eng, const appt_csr_result* C, int64_t* rowptr,
+                              int32_t* col, double* val);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* APPT_SPGEMM_H */

@@ -1,0 +1,260 @@
+"""GPU parity: every product computed by the HIP engine (through the C ABI)
+against the CPU oracle on the same inputs.  Structure must be identical;
+values bit-equal where the row runs in a wave-level bin (C1, stencils,
+banded, Erdos-Renyi), else within 1e-12 * sum|a*b| per entry."""
+from pathlib import Path
+import sys
+
+import numpy as np
+import pytest
+
+import appt_spgemm_amd as ap
+import oracle
+from tests._util import assert_product, assert_structure, assert_values, ref, rows_sorted
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden" / "golden_small.npz"
+OPS = ["AA", "AAt", "AAtS"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = ap.Engine(0)
+    yield e
+    e.close()
+
+
+def host(n, rp, col, val):
+    return ap.HostMatrix(n, np.asarray(rp, np.int32), np.asarray(col, np.int32), np.asarray(val, np.float64))
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_laplace_c1(eng, op):
+    say("case laplace2d g=64", op)
+    m = ap.laplace2d(64)
+    got = eng.multiply(m, op)
+    assert got.nnz == 51972
+    assert_product(got, ref(m, op), exact=True)
+
+
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("drop", [0.0, 0.1])
+def test_stencil_small(eng, op, drop):
+    say("case stencil27 g=14", drop, op)
+    m = ap.stencil27(14, drop, 5)
+    got = eng.multiply(m, op)
+    assert_product(got, ref(m, op), exact=True)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_golden_fixtures(eng, op):
+    z = np.load(GOLD)
+    for t in range(int(z["ncases"])):
+        n = int(z[f"c{t}_n"])
+        m = host(n, z[f"c{t}_rowptr"], z[f"c{t}_col"], z[f"c{t}_val"])
+        say("case golden", t, op)
+        got = eng.multiply(m, op)
+        r = ref(m, op)
+        # rows with duplicate columns may associate differently: tolerance only
+        assert_product(got, r, exact=False)
+        key = "AAt" if op == "AAtS" else op
+        assert_structure(got.rowptr, got.col, z[f"c{t}_{key}_rowptr"], z[f"c{t}_{key}_col"])
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_banded_and_er(eng, op):
+    for m in (ap.banded(4000, 32, 0.7, 3), ap.banded(3000, 1, 0.7, 4), ap.erdos_renyi(6000, 12.0, 5),
+              ap.erdos_renyi(5000, 2.0, 6)):
+        say("case", m.meta, op)
+        got = eng.multiply(m, op)
+        assert_product(got, ref(m, op), exact=True)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_rmat_small(eng, op):
+    m = ap.rmat(12, 16, seed=1)
+    say("case rmat scale 12", op)
+    got = eng.multiply(m, op)
+    assert_product(got, ref(m, op), exact=False)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_heavy_rows(eng, op):
+    # arrow matrix: one dense row and column -> workgroup and global bins
+    n = 20000
+    rng = np.random.default_rng(7)
+    rows = [np.arange(n)]
+    cols = [np.arange(n)]
+    rows.append(np.zeros(n - 1, np.int64)); cols.append(np.arange(1, n))
+    rows.append(np.arange(1, n)); cols.append(np.zeros(n - 1, np.int64))
+    for k in (300, 3000):
+        r = rng.choice(n, k, replace=False); rows.append(r); cols.append(np.full(k, 17))
+    r = np.concatenate(rows); c = np.concatenate(cols)
+    key = np.unique(r * n + c)
+    r, c = key // n, key % n
+    rp = np.zeros(n + 1, np.int64); np.add.at(rp, r + 1, 1); rp = np.cumsum(rp)
+    m = host(n, rp, c, rng.uniform(0, 1, len(c)))
+    say("case arrow n=20000", op)
+    got = eng.multiply(m, op)
+    assert_product(got, ref(m, op), exact=False)
+
+
+def edge_matrices():
+    yield "n0", host(0, [0], [], [])
+    yield "n1", host(1, [0, 1], [0], [3.0])
+    yield "n1_empty", host(1, [0, 0], [], [])
+    yield "all_empty", host(10, [0] * 11, [], [])
+    yield "empty_rows", host(6, [0, 2, 2, 3, 3, 5, 5], [1, 4, 0, 2, 5], [1.0, 2.0, 3.0, 4.0, 5.0])
+    yield "unsorted", host(4, [0, 3, 4, 6, 7], [3, 0, 2, 1, 3, 0, 2], [1, 2, 3, 4, 5, 6, 7.0])
+    yield "duplicates", host(3, [0, 3, 4, 6], [1, 1, 0, 2, 0, 0], [1.0, 2.0, 3.0, 4.0, 5.0, 6.0])
+    yield "explicit_zeros", host(3, [0, 2, 3, 5], [0, 1, 2, 0, 2], [0.0, 1.0, 0.0, 2.0, 0.0])
+    yield "cancel", host(2, [0, 2, 4], [0, 1, 0, 1], [1.0, 1.0, 1.0, -1.0])
+    yield "last_col_only", host(5, [0, 1, 2, 3, 4, 5], [4, 4, 4, 4, 4], [1.0, 2.0, 3.0, 4.0, 5.0])
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_edge_cases(eng, op):
+    for name, m in edge_matrices():
+        say("case edge", name, op)
+        got = eng.multiply(m, op)
+        assert_product(got, ref(m, op), exact=name != "duplicates")
+
+
+def test_cancellation_structural(eng):
+    m = dict(edge_matrices())["cancel"]
+    got = eng.multiply(m, "AAt")
+    assert got.rowptr.tolist() == [0, 2, 4] and got.col.tolist() == [0, 1, 0, 1]
+    assert got.val.tolist() == [2.0, 0.0, 0.0, 2.0]
+
+
+def test_deterministic_repeat(eng):
+    m = ap.stencil27(12, 0.1, 9)
+    a = eng.multiply(m, "AAt")
+    b = eng.multiply(m, "AAt")
+    np.testing.assert_array_equal(a.col, b.col)
+    np.testing.assert_array_equal(a.val, b.val)
+
+
+def test_symmetric_invariants_gpu(eng):
+    m = ap.laplace2d(40)
+    aa, aat = eng.multiply(m, "AA"), eng.multiply(m, "AAt")
+    np.testing.assert_array_equal(aa.rowptr, aat.rowptr)
+    np.testing.assert_array_equal(aa.col, aat.col)
+    np.testing.assert_array_equal(aa.val, aat.val)
+    u = ap.stencil27(10, 0.1, 2)
+    aats = eng.multiply(u, "AAtS")
+    C = aats.to_scipy() if hasattr(aats, "to_scipy") else None
+    C = ap.HostMatrix(u.n, aats.rowptr, aats.col, aats.val).to_scipy()
+    assert (C != C.T).nnz == 0
+
+
+def test_run_timed_checksums(eng):
+    # run_ours contract: nnz + (s, as, sq) of the sorted values (src/spgemm_engine.cuh:194-196)
+    m = ap.stencil27(16, 0.1, 4)
+    A = eng.upload(m)
+    try:
+        for task in OPS:
+            say("case run_timed", task)
+            r = eng.run_timed(task, A, 3)
+            rp, col, val, _ = ref(m, task)
+            s, as_, sq = oracle.checksum(val)
+            assert r.nnz == rp[-1]
+            for a, b in ((r.s, s), (r.as_, as_), (r.sq, sq)):
+                assert abs(a - b) <= 1e-6 * max(1.0, abs(b))  # bench/run_compare.py:42-43
+            assert r.ms > 0
+    finally:
+        A.free()
+
+
+def test_run_ours_free_function(eng):
+    m = ap.laplace2d(32)
+    A = eng.upload(m)
+    try:
+        r = ap.run_ours("AA", A.view.rowptr, A.view.col, A.view.val, m.n, m.nnz, 2)
+        assert r.nnz == oracle.spgemm(m.n, m.rowptr, m.col, m.val, "AA")[0][-1]
+    finally:
+        A.free()
+
+
+def test_descriptors_match_oracle(eng):
+    for m in (ap.rmat(11, 16, seed=3), ap.laplace2d(30), ap.stencil27(8, 0.1, 1)):
+        A = eng.upload(m)
+        try:
+            for op in ("AA", "AAt"):
+                d = eng.descriptors(A, op)
+                o = oracle.descriptors(m.n, m.rowptr, m.col)
+                for k in ("deg_mean", "deg_cv", "fill_aa", "fill_aat", "psym"):
+                    assert d[k] == pytest.approx(o[k], rel=1e-12, abs=1e-12), k
+                assert d["products_aa"] == o["products_aa"]
+                assert d["products_aat"] == o["products_aat"]
+                assert d["bandwidth"] == o["bandwidth"]
+                assert d["deg_max"] == o["deg_max"]
+                fill = o["fill_aa"] if op == "AA" else o["fill_aat"]
+                assert d["pick_ours"] == oracle.pick_ours(o["deg_cv"], fill, m.nnz)
+                assert d["plan_bins"] >= 1
+        finally:
+            A.free()
+
+
+def test_phase_times_and_products(eng):
+    m = ap.stencil27(20, 0.0, 1)
+    eng.set_timing(True)
+    try:
+        A = eng.upload(m)
+        R = eng.compute(A, "AA")
+        t = eng.phase_times()
+        assert t["products"] == (9 * 20 - 10) ** 3
+        assert t["nnz_c"] == (5 * 20 - 6) ** 3
+        assert t["total_ms"] > 0 and t["sync_points"] >= 2
+        R.release()
+        A.free()
+    finally:
+        eng.set_timing(False)
+
+
+def test_budget_rejects_cleanly():
+    e = ap.Engine(0, hbm_budget_bytes=1 << 20)
+    try:
+        m = ap.stencil27(30, 0.0, 1)
+        A = e.upload(m)
+        with pytest.raises(ap.ApptError) as ex:
+            e.compute(A, "AA")
+        assert ex.value.status_name == "APPT_ERR_OOM"
+        A.free()
+    finally:
+        e.close()
+
+
+def test_full_size_stencil_c2(eng):
+    """27-point stencil 96^3 (n = 884,736): closed-form counts, sorted rows,
+    row sums, and a sample of rows bit-equal to the oracle."""
+    for drop, seed in ((0.0, 1), (0.1, 1)):
+        m = ap.stencil27(96, drop, seed)
+        say("case stencil27 g=96 drop", drop)
+        A = eng.upload(m)
+        try:
+            for op in ("AA", "AAt"):
+                R = eng.compute(A, op)
+                got = R.download()
+                R.release()
+                if drop == 0.0:
+                    assert got.nnz == 474 ** 3
+                assert rows_sorted(got.rowptr, got.col)
+                S = m.to_scipy()
+                B = S if op == "AA" else S.T.tocsr()
+                C = got.to_scipy() if False else ap.HostMatrix(m.n, got.rowptr, got.col, got.val).to_scipy()
+                ones = np.ones(m.n)
+                np.testing.assert_allclose(C @ ones, S @ (B @ ones), rtol=1e-9, atol=1e-9)
+                for r0 in (0, 400000, m.n - 3000):
+                    off, c2, v2 = oracle.rows_range(m.n, m.rowptr, m.col, m.val, r0, r0 + 3000, op)
+                    np.testing.assert_array_equal(got.rowptr[r0:r0 + 3001] - got.rowptr[r0], off)
+                    lo, hi = got.rowptr[r0], got.rowptr[r0 + 3000]
+                    np.testing.assert_array_equal(got.col[lo:hi], c2)
+                    np.testing.assert_array_equal(got.val[lo:hi], v2)
+        finally:
+            A.free()
