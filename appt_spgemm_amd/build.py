@@ -17,6 +17,8 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = PKG / "libappt_spgemm.so"
+BIN = PKG / "bin"
+CLI_TOOLS = ["appt_spgemm", "appt_dispatch"]
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = "gfx950"
 
@@ -73,6 +75,18 @@ def build(verbose: bool = False) -> Path:
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *objs,
           f"-L{ROCM / 'lib'}", "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     os.replace(tmp, LIB)
+    BIN.mkdir(exist_ok=True)
+    cli_jobs = []
+    for tool in CLI_TOOLS:
+        exe = BIN / tool
+        cli_jobs.append(["g++", *CXX_FLAGS, str(CSRC / "cli" / f"{tool}.cpp"), "-o", str(exe) + ".tmp",
+                         f"-L{PKG}", "-lappt_spgemm", f"-Wl,-rpath,$ORIGIN/..", f"-L{ROCM / 'lib'}",
+                         "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
+    with cf.ThreadPoolExecutor(len(cli_jobs)) as ex:
+        for f in [ex.submit(_run, j) for j in cli_jobs]:
+            f.result()
+    for tool in CLI_TOOLS:
+        os.replace(str(BIN / tool) + ".tmp", BIN / tool)
     if verbose:
         print(f"built {LIB}")
     return LIB

@@ -75,19 +75,14 @@ void Pool::trim() {
 // ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
-static BinParams default_sym_params() {
+static BinParams default_params() {
   BinParams P{};
   P.ncls = 3;
   P.cls_max[0] = 128; P.cls_max[1] = 512; P.cls_max[2] = 2048;
-  P.cls_bm_words[0] = 1024; P.cls_bm_words[1] = 2048; P.cls_bm_words[2] = 4096;
+  P.cls_bytes[0] = 16 * 1024; P.cls_bytes[1] = 24 * 1024; P.cls_bytes[2] = 32 * 1024;
   P.glen_max[0] = 8; P.glen_max[1] = 16; P.glen_max[2] = 32;
   P.gl_bm_words = 1 << 21;
-  return P;
-}
-static BinParams default_num_params() {
-  BinParams P = default_sym_params();
-  P.cls_max[0] = 128; P.cls_max[1] = 512; P.cls_max[2] = 1024;
-  P.cls_bm_words[0] = 1024; P.cls_bm_words[1] = 2048; P.cls_bm_words[2] = 2048;
+  P.dense_ratio = 4;
   return P;
 }
 
@@ -116,11 +111,16 @@ Engine::Engine(const appt_options* opt) : pool_(0) {
   int lds = 0;
   APPT_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
   lds_max_ = lds > 0 ? lds : 65536;
+  int ncu = 0, lds_cu = 0;
+  APPT_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device_));
+  APPT_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device_));
+  set_num_cus(ncu);
+  set_lds_per_cu(lds_cu > 0 ? lds_cu : lds_max_);
   APPT_HIP(hipHostMalloc((void**)&hs_, sizeof(HostStats), hipHostMallocDefault));
   for (auto& e : ev_) APPT_HIP(hipEventCreate(&e));
 
-  sym_params = default_sym_params();
-  num_params = default_num_params();
+  sym_params = default_params();
+  num_params = default_params();
   int64_t group_budget = 64 * 1024;
   if (opt) {
     const appt_tuning& t = opt->tuning;
@@ -131,24 +131,19 @@ Engine::Engine(const appt_options* opt) : pool_(0) {
       for (int c = 0; c < nc; c++) { sym_params.cls_max[c] = t.f_class_max[c]; num_params.cls_max[c] = t.f_class_max[c]; }
     }
     if (t.lds_budget_bytes > 0) group_budget = t.lds_budget_bytes;
+    if (t.bitmap_words_factor > 0) num_params.dense_ratio = t.bitmap_words_factor;
     for (int g = 0; g < 3; g++)
       if (t.group_for_len[g] > 0) sym_params.glen_max[g] = num_params.glen_max[g] = t.group_for_len[g];
   }
   if (group_budget > lds_max_) group_budget = lds_max_;
   set_lds_budget(group_budget);
-  // workgroup-bin limits from the device's LDS per block (minus static LDS)
-  const int64_t avail = lds_max_ - 8192;
-  int64_t T = 2;
-  while (T * 2 * 4 <= avail) T *= 2;
-  int64_t wg_sym = (3 * T - 1) / 4;
-  int64_t Tn = 2;
-  while (Tn * 2 * 12 <= avail) Tn *= 2;
-  int64_t wg_num = (2 * Tn - 1) / 3;
-  sym_params.wg_max = (int32_t)std::max<int64_t>(wg_sym, sym_params.cls_max[sym_params.ncls - 1]);
-  sym_params.wg_bm_words = (int32_t)(avail / 4);
-  num_params.wg_max = (int32_t)std::max<int64_t>(wg_num, num_params.cls_max[num_params.ncls - 1]);
-  num_params.wg_bm_words = (int32_t)std::max<int64_t>((avail - wg_num * 12) / 8, 0);
-  if (opt && opt->tuning.wg_f_max > 0) sym_params.wg_max = std::min(sym_params.wg_max, opt->tuning.wg_f_max);
+  // workgroup-per-row bins may use the device's LDS per block minus static LDS
+  const int32_t wg = (int32_t)(lds_max_ - 8192);
+  sym_params.wg_bytes = num_params.wg_bytes = wg;
+  if (opt && opt->tuning.wg_f_max > 0) {
+    // cap the workgroup bins (rows above go to the global path)
+    sym_params.wg_bytes = std::min<int32_t>(wg, opt->tuning.wg_f_max);
+  }
 }
 
 Engine::~Engine() {
@@ -157,7 +152,7 @@ Engine::~Engine() {
   Scratch* all[] = {&ws_.bptr, &ws_.bcol, &ws_.bval, &ws_.cursor, &ws_.bmin, &ws_.bmax, &ws_.rowf,
                     &ws_.rowlo, &ws_.rowhi, &ws_.symbin, &ws_.numbin, &ws_.rowcnt, &ws_.rowoff,
                     &ws_.symrows, &ws_.numrows, &ws_.scan_tmp, &ws_.stats, &ws_.gsym, &ws_.gnum,
-                    &ws_.low, &ws_.fullcnt, &ws_.uoff, &ws_.ucol, &ws_.uval, &ws_.coff};
+                    &ws_.low, &ws_.fullcnt, &ws_.uoff, &ws_.ucol, &ws_.uval, &ws_.coff, &ws_.rowwords, &ws_.wlist};
   for (Scratch* s : all) free_scratch(*s);
   pool_.trim();
   for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
@@ -218,8 +213,7 @@ void Engine::prepass_stage(const appt_csr_view& A, appt_op op, const int32_t*& b
     bp = tp; bc = tc; bv = tv;
   }
   record(1);
-  int32_t* bmin = as<int32_t>(ensure(ws_.bmin, sizeof(int32_t) * (size_t)std::max(n, 1)));
-  int32_t* bmax = as<int32_t>(ensure(ws_.bmax, sizeof(int32_t) * (size_t)std::max(n, 1)));
+  int4* bmeta = as<int4>(ensure(ws_.bmin, sizeof(int4) * (size_t)std::max(n, 1)));
   int64_t* rowf = as<int64_t>(ensure(ws_.rowf, sizeof(int64_t) * (size_t)std::max(n, 1)));
   int32_t* rowlo = as<int32_t>(ensure(ws_.rowlo, sizeof(int32_t) * (size_t)std::max(n, 1)));
   int32_t* rowhi = as<int32_t>(ensure(ws_.rowhi, sizeof(int32_t) * (size_t)std::max(n, 1)));
@@ -230,9 +224,8 @@ void Engine::prepass_stage(const appt_csr_view& A, appt_op op, const int32_t*& b
   unsigned long long* dprod = reinterpret_cast<unsigned long long*>(dstats + stat_words * 2);
   launch_fill_i32(dstats, (int64_t)(stat_words * 2 + 4 + kNumBins), 0, s);
   const int upper = op == APPT_OP_AATS ? 1 : 0;
-  launch_rowstats(n, bp, bc, bmin, bmax, s);
-  launch_prepass(n, A.rowptr, A.col, bp, bmin, bmax, upper, sym_params, rowf, rowlo, rowhi, symbin, dstats, dprod,
-                 s);
+  launch_rowstats(n, bp, bc, bmeta, s);
+  launch_prepass(n, A.rowptr, A.col, bmeta, upper, sym_params, rowf, rowlo, rowhi, symbin, dstats, dprod, s);
   APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, sizeof(int32_t) * stat_words, hipMemcpyDeviceToHost, s));
   APPT_HIP(hipMemcpyAsync(&hs_->products, dprod, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   APPT_HIP(hipStreamSynchronize(s));
@@ -265,6 +258,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   syncs++;
   const int upper = op == APPT_OP_AATS ? 1 : 0;
   int32_t* rowcnt = as<int32_t>(ensure(ws_.rowcnt, sizeof(int32_t) * (size_t)std::max(n, 1)));
+  int32_t* rowwords = as<int32_t>(ensure(ws_.rowwords, sizeof(int32_t) * (size_t)std::max(n, 1)));
   int64_t* rowoff = as<int64_t>(ensure(ws_.rowoff, sizeof(int64_t) * ((size_t)n + 1)));
   int32_t* symrows = as<int32_t>(ensure(ws_.symrows, sizeof(int32_t) * (size_t)std::max(n, 1)));
   int32_t* numrows = as<int32_t>(ensure(ws_.numrows, sizeof(int32_t) * (size_t)std::max(n, 1)));
@@ -273,6 +267,8 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   const int32_t* rowlo = as<int32_t>(ws_.rowlo.p);
   const int32_t* rowhi = as<int32_t>(ws_.rowhi.p);
   const uint8_t* symbin = as<uint8_t>(ws_.symbin.p);
+  const int4* bmeta = as<int4>(ws_.bmin.p);
+  uint2* wlist = as<uint2>(ensure(ws_.wlist, sizeof(uint2) * kWordListCap * (size_t)std::max(n, 1)));
   uint8_t* numbin = as<uint8_t>(ensure(ws_.numbin, (size_t)std::max(n, 1)));
   const size_t stat_words = (size_t)kNumBins * kStatFields;
   int32_t* dstats = as<int32_t>(ws_.stats.p);
@@ -295,6 +291,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   APPT_HIP(hipMemcpyAsync(dcursor, hs_->cursor, sizeof(int32_t) * kNumBins, hipMemcpyHostToDevice, s));
   launch_bin_scatter(n, symbin, dcursor, symrows, s);
   launch_fill_i32(rowcnt, n, 0, s);
+  launch_fill_i32(rowwords, n, 0, s);
   record(2);
 
   // ---- symbolic -------------------------------------------------------------
@@ -305,7 +302,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     nbins_used++;
     const int32_t mmax = hs_->sym[b * kStatFields + 1];
     const int32_t wmax = hs_->sym[b * kStatFields + 2];
-    RowArgs ra{symrows + symoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, upper};
+    RowArgs ra{symrows + symoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, upper, A.nnz, bmeta, wlist};
     if (b == kBinGlBitmap) {
       const int64_t slot_bytes = (int64_t)wmax * 4;
       int64_t slots = std::min<int64_t>(cnt, gslots_);
@@ -315,9 +312,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     } else {
       const int64_t rb = symbolic_row_lds_bytes(b, mmax, wmax);
       if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "symbolic row exceeds LDS");
-      const bool bm = (b == kBinWgBitmap) || (b < kBinWgBitmap && bin_is_bitmap(b));
-      const int32_t words = bm ? wmax : (int32_t)sym_slots(mmax);
-      launch_symbolic_bin(b, ra, words, rowcnt, nullptr, 0, s);
+      launch_symbolic_bin(b, ra, wmax, rowcnt, nullptr, 0, s);
     }
   }
   record(3);
@@ -325,7 +320,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   // ---- row offsets, numeric binning ----------------------------------------
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)n + 1)));
   launch_scan(rowcnt, n, rowoff, rp32, tmp, s);
-  launch_numbin(n, A.rowptr, rowf, rowcnt, rowlo, rowhi, num_params, numbin, dnum, s);
+  launch_numbin(n, A.rowptr, rowf, rowcnt, rowwords, rowlo, rowhi, num_params, numbin, dnum, s);
   APPT_HIP(hipMemcpyAsync(hs_->num, dnum, sizeof(int32_t) * stat_words, hipMemcpyDeviceToHost, s));
   APPT_HIP(hipMemcpyAsync(&hs_->total, rowoff + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   APPT_HIP(hipStreamSynchronize(s));
@@ -366,7 +361,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     if (cnt == 0) continue;
     const int32_t cmax = hs_->num[b * kStatFields + 1];
     const int32_t wmax = hs_->num[b * kStatFields + 2];
-    RowArgs ra{numrows + numoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, upper};
+    RowArgs ra{numrows + numoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, upper, A.nnz, bmeta, wlist};
     if (b == kBinGlBitmap) {
       const int64_t slot_bytes = (int64_t)wmax * 8;
       int64_t slots = std::min<int64_t>(cnt, gslots_);

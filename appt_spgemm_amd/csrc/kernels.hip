@@ -14,6 +14,8 @@
 #include "kernels.hpp"
 #include "device_common.hpp"
 
+#include <algorithm>
+
 namespace appt {
 
 // ---------------------------------------------------------------------------
@@ -55,10 +57,12 @@ void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s) 
 }
 
 // ---------------------------------------------------------------------------
-// per-row min / max column (span of a B row); 16 lanes per row
+// B-row metadata {start, end, min column, max column}; 16 lanes per row.
+// One 16-byte gather per A entry then serves the pre-pass and both product
+// passes.
 // ---------------------------------------------------------------------------
 __global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                           int32_t* __restrict__ rmin, int32_t* __restrict__ rmax) {
+                           int4* __restrict__ meta) {
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
@@ -72,15 +76,14 @@ __global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int
     }
     mn = gmin<G>(mn);
     mx = gmax<G>(mx);
-    if (lane == 0) { rmin[i] = mn; rmax[i] = mx; }
+    if (lane == 0) meta[i] = make_int4(s, e, mn, mx);
   }
 }
-void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* rmin, int32_t* rmax,
-                     hipStream_t s) {
+void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_rowstats, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, rmin, rmax);
+  hipLaunchKernelGGL(k_rowstats, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, meta);
 }
 
 // ---------------------------------------------------------------------------
@@ -205,20 +208,90 @@ void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, i
 // ---------------------------------------------------------------------------
 // binning
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int bin_of(const BinParams& P, int64_t metric, int64_t words, int64_t L) {
-  if (metric <= 0 || words <= 0) return 0;
-  const int gidx = L <= P.glen_max[0] ? 0 : L <= P.glen_max[1] ? 1 : L <= P.glen_max[2] ? 2 : 3;
+__device__ __forceinline__ int64_t span_words(int lo, int hi) {
+  return hi < lo ? 0 : (int64_t)((hi - lo) >> 5) + 1;
+}
+
+constexpr int kTouchCap = 64;   // touched-word list length per row
+// rowwords[i] holds the row's distinct-word count; kNoWordList marks rows whose
+// symbolic pass left no sorted word list behind
+constexpr int32_t kNoWordList = 1 << 30, kWordsMask = kNoWordList - 1;
+
+template <typename T, bool W64>
+__device__ __forceinline__ T ld_idx(const T* base, uint32_t idx) {
+  if constexpr (W64) return base[idx];
+  else return *(const T*)((const char*)base + (size_t)(idx * (uint32_t)sizeof(T)));
+}
+
+__host__ __device__ __forceinline__ int64_t align16(int64_t x) { return (x + 15) & ~(int64_t)15; }
+
+// symbolic window LDS per group: meta (G int2) | bits (W + G u32) | touched (TL u32) | ctl (4 int)
+__host__ __device__ __forceinline__ int64_t sym_window_bytes(int G, int64_t W) {
+  return align16(G * 8) + align16((W + G) * 4) + align16(kTouchCap * 4) + 16;
+}
+// numeric window LDS per group: meta (G int2 + G f64) | bits (W + G u32) | pre (W u16) |
+// vals (Cm + G f64) | cols (Cm i32)
+__host__ __device__ __forceinline__ int64_t num_window_bytes(int G, int64_t W, int64_t Cm) {
+  return align16(G * 16) + align16((W + G) * 4) + align16(W * 2) + align16((Cm + G) * 8) + align16(Cm * 4);
+}
+
+__host__ __device__ __forceinline__ int64_t np2_hd(int64_t x) {
+  if (x <= 2) return 2;
+  return (int64_t)1 << (64 - __builtin_clzll((unsigned long long)(x - 1)));
+}
+// Lanes per row from the mean B-row length f / nA (compared by multiplication:
+// f <= g * nA  <=>  ceil(f / nA) <= g).
+__device__ __forceinline__ int group_index(const BinParams& P, int64_t f, int64_t nA) {
+  return f <= (int64_t)P.glen_max[0] * nA ? 0 : f <= (int64_t)P.glen_max[1] * nA ? 1
+       : f <= (int64_t)P.glen_max[2] * nA ? 2 : 3;
+}
+
+// Symbolic bin from products f, span words and mean B-row length L.  *stat
+// receives the LDS sizing statistic: span words (dense) or key bound (hash).
+__device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t words, int64_t nA, int64_t* stat) {
+  *stat = 0;
+  if (f <= 0 || words <= 0) return 0;
+  const int gidx = group_index(P, f, nA);
+  const int64_t keys = f < words ? f : words;
+  const int64_t hash_bytes = np2_hd((keys * 4) / 3 + 1) * 8;
   for (int c = 0; c < P.ncls; c++)
-    if (metric <= P.cls_max[c]) {
-      const int mode = words <= P.cls_bm_words[c] ? 0 : 1;
-      return 1 + (c * 2 + mode) * 4 + gidx;
+    if (f <= P.cls_max[c]) {
+      if (sym_window_bytes(8 << gidx, words) <= P.cls_bytes[c]) { *stat = words; return 1 + (c * 2 + 0) * 4 + gidx; }
+      if (hash_bytes <= P.cls_bytes[c]) { *stat = keys; return 1 + (c * 2 + 1) * 4 + gidx; }
+      break;
     }
-  if (metric <= P.wg_max) return words <= P.wg_bm_words ? kBinWgBitmap : kBinWgHash;
+  if (words * 4 <= P.wg_bytes) { *stat = words; return kBinWgBitmap; }
+  if (hash_bytes <= P.wg_bytes) { *stat = keys; return kBinWgHash; }
+  *stat = words;
   return words <= P.gl_bm_words ? kBinGlBitmap : kBinGlHash;
 }
 
-__device__ __forceinline__ int64_t span_words(int lo, int hi) {
-  return hi < lo ? 0 : (int64_t)((hi - lo) >> 5) + 1;
+__host__ __device__ __forceinline__ int64_t whash_bytes_hd(int64_t w, int64_t c) {
+  const int64_t T = np2_hd((w * 3) / 2 + 1);
+  const int64_t v = c > T ? c : T;
+  return ((T * 12 + v * 8 + c * 4) + 15) & ~(int64_t)15;
+}
+
+// Numeric bin from row nnz c, span words, distinct words w and L.
+__device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t words, int64_t w, int64_t f,
+                                          int64_t nA, int64_t* stat) {
+  *stat = 0;
+  if (c <= 0 || words <= 0) return 0;
+  const int gidx = group_index(P, f, nA);
+  const int64_t lim = (int64_t)P.dense_ratio * w;
+  const bool dense_ok = words <= (lim > 64 ? lim : 64);
+  const int64_t dense_bytes = ((words * 8 + c * 12) + 15) & ~(int64_t)15;
+  for (int cl = 0; cl < P.ncls; cl++)
+    if (c <= P.cls_max[cl]) {
+      if (num_window_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) { *stat = words; return 1 + (cl * 2 + 0) * 4 + gidx; }
+      if (whash_bytes_hd(w, c) <= P.cls_bytes[cl]) { *stat = w; return 1 + (cl * 2 + 1) * 4 + gidx; }
+      break;
+    }
+  // workgroup bins: bound each term separately so bin maxima still fit
+  if (dense_ok && words * 8 <= P.wg_bytes / 2 && c * 12 <= P.wg_bytes / 2) { *stat = words; return kBinWgBitmap; }
+  if (whash_bytes_hd(c, c) <= P.wg_bytes) { *stat = w; return kBinWgHash; }
+  *stat = words;
+  return words <= P.gl_bm_words ? kBinGlBitmap : kBinGlHash;
 }
 
 // Block-local bin statistics flushed with one global atomic per bin.
@@ -229,10 +302,31 @@ __device__ __forceinline__ void stats_init(BinStatLds& st) {
   for (int t = threadIdx.x; t < kNumBins * kStatFields; t += blockDim.x) st.v[t] = 0;
   __syncthreads();
 }
-__device__ __forceinline__ void stats_add(BinStatLds& st, int bin, int64_t metric, int64_t words) {
-  atomicAdd(&st.v[bin * kStatFields + 0], 1);
-  atomicMax(&st.v[bin * kStatFields + 1], (int32_t)(metric > INT32_MAX ? INT32_MAX : metric));
-  atomicMax(&st.v[bin * kStatFields + 2], (int32_t)(words > INT32_MAX ? INT32_MAX : words));
+// Wave-aggregated update: every active lane passes its bin (or -1); one LDS
+// atomic per distinct bin in the wave.
+__device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) { const int32_t o = __shfl_xor(v, m, 64); v = o > v ? o : v; }
+  return v;
+}
+__device__ __forceinline__ void stats_add_wave(BinStatLds& st, int bin, int64_t metric, int64_t words) {
+  const int lane = threadIdx.x % kWave;
+  const int32_t m32 = (int32_t)(metric > INT32_MAX ? INT32_MAX : metric);
+  const int32_t w32 = (int32_t)(words > INT32_MAX ? INT32_MAX : words);
+  uint64_t todo = __ballot(bin >= 0);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const int lb = __shfl(bin, leader, kWave);
+    const uint64_t same = __ballot(bin == lb);
+    const int32_t mm = wave_max_i32(bin == lb ? m32 : 0);
+    const int32_t ww = wave_max_i32(bin == lb ? w32 : 0);
+    if (lane == leader) {
+      atomicAdd(&st.v[lb * kStatFields + 0], (int32_t)__popcll(same));
+      atomicMax(&st.v[lb * kStatFields + 1], mm);
+      atomicMax(&st.v[lb * kStatFields + 2], ww);
+    }
+    todo &= ~same;
+  }
 }
 __device__ __forceinline__ void stats_flush(BinStatLds& st, int32_t* g) {
   __syncthreads();
@@ -249,8 +343,7 @@ __device__ __forceinline__ void stats_flush(BinStatLds& st, int32_t* g) {
 // Structural pre-pass: products f_i (k_flops of the reference), output span
 // [lo, hi] from the B rows' min/max columns, symbolic bin, bin statistics.
 __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
-                          const int32_t* __restrict__ bp, const int32_t* __restrict__ bmin,
-                          const int32_t* __restrict__ bmax, int upper, BinParams P,
+                          const int4* __restrict__ bmeta, int upper, BinParams P,
                           int64_t* __restrict__ rowf, int32_t* __restrict__ rowlo, int32_t* __restrict__ rowhi,
                           uint8_t* __restrict__ bin, int32_t* __restrict__ gstats,
                           unsigned long long* __restrict__ total_products) {
@@ -261,36 +354,49 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
   unsigned long long myprod = 0;
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb) {
-    const int s = ap[i], e = ap[i + 1];
+  // every wave runs the same number of iterations so the wave-wide statistics
+  // update below is executed by converged waves
+  const int64_t stride = (int64_t)gridDim.x * gpb;
+  const int64_t i0 = (int64_t)blockIdx.x * gpb + threadIdx.x / G;
+  const int64_t wave_first = (int64_t)blockIdx.x * gpb + (threadIdx.x / kWave) * (kWave / G);
+  for (int64_t wi = wave_first; wi < n; wi += stride) {
+    const int64_t i = i0 + (wi - wave_first);
+    const bool live = i < n;
     int64_t f = 0;
-    int lo = INT32_MAX, hi = -1;
-    for (int p = s + lane; p < e; p += G) {
-      const int k = ac[p];
-      const int len = bp[k + 1] - bp[k];
-      f += len;
-      if (len > 0) {
-        const int a = bmin[k], b = bmax[k];
-        lo = a < lo ? a : lo;
-        hi = b > hi ? b : hi;
+    int lo = INT32_MAX, hi = -1, s = 0, e = 0;
+    if (live) {
+      s = ap[i];
+      e = ap[i + 1];
+      for (int p = s + lane; p < e; p += G) {
+        const int4 bm = bmeta[ac[p]];
+        const int len = bm.y - bm.x;
+        f += len;
+        if (len > 0) {
+          lo = bm.z < lo ? bm.z : lo;
+          hi = bm.w > hi ? bm.w : hi;
+        }
       }
     }
     f = gsum<G, int64_t>(f, nullptr);
     lo = gmin<G>(lo);
     hi = gmax<G>(hi);
-    if (lane == 0) {
-      if (upper && lo < (int)i) lo = (int)i;
-      const int64_t words = span_words(lo, hi);
-      const int64_t nA = e - s;
-      const int64_t L = nA > 0 ? (f + nA - 1) / nA : 0;
-      const int b = bin_of(P, f, words, L);
+    if (upper && lo < (int)i) lo = (int)i;
+    // bitmap windows start on a 32-column boundary, so a window word and a
+    // word-hash key (j >> 5) cover the same columns and distinct-word counts
+    // agree between accumulator modes
+    if (lo <= hi) lo &= ~31;
+    const int64_t words = span_words(lo, hi);
+    int64_t stat;
+    const int b = sym_bin_of(P, f, words, (int64_t)(e - s), &stat);
+    const bool leader = live && lane == 0;
+    if (leader) {
       rowf[i] = f;
       rowlo[i] = lo;
       rowhi[i] = hi;
       bin[i] = (uint8_t)b;
-      stats_add(st, b, f, words);
       myprod += (unsigned long long)f;
     }
+    stats_add_wave(st, leader ? b : -1, f, stat);
   }
   // block reduction of the product count
 #pragma unroll
@@ -303,73 +409,97 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     if (t) atomicAdd(total_products, t);
   }
 }
-void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int32_t* bp, const int32_t* bmin,
-                    const int32_t* bmax, int upper, const BinParams& P, int64_t* rowf, int32_t* rowlo,
-                    int32_t* rowhi, uint8_t* bin, int32_t* stats, unsigned long long* total_products,
-                    hipStream_t s) {
+void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4* bmeta, int upper,
+                    const BinParams& P, int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin,
+                    int32_t* stats, unsigned long long* total_products, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_prepass, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, bp, bmin, bmax, upper, P,
-                     rowf, rowlo, rowhi, bin, stats, total_products);
+  hipLaunchKernelGGL(k_prepass, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, bmeta, upper, P, rowf, rowlo,
+                     rowhi, bin, stats, total_products);
 }
 
 // Numeric bin of each row from its exact output count.
 __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_t* __restrict__ rowf,
-                         const int32_t* __restrict__ rowcnt, const int32_t* __restrict__ rowlo,
-                         const int32_t* __restrict__ rowhi, BinParams P, uint8_t* __restrict__ bin,
-                         int32_t* __restrict__ gstats) {
+                         const int32_t* __restrict__ rowcnt, const int32_t* __restrict__ rowwords,
+                         const int32_t* __restrict__ rowlo, const int32_t* __restrict__ rowhi, BinParams P,
+                         uint8_t* __restrict__ bin, int32_t* __restrict__ gstats) {
   __shared__ BinStatLds st;
   stats_init(st);
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t c = rowcnt[i];
-    const int64_t words = span_words(rowlo[i], rowhi[i]);
-    const int64_t nA = ap[i + 1] - ap[i];
-    const int64_t f = rowf[i];
-    const int64_t L = nA > 0 ? (f + nA - 1) / nA : 0;
-    const int b = bin_of(P, c, words, L);
-    bin[i] = (uint8_t)b;
-    stats_add(st, b, c, words);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t wave_first = (int64_t)blockIdx.x * blockDim.x + (threadIdx.x / kWave) * kWave;
+  for (int64_t wi = wave_first; wi < n; wi += stride) {
+    const int64_t i = wi + threadIdx.x % kWave;
+    int b = -1;
+    int64_t c = 0, stat = 0;
+    if (i < n) {
+      c = rowcnt[i];
+      const int64_t words = span_words(rowlo[i], rowhi[i]);
+      const int64_t nA = ap[i + 1] - ap[i];
+      b = num_bin_of(P, c, words, rowwords[i] & kWordsMask, rowf[i], nA, &stat);
+      bin[i] = (uint8_t)b;
+    }
+    stats_add_wave(st, b, c, stat);
   }
   stats_flush(st, gstats);
 }
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
-                   const int32_t* rowlo, const int32_t* rowhi, const BinParams& P, uint8_t* bin,
-                   int32_t* stats, hipStream_t s) {
+                   const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
+                   uint8_t* bin, int32_t* stats, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_numbin, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, rowf, rowcnt, rowlo, rowhi, P,
-                     bin, stats);
+  hipLaunchKernelGGL(k_numbin, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, rowf, rowcnt, rowwords, rowlo,
+                     rowhi, P, bin, stats);
 }
 
-// Partition rows by bin: one global atomic per (block, bin); rows keep their
-// relative order within a block tile only approximately (LDS atomics), which
-// affects scheduling, never results.
-__global__ void k_bin_scatter(int32_t n, const uint8_t* __restrict__ bin, int32_t* __restrict__ cursor,
-                              int32_t* __restrict__ rows) {
-  constexpr int kItems = 4;
-  __shared__ int32_t cnt[kNumBins];
-  __shared__ int32_t base[kNumBins];
-  for (int64_t tile = (int64_t)blockIdx.x * blockDim.x * kItems; tile < n;
-       tile += (int64_t)gridDim.x * blockDim.x * kItems) {
-    for (int t = threadIdx.x; t < kNumBins; t += blockDim.x) cnt[t] = 0;
+// Partition rows by bin, keeping ascending row order inside each block tile
+// (neighbouring rows share B rows, so order matters for cache reuse, never
+// for results).  A tile is 1024 rows: wave w owns rows [w*256, w*256+256) of
+// it in four chunks of 64; ranks come from wave ballots per distinct bin; one
+// global atomic per (tile, bin) reserves the tile's range.
+__global__ void __launch_bounds__(256) k_bin_scatter(int32_t n, const uint8_t* __restrict__ bin,
+                                                     int32_t* __restrict__ cursor, int32_t* __restrict__ rows) {
+  constexpr int kItems = 4, kWaves = 4;
+  __shared__ int32_t cnt[kWaves][kNumBins];
+  __shared__ int32_t woff[kWaves][kNumBins];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  for (int64_t tile = (int64_t)blockIdx.x * 1024; tile < n; tile += (int64_t)gridDim.x * 1024) {
+    for (int t = lane; t < kNumBins; t += kWave) cnt[w][t] = 0;
     __syncthreads();
     int myb[kItems], myr[kItems];
 #pragma unroll
     for (int t = 0; t < kItems; t++) {
-      const int64_t i = tile + (int64_t)t * blockDim.x + threadIdx.x;
-      myb[t] = i < n ? (int)bin[i] : -1;
-      myr[t] = (myb[t] > 0) ? atomicAdd(&cnt[myb[t]], 1) : 0;
+      const int64_t i = tile + (int64_t)w * 256 + t * kWave + lane;
+      const int b = i < n ? (int)bin[i] : 0;
+      myb[t] = b;
+      myr[t] = 0;
+      uint64_t todo = __ballot(b > 0);
+      while (todo) {
+        const int leader = __builtin_ctzll(todo);
+        const int lb = __shfl(b, leader, kWave);
+        const uint64_t same = __ballot(b == lb) & todo;
+        if (b == lb && b > 0) myr[t] = cnt[w][lb] + __popcll(same & ((1ull << lane) - 1ull));
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (lane == leader) cnt[w][lb] += __popcll(same);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        todo &= ~same;
+      }
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < kNumBins; t += blockDim.x)
-      base[t] = (t > 0 && cnt[t]) ? atomicAdd(&cursor[t], cnt[t]) : 0;
+    for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) {
+      int tot = 0;
+      for (int v = 0; v < kWaves; v++) tot += cnt[v][b];
+      int base = (b > 0 && tot) ? atomicAdd(&cursor[b], tot) : 0;
+      for (int v = 0; v < kWaves; v++) { woff[v][b] = base; base += cnt[v][b]; }
+    }
     __syncthreads();
 #pragma unroll
     for (int t = 0; t < kItems; t++) {
-      const int64_t i = tile + (int64_t)t * blockDim.x + threadIdx.x;
-      if (myb[t] > 0) rows[base[myb[t]] + myr[t]] = (int32_t)i;
+      const int64_t i = tile + (int64_t)w * 256 + t * kWave + lane;
+      if (myb[t] > 0) rows[woff[w][myb[t]] + myr[t]] = (int32_t)i;
     }
     __syncthreads();
   }
@@ -377,7 +507,7 @@ __global__ void k_bin_scatter(int32_t n, const uint8_t* __restrict__ bin, int32_
 void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n + 1023) / 1024;
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(256), 0, s, n, bin, cursor, rows);
 }
 
@@ -397,7 +527,27 @@ struct Stage {
   double a[256];
 };
 
-template <int G, bool VALS, typename Fn>
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Renumber them so that each XCD walks one contiguous eighth of the
+// row list: neighbouring rows share B rows, and those then hit the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int x = b % kXcd, k = b / kXcd;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
+// fn(j, prod) is called once per product with j = B column and prod = a*b
+// (rounded product; 0 when VALS is false).
+//
+// G <= 64: the row's A entries are fetched G at a time (one per lane) and
+// broadcast with lane shuffles.  U consecutive entries are handled per step:
+// their first G B-row entries are loaded together (U independent gathers in
+// flight per lane), then applied strictly in entry order, each entry's
+// remainder (B rows longer than G) before the next entry, so every output
+// column still receives its contributions in A-row storage order.
+// G > 64: the whole workgroup owns the row; A-entry metadata is staged in LDS.
+template <int G, bool VALS, int U = 8, typename Fn>
 __device__ __forceinline__ void for_each_product(int i, int lane, const RowArgs& A, Stage* stage, Fn&& fn) {
   const int s = A.ap[i], e = A.ap[i + 1];
   for (int base = s; base < e; base += G) {
@@ -412,10 +562,25 @@ __device__ __forceinline__ void for_each_product(int i, int lane, const RowArgs&
     }
     const int m = (e - base) < G ? (e - base) : G;
     if constexpr (G <= kWave) {
-      for (int t = 0; t < m; t++) {
-        const int tbs = __shfl(bs, t, G), tbe = __shfl(be, t, G);
-        const double ta = VALS ? __shfl(a, t, G) : 0.0;
-        for (int q = tbs + lane; q < tbe; q += G) fn(A.bc[q], q, ta);
+      for (int t0 = 0; t0 < m; t0 += U) {
+        int jj[U], qb[U], qe[U];
+        double pv[U], aa[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          const int t = t0 + u;
+          const int tt = t < m ? t : 0;
+          qb[u] = __shfl(bs, tt, G) + lane;
+          qe[u] = t < m ? __shfl(be, tt, G) : 0;
+          aa[u] = VALS ? __shfl(a, tt, G) : 0.0;
+          const bool ok = qb[u] < qe[u];
+          jj[u] = ok ? A.bc[qb[u]] : -1;
+          pv[u] = (VALS && ok) ? aa[u] * A.bv[qb[u]] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+          if (jj[u] >= 0) fn(jj[u], pv[u]);
+          for (int q = qb[u] + G; q < qe[u]; q += G) fn(A.bc[q], VALS ? aa[u] * A.bv[q] : 0.0);
+        }
       }
     } else {
       __syncthreads();
@@ -426,63 +591,96 @@ __device__ __forceinline__ void for_each_product(int i, int lane, const RowArgs&
       for (int t = 0; t < m; t++) {
         const int tbs = stage->bs[t], tbe = stage->be[t];
         const double ta = VALS ? stage->a[t] : 0.0;
-        for (int q = tbs + lane; q < tbe; q += G) fn(A.bc[q], q, ta);
+        for (int q = tbs + lane; q < tbe; q += G) fn(A.bc[q], VALS ? ta * A.bv[q] : 0.0);
       }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// symbolic: distinct output columns per row
+// Accumulator modes of a row (per bin):
+//   dense   a bitmap over the row's whole output span [lo, hi] (1 bit per
+//           column); output positions are prefix popcounts of the words.
+//   whash   "word hash": an LDS open-addressing table keyed by the 32-column
+//           word index j>>5 with a 32-bit column mask per key.  Rows whose
+//           span is wide but whose columns cluster (stencils) need a handful
+//           of keys; scattered rows degrade to one key per column.  The
+//           numeric pass sorts only the keys, then uses prefix popcounts.
 // ---------------------------------------------------------------------------
-template <int G, bool BITMAP>
-__global__ void __launch_bounds__(1024) k_symbolic_lds(RowArgs A, int32_t row_words, int32_t* __restrict__ rowcnt) {
+
+// Find or claim the slot of word key wk.  Returns the slot; *claimed is set
+// when this call inserted the key.
+__device__ __forceinline__ uint32_t whash_insert(int32_t* keys, int logT, uint32_t mask, int32_t wk, bool* claimed) {
+  uint32_t h = hash_slot(wk, logT);
+  while (true) {
+    const int cur = __atomic_load_n(&keys[h], __ATOMIC_RELAXED);
+    if (cur == wk) { *claimed = false; return h; }
+    if (cur == -1) {
+      const int old = atomicCAS(&keys[h], -1, wk);
+      if (old == -1) { *claimed = true; return h; }
+      if (old == wk) { *claimed = false; return h; }
+    }
+    h = (h + 1u) & mask;
+  }
+}
+// Slot of a key known to be present.
+__device__ __forceinline__ uint32_t whash_find(const int32_t* keys, int logT, uint32_t mask, int32_t wk) {
+  uint32_t h = hash_slot(wk, logT);
+  while (keys[h] != wk) h = (h + 1u) & mask;
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// symbolic: distinct output columns (rowcnt) and distinct 32-column words
+// (rowwords) per row
+// ---------------------------------------------------------------------------
+template <int G, bool DENSE>
+__global__ void __launch_bounds__(256) k_symbolic_lds(RowArgs A, int32_t row_words, int32_t* __restrict__ rowcnt) {
   extern __shared__ uint32_t lds[];
   __shared__ Stage stage_buf[G > kWave ? 1 : 0 + 1];
   __shared__ int32_t red[16];
   const int R = blockDim.x / G;
   const int gi = threadIdx.x / G, lane = threadIdx.x % G;
-  const int r = blockIdx.x * R + gi;
+  const int r = xcd_block(blockIdx.x, gridDim.x) * R + gi;
   if (G <= kWave && r >= A.nrows) return;
   const int i = A.rows[r];
   uint32_t* tab = lds + (size_t)gi * row_words;
-  int cnt = 0;
-  if constexpr (BITMAP) {
-    const int lo = A.rowlo[i], hi = A.rowhi[i];
+  int cnt = 0, nw = 0;
+  const int lo = A.rowlo[i], hi = A.rowhi[i];
+  if constexpr (DENSE) {
     const int words = (int)span_words(lo, hi);
     for (int t = lane; t < words; t += G) tab[t] = 0u;
     gsync<G>();
-    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, int, double) {
+    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
       if (A.upper && j < i) return;
       const uint32_t b = (uint32_t)(j - lo);
       const uint32_t bit = 1u << (b & 31u);
       const uint32_t old = atomicOr(&tab[b >> 5], bit);
       cnt += (old & bit) ? 0 : 1;
+      nw += old == 0u ? 1 : 0;
     });
   } else {
-    const int64_t T = np2_i64((A.rowf[i] * 4) / 3 + 1);
+    const int64_t keys_bound = A.rowf[i] < span_words(lo, hi) ? A.rowf[i] : span_words(lo, hi);
+    const int64_t T = np2_i64((keys_bound * 4) / 3 + 1);
     const int logT = __builtin_ctzll((unsigned long long)T);
     const uint32_t mask = (uint32_t)T - 1u;
     int32_t* keys = (int32_t*)tab;
-    for (int t = lane; t < (int)T; t += G) keys[t] = -1;
+    uint32_t* masks = tab + T;
+    for (int t = lane; t < (int)T; t += G) { keys[t] = -1; masks[t] = 0u; }
     gsync<G>();
-    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, int, double) {
+    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
       if (A.upper && j < i) return;
-      uint32_t h = hash_slot(j, logT);
-      while (true) {
-        const int cur = __atomic_load_n(&keys[h], __ATOMIC_RELAXED);
-        if (cur == j) break;
-        if (cur == -1) {
-          const int old = atomicCAS(&keys[h], -1, j);
-          if (old == -1) { cnt++; break; }
-          if (old == j) break;
-        }
-        h = (h + 1u) & mask;
-      }
+      bool claimed;
+      const uint32_t h = whash_insert(keys, logT, mask, j >> 5, &claimed);
+      nw += claimed ? 1 : 0;
+      const uint32_t bit = 1u << (j & 31);
+      const uint32_t old = atomicOr(&masks[h], bit);
+      cnt += (old & bit) ? 0 : 1;
     });
   }
   cnt = gsum<G, int32_t>(cnt, red);
-  if (lane == 0) rowcnt[i] = cnt;
+  nw = gsum<G, int32_t>(nw, red);
+  if (lane == 0) { rowcnt[i] = cnt; A.rowwords[i] = nw | kNoWordList; }
 }
 
 // Heavy rows: one workgroup per row, bitmap window in a global scratch slot.
@@ -501,16 +699,18 @@ __global__ void __launch_bounds__(256) k_symbolic_global(RowArgs A, int32_t slot
     for (int t = lane; t < words; t += G) tab[t] = 0u;
     __threadfence();
     __syncthreads();
-    int cnt = 0;
-    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, int, double) {
+    int cnt = 0, nw = 0;
+    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
       if (A.upper && j < i) return;
       const uint32_t b = (uint32_t)(j - lo);
       const uint32_t bit = 1u << (b & 31u);
       const uint32_t old = atomicOr(&tab[b >> 5], bit);
       cnt += (old & bit) ? 0 : 1;
+      nw += old == 0u ? 1 : 0;
     });
     cnt = gsum<G, int32_t>(cnt, red);
-    if (lane == 0) rowcnt[i] = cnt;
+    nw = gsum<G, int32_t>(nw, red);
+    if (lane == 0) { rowcnt[i] = cnt; A.rowwords[i] = nw | kNoWordList; }
   }
 }
 
@@ -518,10 +718,10 @@ __global__ void __launch_bounds__(256) k_symbolic_global(RowArgs A, int32_t slot
 // numeric
 // ---------------------------------------------------------------------------
 
-// In-LDS bitonic sort (all comparators ascending; positions >= len act as
-// +infinity and are never touched).
-template <int G>
-__device__ __forceinline__ void lds_sort_pairs(int32_t* keys, double* vals, int len, int lane) {
+// In-LDS bitonic sort by key (all comparators ascending; positions >= len act
+// as +infinity and are never touched).
+template <int G, typename V>
+__device__ __forceinline__ void lds_sort_pairs(int32_t* keys, V* vals, int len, int lane) {
   int N = 1;
   while (N < len) N <<= 1;
   for (int k = 2; k <= N; k <<= 1) {
@@ -533,7 +733,7 @@ __device__ __forceinline__ void lds_sort_pairs(int32_t* keys, double* vals, int 
         const int32_t k1 = keys[i1], k2 = keys[i2];
         if (k1 > k2) {
           keys[i1] = k2; keys[i2] = k1;
-          const double v1 = vals[i1]; vals[i1] = vals[i2]; vals[i2] = v1;
+          const V v1 = vals[i1]; vals[i1] = vals[i2]; vals[i2] = v1;
         }
       }
     }
@@ -546,7 +746,7 @@ __device__ __forceinline__ void lds_sort_pairs(int32_t* keys, double* vals, int 
           const int32_t k1 = keys[i1], k2 = keys[i2];
           if (k1 > k2) {
             keys[i1] = k2; keys[i2] = k1;
-            const double v1 = vals[i1]; vals[i1] = vals[i2]; vals[i2] = v1;
+            const V v1 = vals[i1]; vals[i1] = vals[i2]; vals[i2] = v1;
           }
         }
       }
@@ -555,13 +755,25 @@ __device__ __forceinline__ void lds_sort_pairs(int32_t* keys, double* vals, int 
   }
 }
 
-// Bitmap-window numeric.  LDS per row: info[words] (uint2: bits, prefix),
+// Group-wide exclusive scan (G <= 64 shuffles, else block scan).
+template <int G>
+__device__ __forceinline__ int group_scan_excl(int v, int lane, int64_t* red64) {
+  if constexpr (G <= kWave) {
+    int tot;
+    return gscan_excl<G, int>(v, lane, &tot);
+  } else {
+    int64_t tot;
+    return (int)block_scan_excl<int64_t>(v, red64, &tot);
+  }
+}
+
+// Dense numeric.  LDS per row: info[words] (uint2: bits, prefix),
 // vals[c_max] (f64), cols[c_max] (i32).  Output positions come from prefix
 // popcounts, so the row is produced already sorted.
 template <int G>
-__device__ __forceinline__ void numeric_bitmap_row(int i, int lane, const RowArgs& A, uint2* info, double* vals,
-                                                   int32_t* cols, const int64_t* rowoff, int32_t* ccol,
-                                                   double* cval, Stage* stage, int64_t* red64) {
+__device__ __forceinline__ void numeric_dense_row(int i, int lane, const RowArgs& A, uint2* info, double* vals,
+                                                  int32_t* cols, const int64_t* rowoff, int32_t* ccol,
+                                                  double* cval, Stage* stage, int64_t* red64) {
   const int lo = A.rowlo[i], hi = A.rowhi[i];
   const int words = (int)span_words(lo, hi);
   const int64_t off = rowoff[i];
@@ -570,26 +782,18 @@ __device__ __forceinline__ void numeric_bitmap_row(int i, int lane, const RowArg
   for (int t = lane; t < c; t += G) vals[t] = 0.0;
   gsync<G>();
   uint32_t* bits = (uint32_t*)info;
-  for_each_product<G, false>(i, lane, A, stage, [&](int j, int, double) {
+  for_each_product<G, false>(i, lane, A, stage, [&](int j, double) {
     if (A.upper && j < i) return;
     const uint32_t b = (uint32_t)(j - lo);
     atomicOr(&bits[(b >> 5) * 2], 1u << (b & 31u));
   });
   gsync<G>();
-  // prefix popcount over words: contiguous chunk per lane
   const int chunk = (words + G - 1) / G;
   const int w0 = lane * chunk;
   const int w1 = (w0 + chunk) < words ? (w0 + chunk) : words;
   int local = 0;
   for (int w = w0; w < w1; w++) local += __popc(info[w].x);
-  int run;
-  if constexpr (G <= kWave) {
-    int tot;
-    run = gscan_excl<G, int>(local, lane, &tot);
-  } else {
-    int64_t tot;
-    run = (int)block_scan_excl<int64_t>(local, red64, &tot);
-  }
+  int run = group_scan_excl<G>(local, lane, red64);
   for (int w = w0; w < w1; w++) {
     uint32_t x = info[w].x;
     info[w].y = (uint32_t)run;
@@ -600,12 +804,12 @@ __device__ __forceinline__ void numeric_bitmap_row(int i, int lane, const RowArg
     }
   }
   gsync<G>();
-  for_each_product<G, true>(i, lane, A, stage, [&](int j, int q, double a) {
+  for_each_product<G, true>(i, lane, A, stage, [&](int j, double prod) {
     if (A.upper && j < i) return;
     const uint32_t b = (uint32_t)(j - lo);
     const uint2 wi = info[b >> 5];
     const uint32_t pos = wi.y + __popc(wi.x & ((1u << (b & 31u)) - 1u));
-    atomicAdd(&vals[pos], a * A.bv[q]);
+    atomicAdd(&vals[pos], prod);
   });
   gsync<G>();
   for (int t = lane; t < c; t += G) {
@@ -614,40 +818,36 @@ __device__ __forceinline__ void numeric_bitmap_row(int i, int lane, const RowArg
   }
 }
 
-// Hash numeric: keys[T] (i32), vals[T] (f64), T = np2(3c/2+1); compact in
-// place, bitonic sort, write.
+// Word-hash numeric.  LDS per row (Tm = table slots of the bin maximum):
+//   hkey[Tm] i32 | hinfo[Tm] uint2 (mask, prefix) | vals[max(c_max, Tm)] f64
+//   (first holding the key sort buffer) | cols[c_max] i32
 template <int G>
-__device__ __forceinline__ void numeric_hash_row(int i, int lane, const RowArgs& A, int32_t* keys, double* vals,
-                                                 const int64_t* rowoff, int32_t* ccol, double* cval, Stage* stage,
-                                                 int64_t* red64) {
+__device__ __forceinline__ void numeric_whash_row(int i, int lane, const RowArgs& A, int32_t* hkey, uint2* hinfo,
+                                                  double* vals, int32_t* cols, const int64_t* rowoff,
+                                                  int32_t* ccol, double* cval, Stage* stage, int64_t* red64) {
   const int64_t off = rowoff[i];
   const int c = (int)(rowoff[i + 1] - off);
-  const int64_t T = np2_i64((int64_t)(c * 3) / 2 + 1);
+  const int w = A.rowwords[i] & kWordsMask;
+  const int64_t T = np2_i64((int64_t)(w * 3) / 2 + 1);
   const int logT = __builtin_ctzll((unsigned long long)T);
   const uint32_t mask = (uint32_t)T - 1u;
-  for (int t = lane; t < (int)T; t += G) { keys[t] = -1; vals[t] = 0.0; }
+  for (int t = lane; t < (int)T; t += G) { hkey[t] = -1; hinfo[t] = make_uint2(0u, 0u); }
   gsync<G>();
-  for_each_product<G, true>(i, lane, A, stage, [&](int j, int q, double a) {
+  uint32_t* hbits = (uint32_t*)hinfo;
+  for_each_product<G, false>(i, lane, A, stage, [&](int j, double) {
     if (A.upper && j < i) return;
-    uint32_t h = hash_slot(j, logT);
-    while (true) {
-      const int cur = __atomic_load_n(&keys[h], __ATOMIC_RELAXED);
-      if (cur == j) break;
-      if (cur == -1) {
-        const int old = atomicCAS(&keys[h], -1, j);
-        if (old == -1 || old == j) break;
-      }
-      h = (h + 1u) & mask;
-    }
-    atomicAdd(&vals[h], a * A.bv[q]);
+    bool claimed;
+    const uint32_t h = whash_insert(hkey, logT, mask, j >> 5, &claimed);
+    atomicOr(&hbits[h * 2], 1u << (j & 31));
   });
   gsync<G>();
-  // in-place compaction (destination index never exceeds source index)
+  // gather the occupied slots: skey = word key, sslot = slot
+  int32_t* skey = (int32_t*)vals;
+  int32_t* sslot = skey + T;
   int count = 0;
   for (int base = 0; base < (int)T; base += G) {
     const int t = base + lane;
-    const int32_t key = t < (int)T ? keys[t] : -1;
-    const double v = t < (int)T ? vals[t] : 0.0;
+    const int32_t key = t < (int)T ? hkey[t] : -1;
     const bool occ = key != -1;
     int rank, tot;
     if constexpr (G <= kWave) {
@@ -659,45 +859,87 @@ __device__ __forceinline__ void numeric_hash_row(int i, int lane, const RowArgs&
       rank = (int)block_scan_excl<int64_t>(occ ? 1 : 0, red64, &t64);
       tot = (int)t64;
     }
-    gsync<G>();
-    if (occ) { keys[count + rank] = key; vals[count + rank] = v; }
+    if (occ) { skey[count + rank] = key; sslot[count + rank] = t; }
     count += tot;
-    gsync<G>();
   }
-  lds_sort_pairs<G>(keys, vals, c, lane);
+  gsync<G>();
+  lds_sort_pairs<G, int32_t>(skey, sslot, w, lane);
+  // prefix popcounts over the sorted keys; emit the sorted column list
+  const int chunk = (w + G - 1) / G;
+  const int k0 = lane * chunk;
+  const int k1 = (k0 + chunk) < w ? (k0 + chunk) : w;
+  int local = 0;
+  for (int k = k0; k < k1; k++) local += __popc(hinfo[sslot[k]].x);
+  int run = group_scan_excl<G>(local, lane, red64);
+  for (int k = k0; k < k1; k++) {
+    const int sl = sslot[k];
+    const int32_t wk = skey[k];
+    uint32_t x = hinfo[sl].x;
+    hinfo[sl].y = (uint32_t)run;
+    while (x) {
+      const int bpos = __builtin_ctz(x);
+      cols[run++] = wk * 32 + bpos;
+      x &= x - 1u;
+    }
+  }
+  gsync<G>();
+  for (int t = lane; t < c; t += G) vals[t] = 0.0;
+  gsync<G>();
+  for_each_product<G, true>(i, lane, A, stage, [&](int j, double prod) {
+    if (A.upper && j < i) return;
+    const uint32_t h = whash_find(hkey, logT, mask, j >> 5);
+    const uint2 wi = hinfo[h];
+    const uint32_t pos = wi.y + __popc(wi.x & ((1u << (j & 31)) - 1u));
+    atomicAdd(&vals[pos], prod);
+  });
+  gsync<G>();
   for (int t = lane; t < c; t += G) {
-    ccol[off + t] = keys[t];
+    ccol[off + t] = cols[t];
     cval[off + t] = vals[t];
   }
 }
 
-template <int G, bool BITMAP>
-__global__ void __launch_bounds__(1024) k_numeric_lds(RowArgs A, int32_t words_max, int32_t c_max,
-                                                      const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
-                                                      double* __restrict__ cval) {
+__host__ __device__ __forceinline__ int64_t whash_slots_num(int64_t w) {
+  int64_t T = 2;
+  while (T < (w * 3) / 2 + 1) T <<= 1;
+  return T;
+}
+__host__ __device__ __forceinline__ int64_t dense_row_bytes(int64_t words_max, int64_t c_max) {
+  return ((words_max * 8 + c_max * 12) + 15) & ~(int64_t)15;
+}
+__host__ __device__ __forceinline__ int64_t whash_row_bytes(int64_t w_max, int64_t c_max) {
+  const int64_t T = whash_slots_num(w_max);
+  const int64_t v = c_max > T ? c_max : T;
+  return ((T * 12 + v * 8 + c_max * 4) + 15) & ~(int64_t)15;
+}
+
+template <int G, bool DENSE>
+__global__ void __launch_bounds__(256) k_numeric_lds(RowArgs A, int32_t words_max, int32_t c_max,
+                                                     const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
+                                                     double* __restrict__ cval) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ Stage stage_buf[G > kWave ? 1 : 0 + 1];
   __shared__ int64_t red64[17];
   const int R = blockDim.x / G;
   const int gi = threadIdx.x / G, lane = threadIdx.x % G;
-  const int r = blockIdx.x * R + gi;
+  const int r = xcd_block(blockIdx.x, gridDim.x) * R + gi;
   if (G <= kWave && r >= A.nrows) return;
   const int i = A.rows[r];
-  if constexpr (BITMAP) {
-    const size_t row_bytes = (size_t)words_max * 8 + (size_t)c_max * 12;
-    const size_t rb = (row_bytes + 15) & ~(size_t)15;
-    unsigned char* base = lds_raw + (size_t)gi * rb;
+  if constexpr (DENSE) {
+    unsigned char* base = lds_raw + (size_t)gi * dense_row_bytes(words_max, c_max);
     uint2* info = (uint2*)base;
     double* vals = (double*)(base + (size_t)words_max * 8);
     int32_t* cols = (int32_t*)(base + (size_t)words_max * 8 + (size_t)c_max * 8);
-    numeric_bitmap_row<G>(i, lane, A, info, vals, cols, rowoff, ccol, cval, stage_buf, red64);
+    numeric_dense_row<G>(i, lane, A, info, vals, cols, rowoff, ccol, cval, stage_buf, red64);
   } else {
-    const int64_t T = np2_i64((int64_t)(c_max * 3) / 2 + 1);
-    const size_t rb = ((size_t)T * 12 + 15) & ~(size_t)15;
-    unsigned char* base = lds_raw + (size_t)gi * rb;
-    double* vals = (double*)base;
-    int32_t* keys = (int32_t*)(base + (size_t)T * 8);
-    numeric_hash_row<G>(i, lane, A, keys, vals, rowoff, ccol, cval, stage_buf, red64);
+    const int64_t Tm = whash_slots_num(words_max);
+    const int64_t vm = c_max > Tm ? c_max : Tm;
+    unsigned char* base = lds_raw + (size_t)gi * whash_row_bytes(words_max, c_max);
+    uint2* hinfo = (uint2*)base;
+    double* vals = (double*)(base + Tm * 8);
+    int32_t* hkey = (int32_t*)(base + Tm * 8 + vm * 8);
+    int32_t* cols = hkey + Tm;
+    numeric_whash_row<G>(i, lane, A, hkey, hinfo, vals, cols, rowoff, ccol, cval, stage_buf, red64);
   }
 }
 
@@ -723,7 +965,7 @@ __global__ void __launch_bounds__(256) k_numeric_global(RowArgs A, int32_t slot_
     for (int t = lane; t < c; t += G) cval[off + t] = 0.0;
     __threadfence();
     __syncthreads();
-    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, int, double) {
+    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
       if (A.upper && j < i) return;
       const uint32_t b = (uint32_t)(j - lo);
       atomicOr(&bits[(b >> 5) * 2], 1u << (b & 31u));
@@ -750,32 +992,425 @@ __global__ void __launch_bounds__(256) k_numeric_global(RowArgs A, int32_t slot_
     __threadfence();
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for_each_product<G, true>(i, lane, A, stage_buf, [&](int j, int q, double a) {
+    for_each_product<G, true>(i, lane, A, stage_buf, [&](int j, double prod) {
       if (A.upper && j < i) return;
       const uint32_t b = (uint32_t)(j - lo);
       const uint2 wi = info[b >> 5];
       const uint32_t pos = wi.y + __popc(wi.x & ((1u << (b & 31u)) - 1u));
-      unsafeAtomicAdd(&cval[off + pos], a * A.bv[q]);
+      unsafeAtomicAdd(&cval[off + pos], prod);
     });
   }
 }
 
 // ---------------------------------------------------------------------------
+// Window kernels (dense mode of the wave-level bins).  Persistent: each block
+// owns a contiguous chunk of the bin's row list (XCD-aware numbering), and
+// each group of G lanes walks rows of that chunk.  Per group, LDS holds a
+// bitmap over the row's output window, kept all-zero between rows by
+// clearing only the words the row touched (recorded in a touched list), so
+// the per-row cost does not grow with the window width.  The product loop is
+// branch-light: A-entry metadata is staged in LDS and read as broadcasts, B
+// gathers use 32-bit offsets and are issued for every lane, and lanes without
+// a product update a private dummy slot instead of branching.
+// ---------------------------------------------------------------------------
+
+constexpr int kBatch = 8;     // A entries whose B gathers are issued together
+constexpr int kWordCap = 32;  // words per row kept in the symbolic -> numeric word list
+
+// Stage G A entries of a row chunk: (bs, be) of their B rows (zero-padded so
+// a batch may read past the chunk end) and, for the value pass, a_ik.
+// Returns true when some staged B row is longer than G.
+template <int G, bool VALS>
+__device__ __forceinline__ bool stage_entries(int base, int e, int lane, const RowArgs& A, int2* mbe, double* ma) {
+  const int p = base + lane;
+  int2 v = make_int2(0, 0);
+  double a = 0.0;
+  if (p < e) {
+    const int4 bm = A.bmeta[A.ac[p]];
+    v = make_int2(bm.x, bm.y);
+    if (VALS) a = A.av[p];
+  }
+  mbe[lane] = v;
+  if (VALS) ma[lane] = a;
+  return group_bits<G>(__ballot(v.y - v.x > G)) != 0ull;
+}
+
+// Software pipeline over a group's rows: the next row's metadata chain
+// rows[r] -> ap[i] -> ac[p] -> bmeta[k] is issued in four steps spread over
+// the current row's phases, so each dependent load is waited for behind work
+// that was already in flight.  The row's first G A entries come from these
+// registers; longer A rows stage further chunks inline.
+struct NextRow {
+  int i, s, e;
+  int2 be;     // this lane's first-chunk entry: B-row start / end
+  double a;    // this lane's first-chunk a_ik (value pass only)
+};
+
+template <int G, bool VALS>
+__device__ __forceinline__ void first_chunk_load(NextRow& nx, int lane, const RowArgs& A) {
+  const int p = nx.s + lane;
+  nx.be = make_int2(0, 0);
+  nx.a = 0.0;
+  if (p < nx.e) {
+    const int4 bm = A.bmeta[A.ac[p]];
+    nx.be = make_int2(bm.x, bm.y);
+    if (VALS) nx.a = A.av[p];
+  }
+}
+
+// Structure pass for one chunk whose entries are staged in mbe (m entries,
+// zero-padded to G): OR every product's bit into the window, recording
+// first-touched words when TRACK.  All gathers of the chunk are issued
+// before the LDS updates.
+template <int G, bool UPPER, bool W64, bool TRACK>
+__device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool lng, const RowArgs& A,
+                                         const int2* mbe, uint32_t* bits, int W, uint32_t* touched, int* ctl) {
+  constexpr int B = G < 32 ? G : 32;
+  auto apply = [&](int j, bool ok) {
+    const bool use = ok && (!UPPER || j >= i);
+    const uint32_t b = (uint32_t)(j - lo);
+    const uint32_t old = atomicOr(&bits[use ? (b >> 5) : (uint32_t)(W + lane)], 1u << (b & 31u));
+    if (TRACK && __builtin_expect(use && old == 0u, 0)) {
+      const int idx = atomicAdd(&ctl[0], 1);
+      if (idx < kTouchCap) touched[idx] = b >> 5;
+    }
+  };
+  for (int t0 = 0; t0 < m; t0 += B) {
+    int jj[B];
+    uint32_t okm = 0u;
+#pragma unroll
+    for (int u = 0; u < B; u++) {
+      const int2 be = mbe[t0 + u];
+      const int q = be.x + lane;
+      const bool ok = q < be.y;
+      okm |= (ok ? 1u : 0u) << u;
+      jj[u] = ld_idx<int32_t, W64>(A.bc, ok ? (uint32_t)q : 0u);
+    }
+#pragma unroll
+    for (int u = 0; u < B; u++) apply(jj[u], (okm >> u) & 1u);
+  }
+  if (lng) {
+    for (int t = 0; t < m; t++) {
+      const int2 be = mbe[t];
+      for (int q = be.x + G + lane; q - lane < be.y; q += G) {
+        const bool ok = q < be.y;
+        apply(ld_idx<int32_t, W64>(A.bc, ok ? (uint32_t)q : 0u), ok);
+      }
+    }
+  }
+}
+
+// Value pass for one staged chunk (mbe, ma): a_ik * b_kj added at its final
+// sorted position, in A-row storage order per output column.
+template <int G, bool UPPER, bool W64>
+__device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool lng, const RowArgs& A,
+                                            const int2* mbe, const double* ma, const uint32_t* bits,
+                                            const uint16_t* pre, double* vals, int Cm) {
+  constexpr int B = G < 32 ? G : 32;
+  auto add = [&](int j, double prod, bool ok) {
+    const bool use = ok && (!UPPER || j >= i);
+    const uint32_t bb = use ? (uint32_t)(j - lo) : 0u;
+    const uint32_t w = bb >> 5;
+    const uint32_t pos = (uint32_t)pre[w] + __popc(bits[w] & ((1u << (bb & 31u)) - 1u));
+    atomicAdd(&vals[use ? pos : (uint32_t)(Cm + lane)], prod);
+  };
+  if (!lng) {
+    for (int t0 = 0; t0 < m; t0 += B) {
+      int jj[B];
+      double pv[B];
+      uint32_t okm = 0u;
+#pragma unroll
+      for (int u = 0; u < B; u++) {
+        const int2 be = mbe[t0 + u];
+        const int q = be.x + lane;
+        const bool ok = q < be.y;
+        okm |= (ok ? 1u : 0u) << u;
+        const uint32_t qq = ok ? (uint32_t)q : 0u;
+        jj[u] = ld_idx<int32_t, W64>(A.bc, qq);
+        pv[u] = ma[t0 + u] * ld_idx<double, W64>(A.bv, qq);
+      }
+#pragma unroll
+      for (int u = 0; u < B; u++) add(jj[u], pv[u], (okm >> u) & 1u);
+    }
+  } else {
+    for (int t = 0; t < m; t++) {
+      const int2 be = mbe[t];
+      const double a = ma[t];
+      for (int q = be.x + lane; q - lane < be.y; q += G) {
+        const bool ok = q < be.y;
+        const uint32_t qq = ok ? (uint32_t)q : 0u;
+        add(ld_idx<int32_t, W64>(A.bc, qq), a * ld_idx<double, W64>(A.bv, qq), ok);
+      }
+    }
+  }
+}
+
+// Symbolic window: row count, distinct words, and (when it fits) the row's
+// sorted word list (word, mask) for the numeric kernel.
+template <int G, bool UPPER, bool W64>
+__global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_t chunk, int32_t* __restrict__ rowcnt) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int R = blockDim.x / G;
+  const int gi = threadIdx.x / G, lane = threadIdx.x % G;
+  unsigned char* base = lds_raw + (size_t)gi * sym_window_bytes(G, W);
+  int2* mbe = (int2*)base;
+  uint32_t* bits = (uint32_t*)(base + align16(G * 8));
+  uint32_t* touched = (uint32_t*)((unsigned char*)bits + align16((W + G) * 4));
+  int* ctl = (int*)((unsigned char*)touched + align16(kTouchCap * 4));
+  for (int t = lane; t < W + G; t += G) bits[t] = 0u;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = lb * chunk;
+  const int r1 = (r0 + chunk) < A.nrows ? (r0 + chunk) : A.nrows;
+  int r = r0 + gi;
+  NextRow cur{0, 0, 0, make_int2(0, 0), 0.0};
+  if (r < r1) {
+    cur.i = A.rows[r];
+    cur.s = A.ap[cur.i];
+    cur.e = A.ap[cur.i + 1];
+    first_chunk_load<G, false>(cur, lane, A);
+  }
+  gsync<G>();
+  for (; r < r1; r += R) {
+    const int rn = r + R;
+    const bool hasn = rn < r1;
+    NextRow nx{0, 0, 0, make_int2(0, 0), 0.0};
+    if (hasn) nx.i = A.rows[rn];                                   // prefetch step 1
+    const int i = cur.i;
+    const int lo = A.rowlo[i];
+    if (lane == 0) ctl[0] = 0;
+    mbe[lane] = cur.be;
+    bool lng = group_bits<G>(__ballot(cur.be.y - cur.be.x > G)) != 0ull;
+    gsync<G>();
+    int m = (cur.e - cur.s) < G ? (cur.e - cur.s) : G;
+    if (hasn) { nx.s = A.ap[nx.i]; nx.e = A.ap[nx.i + 1]; }         // prefetch step 2
+    or_chunk<G, UPPER, W64, true>(i, lo, lane, m, lng, A, mbe, bits, W, touched, ctl);
+    gsync<G>();
+    for (int b0 = cur.s + G; b0 < cur.e; b0 += G) {                // long A rows
+      const bool l2 = stage_entries<G, false>(b0, cur.e, lane, A, mbe, nullptr);
+      gsync<G>();
+      m = (cur.e - b0) < G ? (cur.e - b0) : G;
+      or_chunk<G, UPPER, W64, true>(i, lo, lane, m, l2, A, mbe, bits, W, touched, ctl);
+      gsync<G>();
+    }
+    int nk = -1;
+    if (hasn && nx.s + lane < nx.e) nk = A.ac[nx.s + lane];          // prefetch step 3
+    const int nt = ctl[0];
+    int c = 0;
+    if (nt <= kTouchCap) {
+      constexpr int PL = (kTouchCap + G - 1) / G;
+      uint32_t myw[PL], mym[PL];
+#pragma unroll
+      for (int k = 0; k < PL; k++) {
+        const int u = lane + k * G;
+        myw[k] = u < nt ? touched[u] : 0u;
+        mym[k] = u < nt ? bits[myw[k]] : 0u;
+        c += __popc(mym[k]);
+      }
+      if (nt <= kWordCap) {
+        uint2* wl = A.wlist + (size_t)i * kWordCap;
+#pragma unroll
+        for (int k = 0; k < PL; k++) {
+          const int u = lane + k * G;
+          if (u < nt) {
+            int rank = 0;
+            for (int v = 0; v < nt; v++) rank += touched[v] < myw[k] ? 1 : 0;
+            wl[rank] = make_uint2(myw[k], mym[k]);
+          }
+        }
+      }
+      gsync<G>();
+#pragma unroll
+      for (int k = 0; k < PL; k++)
+        if (lane + k * G < nt) bits[myw[k]] = 0u;
+    } else {
+      const int words = (int)span_words(lo, A.rowhi[i]);
+      for (int w = lane; w < words; w += G) { c += __popc(bits[w]); bits[w] = 0u; }
+    }
+    c = gsum<G, int32_t>(c, nullptr);
+    if (lane == 0) { rowcnt[i] = c; A.rowwords[i] = nt; }
+    if (nk >= 0) { const int4 bm = A.bmeta[nk]; nx.be = make_int2(bm.x, bm.y); }   // prefetch step 4
+    cur = nx;
+    gsync<G>();
+  }
+}
+
+template <int G, bool UPPER, bool W64>
+__global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_t Cm, int32_t chunk,
+                                                    const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
+                                                    double* __restrict__ cval) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int R = blockDim.x / G;
+  const int gi = threadIdx.x / G, lane = threadIdx.x % G;
+  unsigned char* base = lds_raw + (size_t)gi * num_window_bytes(G, W, Cm);
+  int2* mbe = (int2*)base;
+  double* ma = (double*)(base + G * 8);
+  uint32_t* bits = (uint32_t*)(base + align16(G * 16));
+  uint16_t* pre = (uint16_t*)((unsigned char*)bits + align16((W + G) * 4));
+  double* vals = (double*)((unsigned char*)pre + align16(W * 2));
+  int32_t* cols = (int32_t*)((unsigned char*)vals + align16((Cm + G) * 8));
+  for (int t = lane; t < W + G; t += G) bits[t] = 0u;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = lb * chunk;
+  const int r1 = (r0 + chunk) < A.nrows ? (r0 + chunk) : A.nrows;
+  constexpr int PL = (kWordCap + G - 1) / G;
+  int r = r0 + gi;
+  NextRow cur{0, 0, 0, make_int2(0, 0), 0.0};
+  if (r < r1) {
+    cur.i = A.rows[r];
+    cur.s = A.ap[cur.i];
+    cur.e = A.ap[cur.i + 1];
+    first_chunk_load<G, true>(cur, lane, A);
+  }
+  gsync<G>();
+  for (; r < r1; r += R) {
+    const int rn = r + R;
+    const bool hasn = rn < r1;
+    NextRow nx{0, 0, 0, make_int2(0, 0), 0.0};
+    if (hasn) nx.i = A.rows[rn];                                   // prefetch step 1
+    const int i = cur.i;
+    const int lo = A.rowlo[i];
+    const int64_t off = rowoff[i];
+    const int c = (int)(rowoff[i + 1] - off);
+    const int rw = A.rowwords[i];
+    const int nt = rw & kWordsMask;
+    const bool have_list = (rw & kNoWordList) == 0 && nt <= kWordCap;
+    uint32_t myw[PL];
+    uint2 ent[PL];
+    if (have_list) {
+      const uint2* wl = A.wlist + (size_t)i * kWordCap;
+#pragma unroll
+      for (int k = 0; k < PL; k++) {
+        const int u = lane + k * G;
+        ent[k] = u < nt ? wl[u] : make_uint2(0u, 0u);
+      }
+    }
+    for (int t = lane; t < c; t += G) vals[t] = 0.0;
+    mbe[lane] = cur.be;
+    ma[lane] = cur.a;
+    const bool lng = group_bits<G>(__ballot(cur.be.y - cur.be.x > G)) != 0ull;
+    if (have_list) {
+      // the symbolic pass left this row's words sorted: place them in the
+      // window with their prefix popcounts and emit the column list
+      int local = 0;
+#pragma unroll
+      for (int k = 0; k < PL; k++) local += __popc(ent[k].y);
+      // entries are strided over lanes (u = lane + k*G): scan in that order
+      int run = 0;
+#pragma unroll
+      for (int k = 0; k < PL; k++) {
+        int tot;
+        const int pc = __popc(ent[k].y);
+        const int ex = gscan_excl<G, int>(pc, lane, &tot);
+        int rr = run + ex;
+        const int u = lane + k * G;
+        if (u < nt) {
+          bits[ent[k].x] = ent[k].y;
+          pre[ent[k].x] = (uint16_t)rr;
+          uint32_t x = ent[k].y;
+          while (x) {
+            cols[rr++] = lo + (int)ent[k].x * 32 + __builtin_ctz(x);
+            x &= x - 1u;
+          }
+        }
+        myw[k] = ent[k].x;
+        run += tot;
+      }
+      (void)local;
+    } else {
+      gsync<G>();
+      const int m0 = (cur.e - cur.s) < G ? (cur.e - cur.s) : G;
+      or_chunk<G, UPPER, W64, false>(i, lo, lane, m0, lng, A, mbe, bits, W, nullptr, nullptr);
+      gsync<G>();
+      for (int b0 = cur.s + G; b0 < cur.e; b0 += G) {
+        const bool l2 = stage_entries<G, false>(b0, cur.e, lane, A, mbe, nullptr);
+        gsync<G>();
+        const int m = (cur.e - b0) < G ? (cur.e - b0) : G;
+        or_chunk<G, UPPER, W64, false>(i, lo, lane, m, l2, A, mbe, bits, W, nullptr, nullptr);
+        gsync<G>();
+      }
+      const int words = (int)span_words(lo, A.rowhi[i]);
+      const int ch = (words + G - 1) / G;
+      const int w0 = lane * ch;
+      const int w1 = (w0 + ch) < words ? (w0 + ch) : words;
+      int local = 0;
+      for (int w = w0; w < w1; w++) local += __popc(bits[w]);
+      int tot;
+      int run = gscan_excl<G, int>(local, lane, &tot);
+      for (int w = w0; w < w1; w++) {
+        uint32_t x = bits[w];
+        pre[w] = (uint16_t)run;
+        while (x) {
+          cols[run++] = lo + w * 32 + __builtin_ctz(x);
+          x &= x - 1u;
+        }
+      }
+      // restage the first chunk (the structure pass reused mbe)
+      mbe[lane] = cur.be;
+      ma[lane] = cur.a;
+    }
+    gsync<G>();
+    if (hasn) { nx.s = A.ap[nx.i]; nx.e = A.ap[nx.i + 1]; }         // prefetch step 2
+    {
+      const int m0 = (cur.e - cur.s) < G ? (cur.e - cur.s) : G;
+      value_chunk<G, UPPER, W64>(i, lo, lane, m0, lng, A, mbe, ma, bits, pre, vals, Cm);
+      gsync<G>();
+      for (int b0 = cur.s + G; b0 < cur.e; b0 += G) {              // long A rows
+        const bool l2 = stage_entries<G, true>(b0, cur.e, lane, A, mbe, ma);
+        gsync<G>();
+        const int m = (cur.e - b0) < G ? (cur.e - b0) : G;
+        value_chunk<G, UPPER, W64>(i, lo, lane, m, l2, A, mbe, ma, bits, pre, vals, Cm);
+        gsync<G>();
+      }
+    }
+    int nk = -1;
+    if (hasn && nx.s + lane < nx.e) nk = A.ac[nx.s + lane];          // prefetch step 3
+    for (int t = lane; t < c; t += G) {
+      ccol[off + t] = cols[t];
+      cval[off + t] = vals[t];
+    }
+    if (have_list) {
+#pragma unroll
+      for (int k = 0; k < PL; k++)
+        if (lane + k * G < nt) bits[myw[k]] = 0u;
+    } else {
+      const int words = (int)span_words(lo, A.rowhi[i]);
+      for (int w = lane; w < words; w += G) bits[w] = 0u;
+    }
+    if (nk >= 0) {                                                  // prefetch step 4
+      const int4 bm = A.bmeta[nk];
+      nx.be = make_int2(bm.x, bm.y);
+      nx.a = A.av[nx.s + lane];
+    }
+    cur = nx;
+    gsync<G>();
+  }
+}
+
+static int g_num_cus = 256;
+void set_num_cus(int n) { g_num_cus = n > 0 ? n : 256; }
+
+// ---------------------------------------------------------------------------
 // launch helpers
 // ---------------------------------------------------------------------------
+static bool bin_dense(int bin) {
+  return bin == kBinWgBitmap || (bin > 0 && bin < kBinWgBitmap && bin_is_bitmap(bin));
+}
+
 int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
   if (bin == kBinGlBitmap || bin == kBinGlHash || bin == 0) return 0;
-  const bool bm = (bin == kBinWgBitmap) || (bin < kBinWgBitmap && bin_is_bitmap(bin));
-  return bm ? (int64_t)words_max * 4 : sym_slots(metric_max) * 4;
+  (void)metric_max;
+  if (bin == kBinWgBitmap) return (int64_t)words_max * 4;
+  if (bin_dense(bin)) return sym_window_bytes(bin_group(bin), words_max);
+  return sym_slots(words_max) * 8;
 }
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max) {
   if (bin == kBinGlBitmap || bin == kBinGlHash || bin == 0) return 0;
-  const bool bm = (bin == kBinWgBitmap) || (bin < kBinWgBitmap && bin_is_bitmap(bin));
-  if (bm) return (((int64_t)words_max * 8 + (int64_t)c_max * 12) + 15) & ~(int64_t)15;
-  return ((num_slots(c_max) * 12) + 15) & ~(int64_t)15;
+  if (bin == kBinWgBitmap) return dense_row_bytes(words_max, c_max);
+  if (bin_dense(bin)) return num_window_bytes(bin_group(bin), words_max, c_max);
+  return whash_row_bytes(words_max, c_max);
 }
 
-// rows per block so that rows * bytes <= budget, at least 1 and at most 1024/G
+// rows per block so that rows * bytes <= budget, at least 1 and at most 256/G
 static int rows_per_block(int G, int64_t row_bytes, int64_t budget) {
   int R = 256 / G;
   if (R < 1) R = 1;
@@ -788,51 +1423,113 @@ static int rows_per_block(int G, int64_t row_bytes, int64_t budget) {
 static int64_t g_lds_budget = 64 * 1024;
 void set_lds_budget(int64_t b) { g_lds_budget = b; }
 
-template <int G, bool BM>
-static void sym_lds(const RowArgs& a, int32_t row_words, int32_t* rowcnt, hipStream_t s) {
+template <int G, bool DENSE>
+static void sym_lds(const RowArgs& a, int32_t words_max, int32_t* rowcnt, hipStream_t s) {
+  const int32_t row_words = DENSE ? words_max : (int32_t)(sym_slots(words_max) * 2);
   const int64_t rb = (int64_t)row_words * 4;
   const int R = G > kWave ? 1 : rows_per_block(G, rb, g_lds_budget);
   const size_t lds = (size_t)R * rb;
   const unsigned blocks = (unsigned)((a.nrows + R - 1) / R);
-  hipFuncSetAttribute((const void*)k_symbolic_lds<G, BM>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_symbolic_lds<G, BM>), dim3(blocks), dim3(R * G), lds, s, a, row_words, rowcnt);
+  (void)hipFuncSetAttribute((const void*)k_symbolic_lds<G, DENSE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_symbolic_lds<G, DENSE>), dim3(blocks), dim3(R * G), lds, s, a, row_words, rowcnt);
 }
 
-template <int G, bool BM>
+template <int G, bool DENSE>
 static void num_lds(const RowArgs& a, int32_t words_max, int32_t c_max, const int64_t* rowoff, int32_t* ccol,
                     double* cval, hipStream_t s) {
-  int64_t rb;
-  if (BM) rb = (((int64_t)words_max * 8 + (int64_t)c_max * 12) + 15) & ~(int64_t)15;
-  else rb = ((num_slots(c_max) * 12) + 15) & ~(int64_t)15;
+  const int64_t rb = DENSE ? dense_row_bytes(words_max, c_max) : whash_row_bytes(words_max, c_max);
   const int R = G > kWave ? 1 : rows_per_block(G, rb, g_lds_budget);
   const size_t lds = (size_t)R * rb;
   const unsigned blocks = (unsigned)((a.nrows + R - 1) / R);
-  hipFuncSetAttribute((const void*)k_numeric_lds<G, BM>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_numeric_lds<G, BM>), dim3(blocks), dim3(R * G), lds, s, a, words_max, c_max, rowoff, ccol,
+  (void)hipFuncSetAttribute((const void*)k_numeric_lds<G, DENSE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_numeric_lds<G, DENSE>), dim3(blocks), dim3(R * G), lds, s, a, words_max, c_max, rowoff, ccol,
                      cval);
 }
 
-void launch_symbolic_bin(int bin, const RowArgs& a, int32_t lds_row_words, int32_t* rowcnt, uint32_t* gscratch,
+static int64_t g_lds_per_cu = 160 * 1024;
+void set_lds_per_cu(int64_t b) { g_lds_per_cu = b > 0 ? b : 65536; }
+
+// Groups per block for a window kernel and the persistent grid.
+static void window_shape(int G, int64_t group_bytes, int32_t nrows, int* R_out, unsigned* blocks_out, int* chunk_out) {
+  int64_t budget = g_lds_per_cu / 2;   // two blocks per CU
+  if (budget > g_lds_budget * 2) budget = g_lds_budget * 2;
+  int R = (int)(budget / (group_bytes > 0 ? group_bytes : 1));
+  const int per_wave = kWave / G;
+  if (R > 256 / G) R = 256 / G;
+  if (R >= per_wave) R -= R % per_wave;
+  if (R < 1) R = 1;
+  const int64_t blocks_per_cu = std::max<int64_t>(1, g_lds_per_cu / ((int64_t)R * group_bytes));
+  int64_t blocks = (nrows + R - 1) / R;
+  const int64_t cap = (int64_t)g_num_cus * blocks_per_cu * 2;
+  if (blocks > cap) blocks = cap;
+  if (blocks < 1) blocks = 1;
+  *R_out = R;
+  *blocks_out = (unsigned)blocks;
+  *chunk_out = (int)((nrows + blocks - 1) / blocks);
+}
+
+constexpr int64_t kNarrowIdx = (int64_t)1 << 29;  // 32-bit byte offsets into B's f64 values
+
+template <int G, bool UPPER, bool W64>
+static void sym_window_launch(const RowArgs& a, int32_t W, int32_t* rowcnt, hipStream_t s) {
+  const int64_t gb = sym_window_bytes(G, W);
+  int R; unsigned blocks; int chunk;
+  window_shape(G, gb, a.nrows, &R, &blocks, &chunk);
+  const size_t lds = (size_t)R * gb;
+  (void)hipFuncSetAttribute((const void*)k_sym_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_sym_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, chunk, rowcnt);
+}
+template <int G, bool UPPER, bool W64>
+static void num_window_launch(const RowArgs& a, int32_t W, int32_t Cm, const int64_t* rowoff, int32_t* ccol,
+                              double* cval, hipStream_t s) {
+  const int64_t gb = num_window_bytes(G, W, Cm);
+  int R; unsigned blocks; int chunk;
+  window_shape(G, gb, a.nrows, &R, &blocks, &chunk);
+  const size_t lds = (size_t)R * gb;
+  (void)hipFuncSetAttribute((const void*)k_num_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_num_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, Cm, chunk, rowoff,
+                     ccol, cval);
+}
+template <int G>
+static void sym_window_any(const RowArgs& a, int32_t W, int32_t* rowcnt, hipStream_t s) {
+  const bool wide = a.bnnz >= kNarrowIdx;
+  if (a.upper) { if (wide) sym_window_launch<G, true, true>(a, W, rowcnt, s); else sym_window_launch<G, true, false>(a, W, rowcnt, s); }
+  else { if (wide) sym_window_launch<G, false, true>(a, W, rowcnt, s); else sym_window_launch<G, false, false>(a, W, rowcnt, s); }
+}
+template <int G>
+static void num_window_any(const RowArgs& a, int32_t W, int32_t Cm, const int64_t* rowoff, int32_t* ccol,
+                           double* cval, hipStream_t s) {
+  const bool wide = a.bnnz >= kNarrowIdx;
+  if (a.upper) {
+    if (wide) num_window_launch<G, true, true>(a, W, Cm, rowoff, ccol, cval, s);
+    else num_window_launch<G, true, false>(a, W, Cm, rowoff, ccol, cval, s);
+  } else {
+    if (wide) num_window_launch<G, false, true>(a, W, Cm, rowoff, ccol, cval, s);
+    else num_window_launch<G, false, false>(a, W, Cm, rowoff, ccol, cval, s);
+  }
+}
+
+void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* rowcnt, uint32_t* gscratch,
                          int32_t gslots, hipStream_t s) {
   if (a.nrows <= 0 || bin <= 0) return;
   if (bin == kBinGlBitmap) {
     const unsigned blocks = (unsigned)(a.nrows < gslots ? a.nrows : gslots);
-    hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(256), 0, s, a, lds_row_words, gscratch, rowcnt);
+    hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(256), 0, s, a, words_max, gscratch, rowcnt);
     return;
   }
-  if (bin == kBinWgBitmap) { sym_lds<256, true>(a, lds_row_words, rowcnt, s); return; }
-  if (bin == kBinWgHash) { sym_lds<256, false>(a, lds_row_words, rowcnt, s); return; }
+  if (bin == kBinWgBitmap) { sym_lds<256, true>(a, words_max, rowcnt, s); return; }
+  if (bin == kBinWgHash) { sym_lds<256, false>(a, words_max, rowcnt, s); return; }
   const int G = bin_group(bin);
   const bool bm = bin_is_bitmap(bin);
   switch (G * 2 + (bm ? 1 : 0)) {
-    case 17: sym_lds<8, true>(a, lds_row_words, rowcnt, s); break;
-    case 16: sym_lds<8, false>(a, lds_row_words, rowcnt, s); break;
-    case 33: sym_lds<16, true>(a, lds_row_words, rowcnt, s); break;
-    case 32: sym_lds<16, false>(a, lds_row_words, rowcnt, s); break;
-    case 65: sym_lds<32, true>(a, lds_row_words, rowcnt, s); break;
-    case 64: sym_lds<32, false>(a, lds_row_words, rowcnt, s); break;
-    case 129: sym_lds<64, true>(a, lds_row_words, rowcnt, s); break;
-    case 128: sym_lds<64, false>(a, lds_row_words, rowcnt, s); break;
+    case 17: sym_window_any<8>(a, words_max, rowcnt, s); break;
+    case 16: sym_lds<8, false>(a, words_max, rowcnt, s); break;
+    case 33: sym_window_any<16>(a, words_max, rowcnt, s); break;
+    case 32: sym_lds<16, false>(a, words_max, rowcnt, s); break;
+    case 65: sym_window_any<32>(a, words_max, rowcnt, s); break;
+    case 64: sym_lds<32, false>(a, words_max, rowcnt, s); break;
+    case 129: sym_window_any<64>(a, words_max, rowcnt, s); break;
+    case 128: sym_lds<64, false>(a, words_max, rowcnt, s); break;
     default: break;
   }
 }
@@ -850,13 +1547,13 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
   const int G = bin_group(bin);
   const bool bm = bin_is_bitmap(bin);
   switch (G * 2 + (bm ? 1 : 0)) {
-    case 17: num_lds<8, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 17: num_window_any<8>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 16: num_lds<8, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
-    case 33: num_lds<16, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 33: num_window_any<16>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 32: num_lds<16, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
-    case 65: num_lds<32, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 65: num_window_any<32>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 64: num_lds<32, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
-    case 129: num_lds<64, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 129: num_window_any<64>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 128: num_lds<64, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     default: break;
   }
@@ -946,7 +1643,7 @@ __global__ void __launch_bounds__(256) k_segsort_wave(int32_t n, const int64_t* 
     const int64_t o = off[r];
     for (int t = lane; t < L; t += kWave) { sk[w][t] = col[o + t]; sv[w][t] = val[o + t]; }
     gsync<kWave>();
-    lds_sort_pairs<kWave>(sk[w], sv[w], L, lane);
+    lds_sort_pairs<kWave, double>(sk[w], sv[w], L, lane);
     for (int t = lane; t < L; t += kWave) { col[o + t] = sk[w][t]; val[o + t] = sv[w][t]; }
     gsync<kWave>();
   }
@@ -965,7 +1662,7 @@ __global__ void __launch_bounds__(256) k_segsort_wg(int32_t n, const int64_t* __
     __syncthreads();
     for (int t = threadIdx.x; t < L; t += 256) { sk[t] = col[o + t]; sv[t] = val[o + t]; }
     __syncthreads();
-    lds_sort_pairs<256>(sk, sv, L, threadIdx.x);
+    lds_sort_pairs<256, double>(sk, sv, L, threadIdx.x);
     for (int t = threadIdx.x; t < L; t += 256) { col[o + t] = sk[t]; val[o + t] = sv[t]; }
   }
 }
@@ -978,7 +1675,7 @@ __global__ void __launch_bounds__(256) k_segsort_global(int32_t n, const int64_t
     if (L <= kSegWg) continue;
     const int64_t o = off[r];
     __syncthreads();
-    lds_sort_pairs<256>(col + o, val + o, L, threadIdx.x);
+    lds_sort_pairs<256, double>(col + o, val + o, L, threadIdx.x);
   }
 }
 

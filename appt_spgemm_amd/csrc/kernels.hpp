@@ -4,9 +4,10 @@
 //   0                 empty output row
 //   1 .. 24           LDS group bins: 1 + (cls*2 + mode)*4 + gidx
 //                       cls  : products (symbolic) / row nnz (numeric) class
-//                       mode : 0 dense bitmap window, 1 open-addressing hash
+//                       mode : 0 dense bitmap over the row's output span,
+//                              1 word hash (32-column words as keys)
 //                       gidx : lanes per row G = 8, 16, 32, 64
-//   25 / 26           one workgroup per row, bitmap / hash in LDS
+//   25 / 26           one workgroup per row, dense / word hash in LDS
 //   27                heavy rows: one workgroup per row, bitmap window in a
 //                     global scratch slot
 //   28                heavy rows whose span is too wide for a bitmap slot
@@ -23,12 +24,12 @@ constexpr int kStatFields = 3;  // count, max metric, max words
 
 struct BinParams {
   int32_t ncls;
-  int32_t cls_max[kNumCls];       // metric bound of class c
-  int32_t cls_bm_words[kNumCls];  // dense window if words(span) <= this
-  int32_t wg_max;                 // metric bound of the workgroup bin
-  int32_t wg_bm_words;
-  int32_t gl_bm_words;            // global bitmap if words <= this
+  int32_t cls_max[kNumCls];       // metric bound of class c (products, or row nnz)
+  int32_t cls_bytes[kNumCls];     // LDS bytes one row of class c may use
+  int32_t wg_bytes;               // LDS bytes of a workgroup-per-row bin
+  int32_t gl_bm_words;            // global bitmap if words(span) <= this
   int32_t glen_max[3];            // mean B-row length bounds for G = 8, 16, 32
+  int32_t dense_ratio;            // numeric: dense if words(span) <= max(ratio * distinct words, 64)
 };
 
 inline int bin_group(int bin) { return 8 << ((bin - 1) % 4); }
@@ -39,12 +40,14 @@ inline bool bin_is_bitmap(int bin) { return ((bin - 1) / 4) % 2 == 0; }
 // reference sizing rule); numeric hash slots for a row with c outputs.
 inline int64_t np2_host(int64_t x) { if (x < 2) return 2; int64_t p = 2; while (p < x) p <<= 1; return p; }
 inline int64_t sym_slots(int64_t f) { return np2_host((f * 4) / 3 + 1); }
+// symbolic bin "words" statistic: span words (dense) or key bound (word hash)
 inline int64_t num_slots(int64_t c) { return np2_host((c * 3) / 2 + 1); }
 
 // ---- launchers (all asynchronous on `s`) ----
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
-void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* rmin, int32_t* rmax,
-                     hipStream_t s);
+// B-row metadata {start, end, min column, max column} (min = INT32_MAX,
+// max = -1 for an empty row).
+void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, hipStream_t s);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
                               int32_t* cursor, int32_t* bc, double* bv, hipStream_t s);
@@ -54,13 +57,13 @@ void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, c
 int64_t scan_tmp_elems(int64_t n);
 void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, int64_t* tmp,
                  hipStream_t s);
-void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int32_t* bp,
-                    const int32_t* bmin, const int32_t* bmax, int upper, const BinParams& P,
+void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4* bmeta, int upper,
+                    const BinParams& P,
                     int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin, int32_t* stats,
                     unsigned long long* total_products, hipStream_t s);
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
-                   const int32_t* rowlo, const int32_t* rowhi, const BinParams& P, uint8_t* bin,
-                   int32_t* stats, hipStream_t s);
+                   const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
+                   uint8_t* bin, int32_t* stats, hipStream_t s);
 void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s);
 
 struct RowArgs {
@@ -69,8 +72,13 @@ struct RowArgs {
   const int32_t* ap; const int32_t* ac; const double* av;
   const int32_t* bp; const int32_t* bc; const double* bv;
   const int64_t* rowf; const int32_t* rowlo; const int32_t* rowhi;
+  int32_t* rowwords;     // distinct 32-column words per output row (symbolic writes, numeric reads)
   int upper;
+  int64_t bnnz;          // nnz of B (selects 32- or 64-bit gather offsets)
+  const int4* bmeta;     // per B row: {start, end, min column, max column}
+  uint2* wlist;          // per output row: up to 32 sorted (word, mask) pairs from the symbolic pass
 };
+constexpr int kWordListCap = 32;
 
 // Symbolic: rowcnt[i] = distinct output columns of row i.
 // lds_row_words: u32 words of LDS per row (bitmap words or hash slots).
@@ -98,6 +106,8 @@ void launch_mirror_scatter(int32_t n, const int64_t* uoff, const int32_t* ucol, 
 void launch_segment_sort(int32_t n, const int64_t* off, const int32_t* len, int32_t* col, double* val,
                          int32_t max_len, hipStream_t s);
 
+void set_num_cus(int n);
+void set_lds_per_cu(int64_t b);
 void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s);
 void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s);
 
