@@ -152,7 +152,7 @@ Engine::~Engine() {
   Scratch* all[] = {&ws_.bptr, &ws_.bcol, &ws_.bval, &ws_.cursor, &ws_.bmin, &ws_.bmax, &ws_.rowf,
                     &ws_.rowlo, &ws_.rowhi, &ws_.symbin, &ws_.numbin, &ws_.rowcnt, &ws_.rowoff,
                     &ws_.symrows, &ws_.numrows, &ws_.scan_tmp, &ws_.stats, &ws_.gsym, &ws_.gnum,
-                    &ws_.low, &ws_.fullcnt, &ws_.uoff, &ws_.ucol, &ws_.uval, &ws_.coff, &ws_.rowwords, &ws_.wlist};
+                    &ws_.low, &ws_.fullcnt, &ws_.uoff, &ws_.ucol, &ws_.uval, &ws_.coff, &ws_.rowwords, &ws_.wlist, &ws_.statpart};
   for (Scratch* s : all) free_scratch(*s);
   pool_.trim();
   for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
@@ -225,7 +225,9 @@ void Engine::prepass_stage(const appt_csr_view& A, appt_op op, const int32_t*& b
   launch_fill_i32(dstats, (int64_t)(stat_words * 2 + 4 + kNumBins), 0, s);
   const int upper = op == APPT_OP_AATS ? 1 : 0;
   launch_rowstats(n, bp, bc, bmeta, s);
-  launch_prepass(n, A.rowptr, A.col, bmeta, upper, sym_params, rowf, rowlo, rowhi, symbin, dstats, dprod, s);
+  int32_t* part = as<int32_t>(ensure(ws_.statpart, sizeof(int32_t) * (size_t)stats_part_elems() +
+                                                      sizeof(unsigned long long) * kStatsMaxBlocks));
+  launch_prepass(n, A.rowptr, A.col, bmeta, upper, sym_params, rowf, rowlo, rowhi, symbin, dstats, dprod, part, s);
   APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, sizeof(int32_t) * stat_words, hipMemcpyDeviceToHost, s));
   APPT_HIP(hipMemcpyAsync(&hs_->products, dprod, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
   APPT_HIP(hipStreamSynchronize(s));
@@ -320,7 +322,8 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   // ---- row offsets, numeric binning ----------------------------------------
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)n + 1)));
   launch_scan(rowcnt, n, rowoff, rp32, tmp, s);
-  launch_numbin(n, A.rowptr, rowf, rowcnt, rowwords, rowlo, rowhi, num_params, numbin, dnum, s);
+  launch_numbin(n, A.rowptr, rowf, rowcnt, rowwords, rowlo, rowhi, num_params, numbin, dnum,
+                as<int32_t>(ws_.statpart.p), s);
   APPT_HIP(hipMemcpyAsync(hs_->num, dnum, sizeof(int32_t) * stat_words, hipMemcpyDeviceToHost, s));
   APPT_HIP(hipMemcpyAsync(&hs_->total, rowoff + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
   APPT_HIP(hipStreamSynchronize(s));

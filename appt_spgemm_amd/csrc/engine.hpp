@@ -55,7 +55,7 @@ struct Scratch {
 
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmin, bmax, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff,
-      symrows, numrows, scan_tmp, stats, gsym, gnum, low, fullcnt, uoff, ucol, uval, coff, rowwords, wlist;
+      symrows, numrows, scan_tmp, stats, gsym, gnum, low, fullcnt, uoff, ucol, uval, coff, rowwords, wlist, statpart;
 };
 
 struct Result {  // what appt_csr_result::_priv points to

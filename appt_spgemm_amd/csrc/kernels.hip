@@ -328,25 +328,73 @@ __device__ __forceinline__ void stats_add_wave(BinStatLds& st, int bin, int64_t 
     todo &= ~same;
   }
 }
-__device__ __forceinline__ void stats_flush(BinStatLds& st, int32_t* g) {
+// Per-block statistics go to a field-major partial array part[field * nb + b]
+// (and the block's product count to prod[b]); k_stats_reduce combines them.
+// No same-address global atomics: thousands of blocks updating one counter
+// serialise in the L2.
+__device__ __forceinline__ void stats_flush(BinStatLds& st, int32_t* part, unsigned long long* prod,
+                                           unsigned long long my_prod) {
   __syncthreads();
-  for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) {
-    const int c = st.v[b * kStatFields + 0];
-    if (c) {
-      atomicAdd(&g[b * kStatFields + 0], c);
-      atomicMax(&g[b * kStatFields + 1], st.v[b * kStatFields + 1]);
-      atomicMax(&g[b * kStatFields + 2], st.v[b * kStatFields + 2]);
+  const int nb = gridDim.x;
+  for (int t = threadIdx.x; t < kNumBins * kStatFields; t += blockDim.x) part[(size_t)t * nb + blockIdx.x] = st.v[t];
+  if (prod && threadIdx.x == 0) prod[blockIdx.x] = my_prod;
+}
+
+__global__ void __launch_bounds__(256) k_stats_reduce(const int32_t* __restrict__ part,
+                                                      const unsigned long long* __restrict__ prod, int nb,
+                                                      int32_t* __restrict__ gstats,
+                                                      unsigned long long* __restrict__ total) {
+  __shared__ int64_t red[17];
+  const int f = blockIdx.x;
+  if (f < kNumBins * kStatFields) {
+    const bool is_sum = (f % kStatFields) == 0;
+    int64_t v = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      const int64_t x = part[(size_t)f * nb + b];
+      v = is_sum ? v + x : (x > v ? x : v);
+    }
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) {
+      const int64_t o = __shfl_xor(v, m, 64);
+      v = is_sum ? v + o : (o > v ? o : v);
+    }
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t r = 0;
+      for (int w = 0; w < (int)blockDim.x / kWave; w++) r = is_sum ? r + red[w] : (red[w] > r ? red[w] : r);
+      gstats[f] = (int32_t)(r > INT32_MAX ? INT32_MAX : r);
+    }
+  } else if (prod) {
+    unsigned long long v = 0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) v += prod[b];
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1) v += __shfl_xor(v, m, 64);
+    if (threadIdx.x % kWave == 0) red[threadIdx.x / kWave] = (int64_t)v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long r = 0;
+      for (int w = 0; w < (int)blockDim.x / kWave; w++) r += (unsigned long long)red[w];
+      *total = r;
     }
   }
 }
+
+static void stats_reduce(const int32_t* part, const unsigned long long* prod, int nb, int32_t* gstats,
+                         unsigned long long* total, hipStream_t s) {
+  const int fields = kNumBins * kStatFields + (prod ? 1 : 0);
+  hipLaunchKernelGGL(k_stats_reduce, dim3(fields), dim3(256), 0, s, part, prod, nb, gstats, total);
+}
+
+int64_t stats_part_elems() { return (int64_t)kNumBins * kStatFields * kStatsMaxBlocks; }
 
 // Structural pre-pass: products f_i (k_flops of the reference), output span
 // [lo, hi] from the B rows' min/max columns, symbolic bin, bin statistics.
 __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
                           const int4* __restrict__ bmeta, int upper, BinParams P,
                           int64_t* __restrict__ rowf, int32_t* __restrict__ rowlo, int32_t* __restrict__ rowhi,
-                          uint8_t* __restrict__ bin, int32_t* __restrict__ gstats,
-                          unsigned long long* __restrict__ total_products) {
+                          uint8_t* __restrict__ bin, int32_t* __restrict__ part,
+                          unsigned long long* __restrict__ prodpart) {
   constexpr int G = 16;
   __shared__ BinStatLds st;
   __shared__ unsigned long long red[16];
@@ -402,28 +450,28 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) myprod += __shfl_xor(myprod, m, 64);
   if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = myprod;
-  stats_flush(st, gstats);
-  if (threadIdx.x == 0) {
-    unsigned long long t = 0;
-    for (int w = 0; w < (int)blockDim.x / 64; w++) t += red[w];
-    if (t) atomicAdd(total_products, t);
-  }
+  __syncthreads();
+  unsigned long long t = 0;
+  for (int w = 0; w < (int)blockDim.x / 64; w++) t += red[w];
+  stats_flush(st, part, prodpart, t);
 }
 void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4* bmeta, int upper,
                     const BinParams& P, int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin,
-                    int32_t* stats, unsigned long long* total_products, hipStream_t s) {
+                    int32_t* stats, unsigned long long* total_products, int32_t* part, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
+  unsigned long long* prodpart = reinterpret_cast<unsigned long long*>(part + stats_part_elems());
   hipLaunchKernelGGL(k_prepass, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, bmeta, upper, P, rowf, rowlo,
-                     rowhi, bin, stats, total_products);
+                     rowhi, bin, part, prodpart);
+  stats_reduce(part, prodpart, (int)blocks, stats, total_products, s);
 }
 
 // Numeric bin of each row from its exact output count.
 __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_t* __restrict__ rowf,
                          const int32_t* __restrict__ rowcnt, const int32_t* __restrict__ rowwords,
                          const int32_t* __restrict__ rowlo, const int32_t* __restrict__ rowhi, BinParams P,
-                         uint8_t* __restrict__ bin, int32_t* __restrict__ gstats) {
+                         uint8_t* __restrict__ bin, int32_t* __restrict__ part) {
   __shared__ BinStatLds st;
   stats_init(st);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -441,16 +489,17 @@ __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_
     }
     stats_add_wave(st, b, c, stat);
   }
-  stats_flush(st, gstats);
+  stats_flush(st, part, nullptr, 0ull);
 }
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
                    const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
-                   uint8_t* bin, int32_t* stats, hipStream_t s) {
+                   uint8_t* bin, int32_t* stats, int32_t* part, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
+  if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   hipLaunchKernelGGL(k_numbin, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, rowf, rowcnt, rowwords, rowlo,
-                     rowhi, P, bin, stats);
+                     rowhi, P, bin, part);
+  stats_reduce(part, nullptr, (int)blocks, stats, nullptr, s);
 }
 
 // Partition rows by bin, keeping ascending row order inside each block tile
@@ -1035,26 +1084,34 @@ __device__ __forceinline__ bool stage_entries(int base, int e, int lane, const R
   return group_bits<G>(__ballot(v.y - v.x > G)) != 0ull;
 }
 
-// Software pipeline over a group's rows: the next row's metadata chain
-// rows[r] -> ap[i] -> ac[p] -> bmeta[k] is issued in four steps spread over
-// the current row's phases, so each dependent load is waited for behind work
-// that was already in flight.  The row's first G A entries come from these
-// registers; longer A rows stage further chunks inline.
-struct NextRow {
-  int i, s, e;
-  int2 be;     // this lane's first-chunk entry: B-row start / end
-  double a;    // this lane's first-chunk a_ik (value pass only)
+// Software pipeline over a group's rows.  A row's metadata is a chain of
+// dependent loads: rows[r] -> (ap[i], rowlo[i], row head) -> ac[p] ->
+// (bmeta[k], av[p]).  At the top of every row iteration, hop h of that chain
+// is issued for row r + h*R (h = 1..4) from values requested one iteration
+// earlier, so no load waits on another load issued in the same iteration.
+struct RowState {
+  int i, s, e, lo;
+  int k;       // this lane's first-chunk A column (-1: none)
+  int2 be;     // B-row start / end of that entry
+  double a;    // a_ik of that entry (value pass only)
 };
 
-template <int G, bool VALS>
-__device__ __forceinline__ void first_chunk_load(NextRow& nx, int lane, const RowArgs& A) {
-  const int p = nx.s + lane;
-  nx.be = make_int2(0, 0);
-  nx.a = 0.0;
-  if (p < nx.e) {
-    const int4 bm = A.bmeta[A.ac[p]];
-    nx.be = make_int2(bm.x, bm.y);
-    if (VALS) nx.a = A.av[p];
+__device__ __forceinline__ void hop_scalars(RowState& st, const RowArgs& A) {
+  st.s = A.ap[st.i];
+  st.e = A.ap[st.i + 1];
+  st.lo = A.rowlo[st.i];
+}
+__device__ __forceinline__ void hop_entry(RowState& st, int lane, const RowArgs& A) {
+  st.k = (st.s + lane < st.e) ? A.ac[st.s + lane] : -1;
+}
+template <bool VALS>
+__device__ __forceinline__ void hop_bmeta(RowState& st, int lane, const RowArgs& A) {
+  st.be = make_int2(0, 0);
+  st.a = 0.0;
+  if (st.k >= 0) {
+    const int4 bm = A.bmeta[st.k];
+    st.be = make_int2(bm.x, bm.y);
+    if (VALS) st.a = A.av[st.s + lane];
   }
 }
 
@@ -1069,7 +1126,8 @@ __device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool ln
   auto apply = [&](int j, bool ok) {
     const bool use = ok && (!UPPER || j >= i);
     const uint32_t b = (uint32_t)(j - lo);
-    const uint32_t old = atomicOr(&bits[use ? (b >> 5) : (uint32_t)(W + lane)], 1u << (b & 31u));
+    const uint32_t wd = (b >> 5) < (uint32_t)W ? (b >> 5) : (uint32_t)W;  // in range by construction
+    const uint32_t old = atomicOr(&bits[use ? wd : (uint32_t)(W + lane)], 1u << (b & 31u));
     if (TRACK && __builtin_expect(use && old == 0u, 0)) {
       const int idx = atomicAdd(&ctl[0], 1);
       if (idx < kTouchCap) touched[idx] = b >> 5;
@@ -1105,14 +1163,17 @@ __device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool ln
 template <int G, bool UPPER, bool W64>
 __device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool lng, const RowArgs& A,
                                             const int2* mbe, const double* ma, const uint32_t* bits,
-                                            const uint16_t* pre, double* vals, int Cm) {
-  constexpr int B = G < 32 ? G : 32;
+                                            const uint16_t* pre, double* vals, int Cm, int W) {
+  constexpr int B = G < 16 ? G : 16;
   auto add = [&](int j, double prod, bool ok) {
     const bool use = ok && (!UPPER || j >= i);
     const uint32_t bb = use ? (uint32_t)(j - lo) : 0u;
-    const uint32_t w = bb >> 5;
+    const uint32_t w = (bb >> 5) < (uint32_t)W ? (bb >> 5) : 0u;  // in range by construction
     const uint32_t pos = (uint32_t)pre[w] + __popc(bits[w] & ((1u << (bb & 31u)) - 1u));
-    atomicAdd(&vals[use ? pos : (uint32_t)(Cm + lane)], prod);
+    // positions are < c <= Cm by construction; the clamp keeps a damaged
+    // input from writing outside this group's LDS region
+    const uint32_t slot = use ? (pos < (uint32_t)Cm ? pos : (uint32_t)Cm) : (uint32_t)(Cm + lane);
+    atomicAdd(&vals[slot], prod);
   };
   if (!lng) {
     for (int t0 = 0; t0 < m; t0 += B) {
@@ -1162,38 +1223,38 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
   const int r0 = lb * chunk;
   const int r1 = (r0 + chunk) < A.nrows ? (r0 + chunk) : A.nrows;
   int r = r0 + gi;
-  NextRow cur{0, 0, 0, make_int2(0, 0), 0.0};
-  if (r < r1) {
-    cur.i = A.rows[r];
-    cur.s = A.ap[cur.i];
-    cur.e = A.ap[cur.i + 1];
-    first_chunk_load<G, false>(cur, lane, A);
-  }
+  // pipeline prologue: rows r .. r+3R filled to hop depths 4, 3, 2, 1
+  RowState q0{}, q1{}, q2{}, q3{};
+  q0.k = q1.k = q2.k = q3.k = -1;
+  if (r < r1) { q0.i = A.rows[r]; hop_scalars(q0, A); hop_entry(q0, lane, A); hop_bmeta<false>(q0, lane, A); }
+  if (r + R < r1) { q1.i = A.rows[r + R]; hop_scalars(q1, A); hop_entry(q1, lane, A); }
+  if (r + 2 * R < r1) { q2.i = A.rows[r + 2 * R]; hop_scalars(q2, A); }
+  if (r + 3 * R < r1) q3.i = A.rows[r + 3 * R];
   gsync<G>();
   for (; r < r1; r += R) {
-    const int rn = r + R;
-    const bool hasn = rn < r1;
-    NextRow nx{0, 0, 0, make_int2(0, 0), 0.0};
-    if (hasn) nx.i = A.rows[rn];                                   // prefetch step 1
-    const int i = cur.i;
-    const int lo = A.rowlo[i];
+    // hops for the rows ahead (each uses values requested one iteration ago)
+    RowState q4{};
+    q4.k = -1;
+    if (r + 4 * R < r1) q4.i = A.rows[r + 4 * R];
+    if (r + 3 * R < r1) hop_scalars(q3, A);
+    if (r + 2 * R < r1) hop_entry(q2, lane, A);
+    if (r + R < r1) hop_bmeta<false>(q1, lane, A);
+    const int i = q0.i;
+    const int lo = q0.lo;
     if (lane == 0) ctl[0] = 0;
-    mbe[lane] = cur.be;
-    bool lng = group_bits<G>(__ballot(cur.be.y - cur.be.x > G)) != 0ull;
+    mbe[lane] = q0.be;
+    bool lng = group_bits<G>(__ballot(q0.be.y - q0.be.x > G)) != 0ull;
     gsync<G>();
-    int m = (cur.e - cur.s) < G ? (cur.e - cur.s) : G;
-    if (hasn) { nx.s = A.ap[nx.i]; nx.e = A.ap[nx.i + 1]; }         // prefetch step 2
+    int m = (q0.e - q0.s) < G ? (q0.e - q0.s) : G;
     or_chunk<G, UPPER, W64, true>(i, lo, lane, m, lng, A, mbe, bits, W, touched, ctl);
     gsync<G>();
-    for (int b0 = cur.s + G; b0 < cur.e; b0 += G) {                // long A rows
-      const bool l2 = stage_entries<G, false>(b0, cur.e, lane, A, mbe, nullptr);
+    for (int b0 = q0.s + G; b0 < q0.e; b0 += G) {                  // long A rows
+      const bool l2 = stage_entries<G, false>(b0, q0.e, lane, A, mbe, nullptr);
       gsync<G>();
-      m = (cur.e - b0) < G ? (cur.e - b0) : G;
+      m = (q0.e - b0) < G ? (q0.e - b0) : G;
       or_chunk<G, UPPER, W64, true>(i, lo, lane, m, l2, A, mbe, bits, W, touched, ctl);
       gsync<G>();
     }
-    int nk = -1;
-    if (hasn && nx.s + lane < nx.e) nk = A.ac[nx.s + lane];          // prefetch step 3
     const int nt = ctl[0];
     int c = 0;
     if (nt <= kTouchCap) {
@@ -1228,8 +1289,7 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
     }
     c = gsum<G, int32_t>(c, nullptr);
     if (lane == 0) { rowcnt[i] = c; A.rowwords[i] = nt; }
-    if (nk >= 0) { const int4 bm = A.bmeta[nk]; nx.be = make_int2(bm.x, bm.y); }   // prefetch step 4
-    cur = nx;
+    q0 = q1; q1 = q2; q2 = q3; q3 = q4;
     gsync<G>();
   }
 }
@@ -1253,85 +1313,92 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
   const int r0 = lb * chunk;
   const int r1 = (r0 + chunk) < A.nrows ? (r0 + chunk) : A.nrows;
   constexpr int PL = (kWordCap + G - 1) / G;
+  // per-row output head: offsets, word count, sorted word list
+  struct Head {
+    int64_t off, end;
+    int rw;
+    uint2 ent[PL];
+  };
+  auto load_head = [&](int i, Head& h) {
+    h.off = rowoff[i];
+    h.end = rowoff[i + 1];
+    h.rw = A.rowwords[i];
+    const uint2* wl = A.wlist + (size_t)i * kWordCap;
+#pragma unroll
+    for (int k = 0; k < PL; k++)   // read without checking the count; used only if valid
+      h.ent[k] = (lane + k * G) < kWordCap ? wl[lane + k * G] : make_uint2(0u, 0u);
+  };
   int r = r0 + gi;
-  NextRow cur{0, 0, 0, make_int2(0, 0), 0.0};
+  // pipeline prologue: rows r .. r+3R filled to hop depths 4, 3, 2, 1
+  RowState q0{}, q1{}, q2{}, q3{};
+  Head h0{}, h1{}, h2{}, h3{};
+  q0.k = q1.k = q2.k = q3.k = -1;
   if (r < r1) {
-    cur.i = A.rows[r];
-    cur.s = A.ap[cur.i];
-    cur.e = A.ap[cur.i + 1];
-    first_chunk_load<G, true>(cur, lane, A);
+    q0.i = A.rows[r]; hop_scalars(q0, A); load_head(q0.i, h0); hop_entry(q0, lane, A); hop_bmeta<true>(q0, lane, A);
   }
+  if (r + R < r1) { q1.i = A.rows[r + R]; hop_scalars(q1, A); load_head(q1.i, h1); hop_entry(q1, lane, A); }
+  if (r + 2 * R < r1) { q2.i = A.rows[r + 2 * R]; hop_scalars(q2, A); load_head(q2.i, h2); }
+  if (r + 3 * R < r1) q3.i = A.rows[r + 3 * R];
   gsync<G>();
   for (; r < r1; r += R) {
-    const int rn = r + R;
-    const bool hasn = rn < r1;
-    NextRow nx{0, 0, 0, make_int2(0, 0), 0.0};
-    if (hasn) nx.i = A.rows[rn];                                   // prefetch step 1
-    const int i = cur.i;
-    const int lo = A.rowlo[i];
-    const int64_t off = rowoff[i];
-    const int c = (int)(rowoff[i + 1] - off);
-    const int rw = A.rowwords[i];
-    const int nt = rw & kWordsMask;
-    const bool have_list = (rw & kNoWordList) == 0 && nt <= kWordCap;
+    // hops for the rows ahead (each uses values requested one iteration ago)
+    RowState q4{};
+    q4.k = -1;
+    if (r + 4 * R < r1) q4.i = A.rows[r + 4 * R];
+    if (r + 3 * R < r1) { hop_scalars(q3, A); load_head(q3.i, h3); }
+    if (r + 2 * R < r1) hop_entry(q2, lane, A);
+    if (r + R < r1) hop_bmeta<true>(q1, lane, A);
+    const int i = q0.i;
+    const int lo = q0.lo;
+    const int64_t off = h0.off;
+    const int c = (int)(h0.end - h0.off);
+    const int nt = h0.rw & kWordsMask;
+    const bool have_list = (h0.rw & kNoWordList) == 0 && nt <= kWordCap;
     uint32_t myw[PL];
-    uint2 ent[PL];
-    if (have_list) {
-      const uint2* wl = A.wlist + (size_t)i * kWordCap;
-#pragma unroll
-      for (int k = 0; k < PL; k++) {
-        const int u = lane + k * G;
-        ent[k] = u < nt ? wl[u] : make_uint2(0u, 0u);
-      }
-    }
     for (int t = lane; t < c; t += G) vals[t] = 0.0;
-    mbe[lane] = cur.be;
-    ma[lane] = cur.a;
-    const bool lng = group_bits<G>(__ballot(cur.be.y - cur.be.x > G)) != 0ull;
+    mbe[lane] = q0.be;
+    ma[lane] = q0.a;
+    const bool lng = group_bits<G>(__ballot(q0.be.y - q0.be.x > G)) != 0ull;
     if (have_list) {
       // the symbolic pass left this row's words sorted: place them in the
-      // window with their prefix popcounts and emit the column list
-      int local = 0;
-#pragma unroll
-      for (int k = 0; k < PL; k++) local += __popc(ent[k].y);
-      // entries are strided over lanes (u = lane + k*G): scan in that order
+      // window with their prefix popcounts and emit the column list; the
+      // entries are strided over lanes (u = lane + k*G), scanned in that order
       int run = 0;
 #pragma unroll
       for (int k = 0; k < PL; k++) {
-        int tot;
-        const int pc = __popc(ent[k].y);
-        const int ex = gscan_excl<G, int>(pc, lane, &tot);
-        int rr = run + ex;
         const int u = lane + k * G;
-        if (u < nt) {
-          bits[ent[k].x] = ent[k].y;
-          pre[ent[k].x] = (uint16_t)rr;
-          uint32_t x = ent[k].y;
-          while (x) {
-            cols[rr++] = lo + (int)ent[k].x * 32 + __builtin_ctz(x);
+        const uint2 e = u < nt ? h0.ent[k] : make_uint2(0u, 0u);
+        int tot;
+        const int ex = gscan_excl<G, int>(__popc(e.y), lane, &tot);
+        int rr = run + ex;
+        if (u < nt && e.x < (uint32_t)W) {   // in range by construction
+          bits[e.x] = e.y;
+          pre[e.x] = (uint16_t)rr;
+          uint32_t x = e.y;
+          while (x && rr < Cm) {
+            cols[rr++] = lo + (int)e.x * 32 + __builtin_ctz(x);
             x &= x - 1u;
           }
         }
-        myw[k] = ent[k].x;
+        myw[k] = e.x;
         run += tot;
       }
-      (void)local;
     } else {
       gsync<G>();
-      const int m0 = (cur.e - cur.s) < G ? (cur.e - cur.s) : G;
+      const int m0 = (q0.e - q0.s) < G ? (q0.e - q0.s) : G;
       or_chunk<G, UPPER, W64, false>(i, lo, lane, m0, lng, A, mbe, bits, W, nullptr, nullptr);
       gsync<G>();
-      for (int b0 = cur.s + G; b0 < cur.e; b0 += G) {
-        const bool l2 = stage_entries<G, false>(b0, cur.e, lane, A, mbe, nullptr);
+      for (int b0 = q0.s + G; b0 < q0.e; b0 += G) {
+        const bool l2 = stage_entries<G, false>(b0, q0.e, lane, A, mbe, nullptr);
         gsync<G>();
-        const int m = (cur.e - b0) < G ? (cur.e - b0) : G;
+        const int m = (q0.e - b0) < G ? (q0.e - b0) : G;
         or_chunk<G, UPPER, W64, false>(i, lo, lane, m, l2, A, mbe, bits, W, nullptr, nullptr);
         gsync<G>();
       }
       const int words = (int)span_words(lo, A.rowhi[i]);
-      const int ch = (words + G - 1) / G;
-      const int w0 = lane * ch;
-      const int w1 = (w0 + ch) < words ? (w0 + ch) : words;
+      const int chw = (words + G - 1) / G;
+      const int w0 = lane * chw;
+      const int w1 = (w0 + chw) < words ? (w0 + chw) : words;
       int local = 0;
       for (int w = w0; w < w1; w++) local += __popc(bits[w]);
       int tot;
@@ -1345,25 +1412,24 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
         }
       }
       // restage the first chunk (the structure pass reused mbe)
-      mbe[lane] = cur.be;
-      ma[lane] = cur.a;
+      mbe[lane] = q0.be;
+      ma[lane] = q0.a;
+#pragma unroll
+      for (int k = 0; k < PL; k++) myw[k] = 0u;
     }
     gsync<G>();
-    if (hasn) { nx.s = A.ap[nx.i]; nx.e = A.ap[nx.i + 1]; }         // prefetch step 2
     {
-      const int m0 = (cur.e - cur.s) < G ? (cur.e - cur.s) : G;
-      value_chunk<G, UPPER, W64>(i, lo, lane, m0, lng, A, mbe, ma, bits, pre, vals, Cm);
+      const int m0 = (q0.e - q0.s) < G ? (q0.e - q0.s) : G;
+      value_chunk<G, UPPER, W64>(i, lo, lane, m0, lng, A, mbe, ma, bits, pre, vals, Cm, W);
       gsync<G>();
-      for (int b0 = cur.s + G; b0 < cur.e; b0 += G) {              // long A rows
-        const bool l2 = stage_entries<G, true>(b0, cur.e, lane, A, mbe, ma);
+      for (int b0 = q0.s + G; b0 < q0.e; b0 += G) {                // long A rows
+        const bool l2 = stage_entries<G, true>(b0, q0.e, lane, A, mbe, ma);
         gsync<G>();
-        const int m = (cur.e - b0) < G ? (cur.e - b0) : G;
-        value_chunk<G, UPPER, W64>(i, lo, lane, m, l2, A, mbe, ma, bits, pre, vals, Cm);
+        const int m = (q0.e - b0) < G ? (q0.e - b0) : G;
+        value_chunk<G, UPPER, W64>(i, lo, lane, m, l2, A, mbe, ma, bits, pre, vals, Cm, W);
         gsync<G>();
       }
     }
-    int nk = -1;
-    if (hasn && nx.s + lane < nx.e) nk = A.ac[nx.s + lane];          // prefetch step 3
     for (int t = lane; t < c; t += G) {
       ccol[off + t] = cols[t];
       cval[off + t] = vals[t];
@@ -1371,17 +1437,13 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
     if (have_list) {
 #pragma unroll
       for (int k = 0; k < PL; k++)
-        if (lane + k * G < nt) bits[myw[k]] = 0u;
+        if (lane + k * G < nt && myw[k] < (uint32_t)W) bits[myw[k]] = 0u;
     } else {
       const int words = (int)span_words(lo, A.rowhi[i]);
       for (int w = lane; w < words; w += G) bits[w] = 0u;
     }
-    if (nk >= 0) {                                                  // prefetch step 4
-      const int4 bm = A.bmeta[nk];
-      nx.be = make_int2(bm.x, bm.y);
-      nx.a = A.av[nx.s + lane];
-    }
-    cur = nx;
+    q0 = q1; q1 = q2; q2 = q3; q3 = q4;
+    h0 = h1; h1 = h2; h2 = h3;
     gsync<G>();
   }
 }

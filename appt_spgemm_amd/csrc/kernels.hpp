@@ -21,6 +21,9 @@ constexpr int kNumCls = 3;
 constexpr int kNumBins = 29;
 constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinGlBitmap = 27, kBinGlHash = 28;
 constexpr int kStatFields = 3;  // count, max metric, max words
+constexpr int kStatsMaxBlocks = 4096;
+// int32 elements of the per-block statistics scratch (plus u64 product counts)
+int64_t stats_part_elems();
 
 struct BinParams {
   int32_t ncls;
@@ -60,10 +63,10 @@ void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, i
 void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4* bmeta, int upper,
                     const BinParams& P,
                     int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin, int32_t* stats,
-                    unsigned long long* total_products, hipStream_t s);
+                    unsigned long long* total_products, int32_t* part, hipStream_t s);
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
                    const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
-                   uint8_t* bin, int32_t* stats, hipStream_t s);
+                   uint8_t* bin, int32_t* stats, int32_t* part, hipStream_t s);
 void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s);
 
 struct RowArgs {
