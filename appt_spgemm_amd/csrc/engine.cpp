@@ -9,9 +9,10 @@
 //   exclusive_scan -> Coff (:168)          scan to int64 offsets (+ int32 copy)
 //   k_numeric (:189)                       one launch per non-empty bin, rows
 //                                          written sorted straight into CSR
-//   k_mirror + radix sort (:190-196)       lower counts, placement, per-row sort
+//   k_mirror + radix sort (:190-196)       full-row structure, values for j >= i,
+//                                          lower entries pulled from the upper part
 //   cub::DeviceRadixSort (:195)            not needed: rows come out sorted
-// Host synchronisations: 2 for AA/AAt (bin sizes; nnz(C)), 3 for AAtS.
+// Host synchronisations: 2 (bin sizes; nnz(C)).
 #include "engine.hpp"
 
 #include <algorithm>
@@ -304,7 +305,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     nbins_used++;
     const int32_t mmax = hs_->sym[b * kStatFields + 1];
     const int32_t wmax = hs_->sym[b * kStatFields + 2];
-    RowArgs ra{symrows + symoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, upper, A.nnz, bmeta, wlist};
+    RowArgs ra{symrows + symoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, 0, A.nnz, bmeta, wlist};
     if (b == kBinGlBitmap) {
       const int64_t slot_bytes = (int64_t)wmax * 4;
       int64_t slots = std::min<int64_t>(cnt, gslots_);
@@ -347,18 +348,12 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   launch_bin_scatter(n, numbin, dcursor, numrows, s);
   record(4);
 
-  // output (or, for AAtS, the upper triangle U) buffers
-  int32_t* ucol;
-  double* uval;
-  if (upper) {
-    ucol = as<int32_t>(ensure(ws_.ucol, sizeof(int32_t) * (size_t)std::max<int64_t>(unnz, 1)));
-    uval = as<double>(ensure(ws_.uval, sizeof(double) * (size_t)std::max<int64_t>(unnz, 1)));
-  } else {
-    ucol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(unnz, 1)));
-    uval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(unnz, 1)));
-  }
+  // output buffers (the symbolic pass sized C exactly)
+  int32_t* ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(unnz, 1)));
+  double* cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(unnz, 1)));
 
   // ---- numeric --------------------------------------------------------------
+  // AAtS: values are formed for j >= i only; the structure is the full row
   for (int b = 1; b < kNumBins; b++) {
     const int32_t cnt = hs_->num[b * kStatFields];
     if (cnt == 0) continue;
@@ -370,47 +365,20 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
       int64_t slots = std::min<int64_t>(cnt, gslots_);
       slots = std::max<int64_t>(1, std::min<int64_t>(slots, ((int64_t)1 << 30) / std::max<int64_t>(slot_bytes, 1)));
       uint32_t* g = as<uint32_t>(ensure(ws_.gnum, (size_t)(slots * slot_bytes)));
-      launch_numeric_bin(b, ra, wmax, cmax, rowoff, ucol, uval, g, (int32_t)slots, s);
+      launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, g, (int32_t)slots, s);
     } else {
       const int64_t rb = numeric_row_lds_bytes(b, cmax, wmax);
       if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "numeric row exceeds LDS");
-      launch_numeric_bin(b, ra, wmax, cmax, rowoff, ucol, uval, nullptr, 0, s);
+      launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, nullptr, 0, s);
     }
   }
   record(5);
 
-  int64_t cnnz = unnz;
+  // ---- AAtS: mirror the upper triangle into the lower entries -------------
+  if (upper) launch_mirror_pull(n, rowoff, ccol, cval, s);
+  const int64_t cnnz = unnz;
   void* rowptr_out = rp32;
-  int32_t* ccol = ucol;
-  double* cval = uval;
-  if (upper) {
-    // ---- mirror the upper triangle ----------------------------------------
-    int32_t* low = as<int32_t>(ensure(ws_.low, sizeof(int32_t) * (size_t)std::max(n, 1)));
-    int32_t* full = as<int32_t>(ensure(ws_.fullcnt, sizeof(int32_t) * (size_t)std::max(n, 1)));
-    int64_t* uoff = as<int64_t>(ensure(ws_.uoff, sizeof(int64_t) * ((size_t)n + 1)));
-    int64_t* coff = as<int64_t>(ensure(ws_.coff, sizeof(int64_t) * ((size_t)n + 1)));
-    int32_t* cur = as<int32_t>(ensure(ws_.cursor, sizeof(int32_t) * ((size_t)n + 1)));
-    launch_copy_i64(rowoff, (int64_t)n + 1, uoff, s);
-    launch_fill_i32(low, n, 0, s);
-    launch_lower_count(n, uoff, ucol, low, s);
-    launch_add_counts(n, low, uoff, full, s);
-    launch_scan(full, n, coff, rp32, tmp, s);
-    APPT_HIP(hipMemcpyAsync(&hs_->total, coff + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-    APPT_HIP(hipStreamSynchronize(s));
-    syncs++;
-    cnnz = hs_->total;
-    ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(cnnz, 1)));
-    cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(cnnz, 1)));
-    launch_fill_i32(cur, n, 0, s);
-    launch_mirror_scatter(n, uoff, ucol, uval, coff, low, cur, ccol, cval, s);
-    launch_segment_sort(n, coff, low, ccol, cval, n, s);
-    if (cnnz > INT32_MAX) {
-      pool_.put(rp32);
-      int64_t* rp64 = as<int64_t>(pool_.get(sizeof(int64_t) * ((size_t)n + 1)));
-      launch_copy_i64(coff, (int64_t)n + 1, rp64, s);
-      rowptr_out = rp64;
-    }
-  } else if (cnnz > INT32_MAX) {
+  if (cnnz > INT32_MAX) {
     pool_.put(rp32);
     int64_t* rp64 = as<int64_t>(pool_.get(sizeof(int64_t) * ((size_t)n + 1)));
     launch_copy_i64(rowoff, (int64_t)n + 1, rp64, s);

@@ -428,7 +428,7 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     f = gsum<G, int64_t>(f, nullptr);
     lo = gmin<G>(lo);
     hi = gmax<G>(hi);
-    if (upper && lo < (int)i) lo = (int)i;
+    (void)upper;
     // bitmap windows start on a 32-column boundary, so a window word and a
     // word-hash key (j >> 5) cover the same columns and distinct-word counts
     // agree between accumulator modes
@@ -574,6 +574,8 @@ struct Stage {
   int32_t bs[256];
   int32_t be[256];
   double a[256];
+  int32_t longs[256];
+  int32_t nlong;
 };
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
@@ -632,14 +634,26 @@ __device__ __forceinline__ void for_each_product(int i, int lane, const RowArgs&
         }
       }
     } else {
+      // whole workgroup per row; the order of contributions is not fixed
+      // here (atomics), so products are balanced over the lanes: B rows of
+      // at most kShortRow entries are walked one lane per A entry, longer
+      // ones with all lanes over the B row
+      constexpr int kShortRow = 32;
       __syncthreads();
       stage->bs[lane] = bs;
       stage->be[lane] = be;
       if (VALS) stage->a[lane] = a;
+      if (lane == 0) stage->nlong = 0;
       __syncthreads();
-      for (int t = 0; t < m; t++) {
-        const int tbs = stage->bs[t], tbe = stage->be[t];
-        const double ta = VALS ? stage->a[t] : 0.0;
+      if (be - bs > kShortRow) stage->longs[atomicAdd(&stage->nlong, 1)] = lane;
+      else
+        for (int q = bs; q < be; q++) fn(A.bc[q], VALS ? a * A.bv[q] : 0.0);
+      __syncthreads();
+      const int nl = stage->nlong;
+      for (int t = 0; t < nl; t++) {
+        const int src = stage->longs[t];
+        const int tbs = stage->bs[src], tbe = stage->be[src];
+        const double ta = VALS ? stage->a[src] : 0.0;
         for (int q = tbs + lane; q < tbe; q += G) fn(A.bc[q], VALS ? ta * A.bv[q] : 0.0);
       }
     }
@@ -701,7 +715,6 @@ __global__ void __launch_bounds__(256) k_symbolic_lds(RowArgs A, int32_t row_wor
     for (int t = lane; t < words; t += G) tab[t] = 0u;
     gsync<G>();
     for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
-      if (A.upper && j < i) return;
       const uint32_t b = (uint32_t)(j - lo);
       const uint32_t bit = 1u << (b & 31u);
       const uint32_t old = atomicOr(&tab[b >> 5], bit);
@@ -718,7 +731,6 @@ __global__ void __launch_bounds__(256) k_symbolic_lds(RowArgs A, int32_t row_wor
     for (int t = lane; t < (int)T; t += G) { keys[t] = -1; masks[t] = 0u; }
     gsync<G>();
     for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
-      if (A.upper && j < i) return;
       bool claimed;
       const uint32_t h = whash_insert(keys, logT, mask, j >> 5, &claimed);
       nw += claimed ? 1 : 0;
@@ -750,7 +762,6 @@ __global__ void __launch_bounds__(256) k_symbolic_global(RowArgs A, int32_t slot
     __syncthreads();
     int cnt = 0, nw = 0;
     for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
-      if (A.upper && j < i) return;
       const uint32_t b = (uint32_t)(j - lo);
       const uint32_t bit = 1u << (b & 31u);
       const uint32_t old = atomicOr(&tab[b >> 5], bit);
@@ -832,7 +843,6 @@ __device__ __forceinline__ void numeric_dense_row(int i, int lane, const RowArgs
   gsync<G>();
   uint32_t* bits = (uint32_t*)info;
   for_each_product<G, false>(i, lane, A, stage, [&](int j, double) {
-    if (A.upper && j < i) return;
     const uint32_t b = (uint32_t)(j - lo);
     atomicOr(&bits[(b >> 5) * 2], 1u << (b & 31u));
   });
@@ -884,7 +894,6 @@ __device__ __forceinline__ void numeric_whash_row(int i, int lane, const RowArgs
   gsync<G>();
   uint32_t* hbits = (uint32_t*)hinfo;
   for_each_product<G, false>(i, lane, A, stage, [&](int j, double) {
-    if (A.upper && j < i) return;
     bool claimed;
     const uint32_t h = whash_insert(hkey, logT, mask, j >> 5, &claimed);
     atomicOr(&hbits[h * 2], 1u << (j & 31));
@@ -1015,7 +1024,6 @@ __global__ void __launch_bounds__(256) k_numeric_global(RowArgs A, int32_t slot_
     __threadfence();
     __syncthreads();
     for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
-      if (A.upper && j < i) return;
       const uint32_t b = (uint32_t)(j - lo);
       atomicOr(&bits[(b >> 5) * 2], 1u << (b & 31u));
     });
@@ -1124,7 +1132,7 @@ __device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool ln
                                          const int2* mbe, uint32_t* bits, int W, uint32_t* touched, int* ctl) {
   constexpr int B = G < 32 ? G : 32;
   auto apply = [&](int j, bool ok) {
-    const bool use = ok && (!UPPER || j >= i);
+    const bool use = ok;
     const uint32_t b = (uint32_t)(j - lo);
     const uint32_t wd = (b >> 5) < (uint32_t)W ? (b >> 5) : (uint32_t)W;  // in range by construction
     const uint32_t old = atomicOr(&bits[use ? wd : (uint32_t)(W + lane)], 1u << (b & 31u));
@@ -1622,141 +1630,40 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
 }
 
 // ---------------------------------------------------------------------------
-// AAtS mirror: lower counts, placement, per-row sort of the lower segments
+// AAtS mirror.  C's structure is symmetric and every row is already complete
+// and sorted; the numeric pass formed values only for j >= i.  Each lower
+// entry (c, r), r < c, takes the value of (r, c) from row r's upper part,
+// found by binary search (columns are ascending).  No atomics, no sort.
 // ---------------------------------------------------------------------------
-__global__ void k_lower_count(int32_t n, const int64_t* __restrict__ uoff, const int32_t* __restrict__ ucol,
-                              int32_t* __restrict__ low) {
+__global__ void k_mirror_pull(int32_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ ccol,
+                              double* __restrict__ cval) {
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
-  for (int64_t r = (int64_t)blockIdx.x * gpb + threadIdx.x / G; r < n; r += (int64_t)gridDim.x * gpb) {
-    for (int64_t t = uoff[r] + lane; t < uoff[r + 1]; t += G) {
-      const int c = ucol[t];
-      if (c > r) atomicAdd(&low[c], 1);
-    }
-  }
-}
-void launch_lower_count(int32_t n, const int64_t* uoff, const int32_t* ucol, int32_t* low, hipStream_t s) {
-  if (n <= 0) return;
-  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_lower_count, dim3((unsigned)blocks), dim3(256), 0, s, n, uoff, ucol, low);
-}
-
-__global__ void k_add_counts(int32_t n, const int32_t* __restrict__ a, const int64_t* __restrict__ boff,
-                             int32_t* __restrict__ out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    out[i] = a[i] + (int32_t)(boff[i + 1] - boff[i]);
-}
-void launch_add_counts(int32_t n, const int32_t* a, const int64_t* boff, int32_t* out, hipStream_t s) {
-  if (n <= 0) return;
-  int64_t blocks = ((int64_t)n + 255) / 256;
-  if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_add_counts, dim3((unsigned)blocks), dim3(256), 0, s, n, a, boff, out);
-}
-
-__global__ void k_mirror_scatter(int32_t n, const int64_t* __restrict__ uoff, const int32_t* __restrict__ ucol,
-                                 const double* __restrict__ uval, const int64_t* __restrict__ coff,
-                                 const int32_t* __restrict__ low, int32_t* __restrict__ cursor,
-                                 int32_t* __restrict__ ccol, double* __restrict__ cval) {
-  constexpr int G = 16;
-  const int lane = threadIdx.x % G;
-  const int64_t gpb = blockDim.x / G;
-  for (int64_t r = (int64_t)blockIdx.x * gpb + threadIdx.x / G; r < n; r += (int64_t)gridDim.x * gpb) {
-    const int64_t u0 = uoff[r], u1 = uoff[r + 1];
-    const int64_t dst = coff[r] + low[r];
-    for (int64_t t = u0 + lane; t < u1; t += G) {
-      const int c = ucol[t];
-      const double v = uval[t];
-      ccol[dst + (t - u0)] = c;
-      cval[dst + (t - u0)] = v;
-      if (c > r) {
-        const int64_t pos = coff[c] + atomicAdd(&cursor[c], 1);
-        ccol[pos] = (int32_t)r;
-        cval[pos] = v;
+  for (int64_t c = (int64_t)blockIdx.x * gpb + threadIdx.x / G; c < n; c += (int64_t)gridDim.x * gpb) {
+    const int64_t s = off[c], e = off[c + 1];
+    for (int64_t b = s; b < e; b += G) {
+      if (ccol[b] >= c) break;                 // lower part is a prefix of the row
+      const int64_t p = b + lane;
+      if (p < e) {
+        const int r = ccol[p];
+        if (r < c) {
+          int64_t lo = off[r], hi = off[r + 1];
+          while (lo < hi) {                    // first position with column >= c
+            const int64_t mid = (lo + hi) >> 1;
+            if (ccol[mid] < c) lo = mid + 1; else hi = mid;
+          }
+          cval[p] = (lo < off[r + 1] && ccol[lo] == c) ? cval[lo] : 0.0;
+        }
       }
     }
   }
 }
-void launch_mirror_scatter(int32_t n, const int64_t* uoff, const int32_t* ucol, const double* uval,
-                           const int64_t* coff, const int32_t* low, int32_t* cursor, int32_t* ccol, double* cval,
-                           hipStream_t s) {
+void launch_mirror_pull(int32_t n, const int64_t* off, const int32_t* ccol, double* cval, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_mirror_scatter, dim3((unsigned)blocks), dim3(256), 0, s, n, uoff, ucol, uval, coff, low,
-                     cursor, ccol, cval);
-}
-
-// Segment sort.  Short segments: one wave per segment, in LDS.  Up to 4096:
-// one workgroup per segment in LDS.  Longer: one workgroup per segment,
-// bitonic network directly in global memory.
-constexpr int kSegWave = 256, kSegWg = 4096;
-
-__global__ void __launch_bounds__(256) k_segsort_wave(int32_t n, const int64_t* __restrict__ off,
-                                                      const int32_t* __restrict__ len, int32_t* __restrict__ col,
-                                                      double* __restrict__ val) {
-  __shared__ int32_t sk[4][kSegWave];
-  __shared__ double sv[4][kSegWave];
-  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < n; r += (int64_t)gridDim.x * 4) {
-    const int L = len[r];
-    if (L <= 1 || L > kSegWave) continue;
-    const int64_t o = off[r];
-    for (int t = lane; t < L; t += kWave) { sk[w][t] = col[o + t]; sv[w][t] = val[o + t]; }
-    gsync<kWave>();
-    lds_sort_pairs<kWave, double>(sk[w], sv[w], L, lane);
-    for (int t = lane; t < L; t += kWave) { col[o + t] = sk[w][t]; val[o + t] = sv[w][t]; }
-    gsync<kWave>();
-  }
-}
-
-__global__ void __launch_bounds__(256) k_segsort_wg(int32_t n, const int64_t* __restrict__ off,
-                                                    const int32_t* __restrict__ len, int32_t* __restrict__ col,
-                                                    double* __restrict__ val) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  double* sv = (double*)lds_raw;
-  int32_t* sk = (int32_t*)(lds_raw + kSegWg * 8);
-  for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
-    const int L = len[r];
-    if (L <= kSegWave || L > kSegWg) continue;
-    const int64_t o = off[r];
-    __syncthreads();
-    for (int t = threadIdx.x; t < L; t += 256) { sk[t] = col[o + t]; sv[t] = val[o + t]; }
-    __syncthreads();
-    lds_sort_pairs<256, double>(sk, sv, L, threadIdx.x);
-    for (int t = threadIdx.x; t < L; t += 256) { col[o + t] = sk[t]; val[o + t] = sv[t]; }
-  }
-}
-
-__global__ void __launch_bounds__(256) k_segsort_global(int32_t n, const int64_t* __restrict__ off,
-                                                        const int32_t* __restrict__ len, int32_t* __restrict__ col,
-                                                        double* __restrict__ val) {
-  for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
-    const int L = len[r];
-    if (L <= kSegWg) continue;
-    const int64_t o = off[r];
-    __syncthreads();
-    lds_sort_pairs<256, double>(col + o, val + o, L, threadIdx.x);
-  }
-}
-
-void launch_segment_sort(int32_t n, const int64_t* off, const int32_t* len, int32_t* col, double* val,
-                         int32_t max_len, hipStream_t s) {
-  if (n <= 0 || max_len <= 1) return;
-  int64_t blocks = ((int64_t)n + 3) / 4;
-  if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_segsort_wave, dim3((unsigned)blocks), dim3(256), 0, s, n, off, len, col, val);
-  if (max_len > kSegWave) {
-    const size_t lds = (size_t)kSegWg * 12;
-    hipFuncSetAttribute((const void*)k_segsort_wg, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    int64_t b2 = n < 8192 ? n : 8192;
-    hipLaunchKernelGGL(k_segsort_wg, dim3((unsigned)b2), dim3(256), lds, s, n, off, len, col, val);
-  }
-  if (max_len > kSegWg) {
-    int64_t b3 = n < 1024 ? n : 1024;
-    hipLaunchKernelGGL(k_segsort_global, dim3((unsigned)b3), dim3(256), 0, s, n, off, len, col, val);
-  }
+  hipLaunchKernelGGL(k_mirror_pull, dim3((unsigned)blocks), dim3(256), 0, s, n, off, ccol, cval);
 }
 
 }  // namespace appt

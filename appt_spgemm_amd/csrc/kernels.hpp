@@ -96,18 +96,9 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
 int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max);
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max);
 
-// AAtS mirror helpers.
-void launch_lower_count(int32_t n, const int64_t* uoff, const int32_t* ucol, int32_t* low,
-                        hipStream_t s);
-void launch_add_counts(int32_t n, const int32_t* a, const int64_t* boff, int32_t* out, hipStream_t s);
-void launch_mirror_scatter(int32_t n, const int64_t* uoff, const int32_t* ucol, const double* uval,
-                           const int64_t* coff, const int32_t* low, int32_t* cursor, int32_t* ccol,
-                           double* cval, hipStream_t s);
-// Sort the segments [off[i], off[i] + len[i]) of (col, val) by col; keys in a
-// segment are distinct.  Segments longer than seg_cap_lds use a global
-// bitmap rank over [kmin[i], kmax[i]] in gscratch.
-void launch_segment_sort(int32_t n, const int64_t* off, const int32_t* len, int32_t* col, double* val,
-                         int32_t max_len, hipStream_t s);
+// AAtS: fill the lower entries of the complete, sorted C from row r's upper
+// part (C's structure is symmetric; values were formed for j >= i only).
+void launch_mirror_pull(int32_t n, const int64_t* off, const int32_t* ccol, double* cval, hipStream_t s);
 
 void set_num_cus(int n);
 void set_lds_per_cu(int64_t b);
