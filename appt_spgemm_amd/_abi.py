@@ -41,9 +41,10 @@ class EngResult(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("f_class_max", C.c_int32 * MAX_CLASSES), ("wg_f_max", C.c_int32),
+    _fields_ = [("f_class_max", C.c_int32 * MAX_CLASSES), ("wg_lds_bytes", C.c_int32),
                 ("lds_budget_bytes", C.c_int32), ("bitmap_words_factor", C.c_int32),
-                ("group_for_len", C.c_int32 * 4), ("reserved", C.c_int32 * 8)]
+                ("group_for_len", C.c_int32 * 4), ("heavy_window_cols", C.c_int32),
+                ("reserved", C.c_int32 * 7)]
 
 
 class Options(C.Structure):

@@ -82,7 +82,6 @@ static BinParams default_params() {
   P.cls_max[0] = 128; P.cls_max[1] = 512; P.cls_max[2] = 2048;
   P.cls_bytes[0] = 16 * 1024; P.cls_bytes[1] = 24 * 1024; P.cls_bytes[2] = 32 * 1024;
   P.glen_max[0] = 8; P.glen_max[1] = 16; P.glen_max[2] = 32;
-  P.gl_bm_words = 1 << 21;
   P.dense_ratio = 4;
   return P;
 }
@@ -141,19 +140,20 @@ Engine::Engine(const appt_options* opt) : pool_(0) {
   // workgroup-per-row bins may use the device's LDS per block minus static LDS
   const int32_t wg = (int32_t)(lds_max_ - 8192);
   sym_params.wg_bytes = num_params.wg_bytes = wg;
-  if (opt && opt->tuning.wg_f_max > 0) {
-    // cap the workgroup bins (rows above go to the global path)
-    sym_params.wg_bytes = std::min<int32_t>(wg, opt->tuning.wg_f_max);
+  if (opt && opt->tuning.wg_lds_bytes > 0) {
+    // cap the workgroup bins (rows above go to the heavy-row bin)
+    sym_params.wg_bytes = num_params.wg_bytes = std::min<int32_t>(wg, opt->tuning.wg_lds_bytes);
   }
+  if (opt && opt->tuning.heavy_window_cols > 0) heavy_cap_ = opt->tuning.heavy_window_cols;
 }
 
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  Scratch* all[] = {&ws_.bptr, &ws_.bcol, &ws_.bval, &ws_.cursor, &ws_.bmin, &ws_.bmax, &ws_.rowf,
-                    &ws_.rowlo, &ws_.rowhi, &ws_.symbin, &ws_.numbin, &ws_.rowcnt, &ws_.rowoff,
-                    &ws_.symrows, &ws_.numrows, &ws_.scan_tmp, &ws_.stats, &ws_.gsym, &ws_.gnum,
-                    &ws_.low, &ws_.fullcnt, &ws_.uoff, &ws_.ucol, &ws_.uval, &ws_.coff, &ws_.rowwords, &ws_.wlist, &ws_.statpart};
+  Scratch* all[] = {&ws_.bptr,   &ws_.bcol,     &ws_.bval,    &ws_.cursor, &ws_.bmin,   &ws_.bmax,
+                    &ws_.rowf,   &ws_.rowlo,    &ws_.rowhi,   &ws_.symbin, &ws_.numbin, &ws_.rowcnt,
+                    &ws_.rowoff, &ws_.symrows,  &ws_.numrows, &ws_.scan_tmp, &ws_.stats, &ws_.rowwords,
+                    &ws_.wlist,  &ws_.statpart, &ws_.ctl,     &ws_.cur};
   for (Scratch* s : all) free_scratch(*s);
   pool_.trim();
   for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
@@ -225,7 +225,10 @@ void Engine::prepass_stage(const appt_csr_view& A, appt_op op, const int32_t*& b
   unsigned long long* dprod = reinterpret_cast<unsigned long long*>(dstats + stat_words * 2);
   launch_fill_i32(dstats, (int64_t)(stat_words * 2 + 4 + kNumBins), 0, s);
   const int upper = op == APPT_OP_AATS ? 1 : 0;
-  launch_rowstats(n, bp, bc, bmeta, s);
+  // control words: [0] / [1] heavy-row claim counters, [2] B-unsorted flag
+  int32_t* ctl = as<int32_t>(ensure(ws_.ctl, sizeof(int32_t) * 4));
+  launch_fill_i32(ctl, 4, 0, s);
+  launch_rowstats(n, bp, bc, bmeta, ctl + 2, s);
   int32_t* part = as<int32_t>(ensure(ws_.statpart, sizeof(int32_t) * (size_t)stats_part_elems() +
                                                       sizeof(unsigned long long) * kStatsMaxBlocks));
   launch_prepass(n, A.rowptr, A.col, bmeta, upper, sym_params, rowf, rowlo, rowhi, symbin, dstats, dprod, part, s);
@@ -288,14 +291,25 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     }
     symoff[kNumBins] = run;
   }
-  if (hs_->sym[kBinGlHash * kStatFields] > 0)
-    throw Error(APPT_ERR_UNSUPPORTED, "rows whose output span exceeds the global bitmap window (" +
-                                          std::to_string((int64_t)sym_params.gl_bm_words * 32) + " columns)");
   APPT_HIP(hipMemcpyAsync(dcursor, hs_->cursor, sizeof(int32_t) * kNumBins, hipMemcpyHostToDevice, s));
   launch_bin_scatter(n, symbin, dcursor, symrows, s);
   launch_fill_i32(rowcnt, n, 0, s);
   launch_fill_i32(rowwords, n, 0, s);
   record(2);
+
+  // heavy-row bin: LDS column windows, a cursor per A entry
+  HeavyArgs heavy{};
+  auto setup_heavy = [&]() {
+    if (heavy.work) return;
+    // dynamic LDS: the block's LDS less the kernel's static staging (~21 KB)
+    const int64_t dyn = lds_max_ - 24 * 1024;
+    heavy.wc_sym = heavy_window_cols(false, dyn, heavy_cap_);
+    heavy.wc_num = heavy_window_cols(true, dyn, heavy_cap_);
+    heavy.work = as<int32_t>(ws_.ctl.p);
+    heavy.bflag = heavy.work + 2;
+    heavy.cur = as<int32_t>(ensure(ws_.cur, sizeof(int32_t) * (size_t)std::max<int64_t>(A.nnz, 1)));
+  };
+  if (hs_->sym[kBinHeavy * kStatFields] > 0) setup_heavy();
 
   // ---- symbolic -------------------------------------------------------------
   int nbins_used = 0;
@@ -306,17 +320,9 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     const int32_t mmax = hs_->sym[b * kStatFields + 1];
     const int32_t wmax = hs_->sym[b * kStatFields + 2];
     RowArgs ra{symrows + symoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, 0, A.nnz, bmeta, wlist};
-    if (b == kBinGlBitmap) {
-      const int64_t slot_bytes = (int64_t)wmax * 4;
-      int64_t slots = std::min<int64_t>(cnt, gslots_);
-      slots = std::max<int64_t>(1, std::min<int64_t>(slots, ((int64_t)1 << 30) / std::max<int64_t>(slot_bytes, 1)));
-      uint32_t* g = as<uint32_t>(ensure(ws_.gsym, (size_t)(slots * slot_bytes)));
-      launch_symbolic_bin(b, ra, wmax, rowcnt, g, (int32_t)slots, s);
-    } else {
-      const int64_t rb = symbolic_row_lds_bytes(b, mmax, wmax);
-      if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "symbolic row exceeds LDS");
-      launch_symbolic_bin(b, ra, wmax, rowcnt, nullptr, 0, s);
-    }
+    const int64_t rb = symbolic_row_lds_bytes(b, mmax, wmax);
+    if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "symbolic row exceeds LDS");
+    launch_symbolic_bin(b, ra, wmax, rowcnt, heavy, s);
   }
   record(3);
 
@@ -340,10 +346,6 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     }
     numoff[kNumBins] = run;
   }
-  if (hs_->num[kBinGlHash * kStatFields] > 0) {
-    pool_.put(rp32);
-    throw Error(APPT_ERR_UNSUPPORTED, "numeric row span exceeds the global bitmap window");
-  }
   APPT_HIP(hipMemcpyAsync(dcursor, hs_->cursor, sizeof(int32_t) * kNumBins, hipMemcpyHostToDevice, s));
   launch_bin_scatter(n, numbin, dcursor, numrows, s);
   record(4);
@@ -360,17 +362,10 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     const int32_t cmax = hs_->num[b * kStatFields + 1];
     const int32_t wmax = hs_->num[b * kStatFields + 2];
     RowArgs ra{numrows + numoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, upper, A.nnz, bmeta, wlist};
-    if (b == kBinGlBitmap) {
-      const int64_t slot_bytes = (int64_t)wmax * 8;
-      int64_t slots = std::min<int64_t>(cnt, gslots_);
-      slots = std::max<int64_t>(1, std::min<int64_t>(slots, ((int64_t)1 << 30) / std::max<int64_t>(slot_bytes, 1)));
-      uint32_t* g = as<uint32_t>(ensure(ws_.gnum, (size_t)(slots * slot_bytes)));
-      launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, g, (int32_t)slots, s);
-    } else {
-      const int64_t rb = numeric_row_lds_bytes(b, cmax, wmax);
-      if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "numeric row exceeds LDS");
-      launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, nullptr, 0, s);
-    }
+    const int64_t rb = numeric_row_lds_bytes(b, cmax, wmax);
+    if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "numeric row exceeds LDS");
+    if (b == kBinHeavy) setup_heavy();   // rows may become heavy only in the numeric binning
+    launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, heavy, s);
   }
   record(5);
 

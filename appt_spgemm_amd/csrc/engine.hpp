@@ -55,7 +55,7 @@ struct Scratch {
 
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmin, bmax, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff,
-      symrows, numrows, scan_tmp, stats, gsym, gnum, low, fullcnt, uoff, ucol, uval, coff, rowwords, wlist, statpart;
+      symrows, numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur;
 };
 
 struct Result {  // what appt_csr_result::_priv points to
@@ -111,7 +111,7 @@ class Engine {
   hipEvent_t ev_[8] = {};
   appt_phase_times times_ = {};
   int64_t lds_max_ = 65536;
-  int32_t gslots_ = 512;
+  int32_t heavy_cap_ = 0;  // column-window cap of the heavy-row bin (0: LDS-limited)
 };
 
 void set_lds_budget(int64_t b);

@@ -62,10 +62,11 @@ void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s) 
 // passes.
 // ---------------------------------------------------------------------------
 __global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                           int4* __restrict__ meta) {
+                           int4* __restrict__ meta, int32_t* __restrict__ flag) {
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
+  bool unsorted = false;
   for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb) {
     const int s = ptr[i], e = ptr[i + 1];
     int mn = INT32_MAX, mx = -1;
@@ -73,17 +74,20 @@ __global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int
       const int c = col[p];
       mn = c < mn ? c : mn;
       mx = c > mx ? c : mx;
+      if (p > s && col[p - 1] > c) unsorted = true;
     }
     mn = gmin<G>(mn);
     mx = gmax<G>(mx);
     if (lane == 0) meta[i] = make_int4(s, e, mn, mx);
   }
+  // one plain store per wave that saw a descent (all writers store 1)
+  if (__ballot(unsorted) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
 }
-void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, hipStream_t s) {
+void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, int32_t* flag, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_rowstats, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, meta);
+  hipLaunchKernelGGL(k_rowstats, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, meta, flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -263,7 +267,7 @@ __device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t
   if (words * 4 <= P.wg_bytes) { *stat = words; return kBinWgBitmap; }
   if (hash_bytes <= P.wg_bytes) { *stat = keys; return kBinWgHash; }
   *stat = words;
-  return words <= P.gl_bm_words ? kBinGlBitmap : kBinGlHash;
+  return kBinHeavy;
 }
 
 __host__ __device__ __forceinline__ int64_t whash_bytes_hd(int64_t w, int64_t c) {
@@ -291,7 +295,7 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
   if (dense_ok && words * 8 <= P.wg_bytes / 2 && c * 12 <= P.wg_bytes / 2) { *stat = words; return kBinWgBitmap; }
   if (whash_bytes_hd(c, c) <= P.wg_bytes) { *stat = w; return kBinWgHash; }
   *stat = words;
-  return words <= P.gl_bm_words ? kBinGlBitmap : kBinGlHash;
+  return kBinHeavy;
 }
 
 // Block-local bin statistics flushed with one global atomic per bin.
@@ -744,36 +748,6 @@ __global__ void __launch_bounds__(256) k_symbolic_lds(RowArgs A, int32_t row_wor
   if (lane == 0) { rowcnt[i] = cnt; A.rowwords[i] = nw | kNoWordList; }
 }
 
-// Heavy rows: one workgroup per row, bitmap window in a global scratch slot.
-__global__ void __launch_bounds__(256) k_symbolic_global(RowArgs A, int32_t slot_words, uint32_t* __restrict__ gscratch,
-                                                         int32_t* __restrict__ rowcnt) {
-  constexpr int G = 256;
-  __shared__ Stage stage_buf[1];
-  __shared__ int32_t red[16];
-  uint32_t* tab = gscratch + (size_t)blockIdx.x * slot_words;
-  const int lane = threadIdx.x;
-  for (int r = blockIdx.x; r < A.nrows; r += gridDim.x) {
-    const int i = A.rows[r];
-    const int lo = A.rowlo[i], hi = A.rowhi[i];
-    const int words = (int)span_words(lo, hi);
-    __syncthreads();
-    for (int t = lane; t < words; t += G) tab[t] = 0u;
-    __threadfence();
-    __syncthreads();
-    int cnt = 0, nw = 0;
-    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
-      const uint32_t b = (uint32_t)(j - lo);
-      const uint32_t bit = 1u << (b & 31u);
-      const uint32_t old = atomicOr(&tab[b >> 5], bit);
-      cnt += (old & bit) ? 0 : 1;
-      nw += old == 0u ? 1 : 0;
-    });
-    cnt = gsum<G, int32_t>(cnt, red);
-    nw = gsum<G, int32_t>(nw, red);
-    if (lane == 0) { rowcnt[i] = cnt; A.rowwords[i] = nw | kNoWordList; }
-  }
-}
-
 // ---------------------------------------------------------------------------
 // numeric
 // ---------------------------------------------------------------------------
@@ -1001,64 +975,147 @@ __global__ void __launch_bounds__(256) k_numeric_lds(RowArgs A, int32_t words_ma
   }
 }
 
-// Heavy rows: global bitmap slot (bits + prefix as uint2 per word); values are
-// accumulated straight into C at their final sorted position.
-__global__ void __launch_bounds__(256) k_numeric_global(RowArgs A, int32_t slot_words, uint32_t* __restrict__ gscratch,
-                                                        const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
-                                                        double* __restrict__ cval) {
-  constexpr int G = 256;
-  __shared__ Stage stage_buf[1];
-  __shared__ int64_t red64[17];
-  uint2* info = (uint2*)(gscratch + (size_t)blockIdx.x * slot_words * 2);
-  uint32_t* bits = (uint32_t*)info;
-  const int lane = threadIdx.x;
-  for (int r = blockIdx.x; r < A.nrows; r += gridDim.x) {
+// ---------------------------------------------------------------------------
+// Heavy rows: outputs too wide for one workgroup's LDS.  One workgroup per
+// row, rows claimed from a counter (heavy rows differ widely in cost).  The
+// row's output span [lo, hi] is processed in column windows of `wc` columns
+// held in LDS: a bitmap (symbolic) or a bitmap plus an f64 accumulator
+// (numeric).  When every B row is sorted, a cursor per A entry (the first B
+// position not yet consumed) carries over from one window to the next, so
+// each product is loaded once; otherwise each window rescans the B rows and
+// filters by column.  Inside a window the products of kHeavyT A entries are
+// flattened (exclusive scan of their segment lengths) and dealt to the
+// threads one product each, so short and long B rows balance.
+// ---------------------------------------------------------------------------
+struct HeavyStage {
+  int64_t off[kHeavyT];  // exclusive scan of the entries' segment lengths
+  int32_t qs[kHeavyT];   // segment start (B position)
+  double a[kHeavyT];     // a_ik
+  int64_t red[17];
+  int32_t row;
+};
+
+template <bool VALS>
+__global__ void __launch_bounds__(kHeavyT) k_heavy(RowArgs A, int32_t wc, int32_t* __restrict__ work,
+                                                   const int32_t* __restrict__ bflag, int32_t* __restrict__ cur,
+                                                   int32_t* __restrict__ rowcnt, const int64_t* __restrict__ rowoff,
+                                                   int32_t* __restrict__ ccol, double* __restrict__ cval) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  __shared__ HeavyStage hs;
+  double* vals = (double*)lds_raw;
+  uint32_t* bits = (uint32_t*)(lds_raw + (VALS ? (size_t)wc * 8 : 0));
+  const int tid = threadIdx.x;
+  const bool sorted = bflag[0] == 0;
+  while (true) {
+    __syncthreads();
+    if (tid == 0) hs.row = atomicAdd(work, 1);
+    __syncthreads();
+    const int r = hs.row;
+    if (r >= A.nrows) break;
     const int i = A.rows[r];
     const int lo = A.rowlo[i], hi = A.rowhi[i];
-    const int words = (int)span_words(lo, hi);
-    const int64_t off = rowoff[i];
-    const int c = (int)(rowoff[i + 1] - off);
-    __syncthreads();
-    for (int t = lane; t < words; t += G) info[t] = make_uint2(0u, 0u);
-    for (int t = lane; t < c; t += G) cval[off + t] = 0.0;
-    __threadfence();
-    __syncthreads();
-    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
-      const uint32_t b = (uint32_t)(j - lo);
-      atomicOr(&bits[(b >> 5) * 2], 1u << (b & 31u));
-    });
-    __threadfence();
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const int chunk = (words + G - 1) / G;
-    const int w0 = lane * chunk;
-    const int w1 = (w0 + chunk) < words ? (w0 + chunk) : words;
-    int local = 0;
-    for (int w = w0; w < w1; w++) local += __popc(__hip_atomic_load(&bits[w * 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    int64_t tot;
-    int run = (int)block_scan_excl<int64_t>(local, red64, &tot);
-    for (int w = w0; w < w1; w++) {
-      uint32_t x = __hip_atomic_load(&bits[w * 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      info[w].y = (uint32_t)run;
-      while (x) {
-        const int bpos = __builtin_ctz(x);
-        ccol[off + run++] = lo + w * 32 + bpos;
-        x &= x - 1u;
-      }
+    const int s = A.ap[i], e = A.ap[i + 1];
+    const int npass = (int)(((int64_t)hi - lo + wc) / wc);
+    const bool carry = sorted && npass > 1;
+    int64_t out = 0, end = 0;
+    if (VALS) {
+      out = rowoff[i];
+      end = rowoff[i + 1];
     }
-    __threadfence();
-    __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for_each_product<G, true>(i, lane, A, stage_buf, [&](int j, double prod) {
-      if (A.upper && j < i) return;
-      const uint32_t b = (uint32_t)(j - lo);
-      const uint2 wi = info[b >> 5];
-      const uint32_t pos = wi.y + __popc(wi.x & ((1u << (b & 31u)) - 1u));
-      unsafeAtomicAdd(&cval[off + pos], prod);
-    });
+    int64_t cnt = 0, nzw = 0;
+    for (int pass = 0; pass < npass; pass++) {
+      const int w0 = lo + pass * wc;
+      const int w1 = (int)((int64_t)w0 + wc < (int64_t)hi + 1 ? (int64_t)w0 + wc : (int64_t)hi + 1);
+      const int width = w1 - w0;
+      const int words = (width + 31) >> 5;
+      for (int t = tid; t < words; t += kHeavyT) bits[t] = 0u;
+      if (VALS)
+        for (int t = tid; t < width; t += kHeavyT) vals[t] = 0.0;
+      for (int cb = s; cb < e; cb += kHeavyT) {
+        const int p = cb + tid;
+        int qs = 0, qe = 0;
+        double a = 0.0;
+        if (p < e) {
+          const int4 bm = A.bmeta[A.ac[p]];
+          qs = (carry && pass > 0) ? cur[p] : bm.x;
+          qe = bm.y;
+          if (carry && qs < qe && bm.w >= w1) {
+            // first position of the B row at or past the window end
+            int l = qs, h = qe;
+            while (l < h) {
+              const int mid = (int)(((unsigned)l + (unsigned)h) >> 1);
+              if (A.bc[mid] < w1) l = mid + 1;
+              else h = mid;
+            }
+            qe = l;
+          }
+          if (carry && pass + 1 < npass) cur[p] = qe;
+          if (VALS) a = A.av[p];
+        }
+        int64_t tot;
+        const int64_t ex = block_scan_excl<int64_t>((int64_t)(qe - qs), hs.red, &tot);
+        hs.off[tid] = ex;
+        hs.qs[tid] = qs;
+        if (VALS) hs.a[tid] = a;
+        const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
+        __syncthreads();
+        for (int64_t t = tid; t < tot; t += kHeavyT) {
+          // owner: the last entry whose segment starts at or before t
+          int l = 0, h = m - 1;
+          while (l < h) {
+            const int mid = (l + h + 1) >> 1;
+            if (hs.off[mid] <= t) l = mid;
+            else h = mid - 1;
+          }
+          const int q = hs.qs[l] + (int)(t - hs.off[l]);
+          const int j = A.bc[q];
+          if (j < w0 || j >= w1) continue;   // only possible when B rows are rescanned
+          const uint32_t b = (uint32_t)(j - w0);
+          atomicOr(&bits[b >> 5], 1u << (b & 31u));
+          if (VALS && (!A.upper || j >= i)) atomicAdd(&vals[b], hs.a[l] * A.bv[q]);
+        }
+        __syncthreads();
+      }
+      if (!VALS) {
+        int64_t c = 0, z = 0;
+        for (int t = tid; t < words; t += kHeavyT) {
+          const uint32_t x = bits[t];
+          c += __popc(x);
+          z += x != 0u ? 1 : 0;
+        }
+        int64_t tc, tz;
+        (void)block_scan_excl<int64_t>(c, hs.red, &tc);
+        (void)block_scan_excl<int64_t>(z, hs.red, &tz);
+        cnt += tc;
+        nzw += tz;
+      } else {
+        // compact the window into C: one word per thread, ascending columns
+        for (int wb = 0; wb < words; wb += kHeavyT) {
+          const int w = wb + tid;
+          const uint32_t x = w < words ? bits[w] : 0u;
+          int64_t tot;
+          int64_t o = out + block_scan_excl<int64_t>((int64_t)__popc(x), hs.red, &tot);
+          uint32_t y = x;
+          while (y) {
+            const int c = w * 32 + __builtin_ctz(y);
+            if (o < end) {
+              ccol[o] = w0 + c;
+              cval[o] = vals[c];
+            }
+            o++;
+            y &= y - 1u;
+          }
+          out += tot;
+        }
+      }
+      __syncthreads();
+    }
+    if (!VALS && tid == 0) {
+      rowcnt[i] = (int32_t)cnt;
+      A.rowwords[i] = (int32_t)nzw | kNoWordList;
+    }
   }
 }
-
 // ---------------------------------------------------------------------------
 // Window kernels (dense mode of the wave-level bins).  Persistent: each block
 // owns a contiguous chunk of the bin's row list (XCD-aware numbering), and
@@ -1467,14 +1524,14 @@ static bool bin_dense(int bin) {
 }
 
 int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
-  if (bin == kBinGlBitmap || bin == kBinGlHash || bin == 0) return 0;
+  if (bin == kBinHeavy || bin == 0) return 0;
   (void)metric_max;
   if (bin == kBinWgBitmap) return (int64_t)words_max * 4;
   if (bin_dense(bin)) return sym_window_bytes(bin_group(bin), words_max);
   return sym_slots(words_max) * 8;
 }
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max) {
-  if (bin == kBinGlBitmap || bin == kBinGlHash || bin == 0) return 0;
+  if (bin == kBinHeavy || bin == 0) return 0;
   if (bin == kBinWgBitmap) return dense_row_bytes(words_max, c_max);
   if (bin_dense(bin)) return num_window_bytes(bin_group(bin), words_max, c_max);
   return whash_row_bytes(words_max, c_max);
@@ -1579,12 +1636,41 @@ static void num_window_any(const RowArgs& a, int32_t W, int32_t Cm, const int64_
   }
 }
 
-void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* rowcnt, uint32_t* gscratch,
-                         int32_t gslots, hipStream_t s) {
+// Column window of the heavy path for a dynamic LDS allowance of `bytes`:
+// 1 bit per column (symbolic) or 8 bytes + 1 bit (numeric), a multiple of 32.
+int32_t heavy_window_cols(bool vals, int64_t bytes, int32_t cap) {
+  int64_t wc = vals ? (bytes * 32) / 260 : bytes * 8;
+  if (cap > 0 && wc > cap) wc = cap;
+  wc &= ~(int64_t)31;
+  if (wc < 32) wc = 32;
+  if (wc > ((int64_t)1 << 30)) wc = (int64_t)1 << 30;
+  return (int32_t)wc;
+}
+int64_t heavy_lds_bytes(bool vals, int32_t wc) { return (vals ? (int64_t)wc * 8 : 0) + (int64_t)(wc / 32) * 4; }
+
+static void heavy_launch(bool vals, const RowArgs& a, const HeavyArgs& h, int32_t* work, int32_t* rowcnt,
+                         const int64_t* rowoff, int32_t* ccol, double* cval, hipStream_t s) {
+  const int32_t wc = vals ? h.wc_num : h.wc_sym;
+  const size_t lds = (size_t)heavy_lds_bytes(vals, wc);
+  // one block per CU (kHeavyT threads and most of the CU's LDS each)
+  const unsigned blocks = (unsigned)(a.nrows < g_num_cus ? a.nrows : g_num_cus);
+  launch_fill_i32(work, 1, 0, s);
+  if (vals) {
+    (void)hipFuncSetAttribute((const void*)k_heavy<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_heavy<true>, dim3(blocks), dim3(kHeavyT), lds, s, a, wc, work, h.bflag, h.cur, rowcnt,
+                       rowoff, ccol, cval);
+  } else {
+    (void)hipFuncSetAttribute((const void*)k_heavy<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_heavy<false>, dim3(blocks), dim3(kHeavyT), lds, s, a, wc, work, h.bflag, h.cur, rowcnt,
+                       rowoff, ccol, cval);
+  }
+}
+
+void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* rowcnt, const HeavyArgs& h,
+                         hipStream_t s) {
   if (a.nrows <= 0 || bin <= 0) return;
-  if (bin == kBinGlBitmap) {
-    const unsigned blocks = (unsigned)(a.nrows < gslots ? a.nrows : gslots);
-    hipLaunchKernelGGL(k_symbolic_global, dim3(blocks), dim3(256), 0, s, a, words_max, gscratch, rowcnt);
+  if (bin == kBinHeavy) {
+    heavy_launch(false, a, h, h.work, rowcnt, nullptr, nullptr, nullptr, s);
     return;
   }
   if (bin == kBinWgBitmap) { sym_lds<256, true>(a, words_max, rowcnt, s); return; }
@@ -1605,11 +1691,10 @@ void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* 
 }
 
 void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_max, const int64_t* rowoff,
-                        int32_t* ccol, double* cval, uint32_t* gscratch, int32_t gslots, hipStream_t s) {
+                        int32_t* ccol, double* cval, const HeavyArgs& h, hipStream_t s) {
   if (a.nrows <= 0 || bin <= 0) return;
-  if (bin == kBinGlBitmap) {
-    const unsigned blocks = (unsigned)(a.nrows < gslots ? a.nrows : gslots);
-    hipLaunchKernelGGL(k_numeric_global, dim3(blocks), dim3(256), 0, s, a, words_max, gscratch, rowoff, ccol, cval);
+  if (bin == kBinHeavy) {
+    heavy_launch(true, a, h, h.work + 1, nullptr, rowoff, ccol, cval, s);
     return;
   }
   if (bin == kBinWgBitmap) { num_lds<256, true>(a, words_max, c_max, rowoff, ccol, cval, s); return; }

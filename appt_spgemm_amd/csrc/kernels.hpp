@@ -8,9 +8,8 @@
 //                              1 word hash (32-column words as keys)
 //                       gidx : lanes per row G = 8, 16, 32, 64
 //   25 / 26           one workgroup per row, dense / word hash in LDS
-//   27                heavy rows: one workgroup per row, bitmap window in a
-//                     global scratch slot
-//   28                heavy rows whose span is too wide for a bitmap slot
+//   27                heavy rows: one workgroup per row, the output span
+//                     walked in LDS column windows
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -18,8 +17,9 @@
 namespace appt {
 
 constexpr int kNumCls = 3;
-constexpr int kNumBins = 29;
-constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinGlBitmap = 27, kBinGlHash = 28;
+constexpr int kNumBins = 28;
+constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27;
+constexpr int kHeavyT = 1024;   // threads of a heavy-row workgroup
 constexpr int kStatFields = 3;  // count, max metric, max words
 constexpr int kStatsMaxBlocks = 4096;
 // int32 elements of the per-block statistics scratch (plus u64 product counts)
@@ -30,7 +30,6 @@ struct BinParams {
   int32_t cls_max[kNumCls];       // metric bound of class c (products, or row nnz)
   int32_t cls_bytes[kNumCls];     // LDS bytes one row of class c may use
   int32_t wg_bytes;               // LDS bytes of a workgroup-per-row bin
-  int32_t gl_bm_words;            // global bitmap if words(span) <= this
   int32_t glen_max[3];            // mean B-row length bounds for G = 8, 16, 32
   int32_t dense_ratio;            // numeric: dense if words(span) <= max(ratio * distinct words, 64)
 };
@@ -50,7 +49,10 @@ inline int64_t num_slots(int64_t c) { return np2_host((c * 3) / 2 + 1); }
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 // B-row metadata {start, end, min column, max column} (min = INT32_MAX,
 // max = -1 for an empty row).
-void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, hipStream_t s);
+// flag[0] is set to 1 when some row's columns are not in nondecreasing order
+// (flag must be zero beforehand).
+void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, int32_t* flag,
+                     hipStream_t s);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
                               int32_t* cursor, int32_t* bc, double* bv, hipStream_t s);
@@ -83,16 +85,27 @@ struct RowArgs {
 };
 constexpr int kWordListCap = 32;
 
+// Heavy-row bin: column windows of wc_sym / wc_num columns, two row-claim
+// counters (work[0] symbolic, work[1] numeric), the B-sortedness flag of
+// launch_rowstats and a cursor per A entry (nnz(A) int32).
+struct HeavyArgs {
+  int32_t wc_sym, wc_num;
+  int32_t* work;
+  const int32_t* bflag;
+  int32_t* cur;
+};
+int32_t heavy_window_cols(bool vals, int64_t lds_bytes, int32_t cap);
+int64_t heavy_lds_bytes(bool vals, int32_t wc);
+
 // Symbolic: rowcnt[i] = distinct output columns of row i.
-// lds_row_words: u32 words of LDS per row (bitmap words or hash slots).
-void launch_symbolic_bin(int bin, const RowArgs& a, int32_t lds_row_words, int32_t* rowcnt,
-                         uint32_t* gscratch, int32_t gslots, hipStream_t s);
+// words_max: the bin's largest LDS statistic (span words or hash keys).
+void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* rowcnt, const HeavyArgs& h,
+                         hipStream_t s);
 // Numeric: writes sorted col/val of each row at rowoff[i].
-// lds_row_bytes: LDS bytes per row; words_max / c_max: bin maxima.
+// words_max / c_max: bin maxima.
 void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_max,
-                        const int64_t* rowoff, int32_t* ccol, double* cval, uint32_t* gscratch,
-                        int32_t gslots, hipStream_t s);
-// Bytes of LDS one row of a bin needs (0 for global bins).
+                        const int64_t* rowoff, int32_t* ccol, double* cval, const HeavyArgs& h, hipStream_t s);
+// Bytes of LDS one row of a bin needs (0 for the heavy bin).
 int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max);
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max);
 

@@ -81,11 +81,13 @@ typedef struct appt_eng_result {
 #define APPT_MAX_CLASSES 8
 typedef struct appt_tuning {
   int32_t f_class_max[APPT_MAX_CLASSES]; /* products upper bound of LDS classes, ascending, 0-terminated */
-  int32_t wg_f_max;           /* rows with products <= this (and above the classes) use a workgroup per row */
+  int32_t wg_lds_bytes;       /* LDS cap of the workgroup-per-row bins; rows needing more take the
+                                 windowed heavy-row bin (0 = the device's LDS per block) */
   int32_t lds_budget_bytes;   /* LDS bytes a workgroup may take */
-  int32_t bitmap_words_factor;/* use the dense bitmap window when words(span) <= factor * table slots / 8 */
+  int32_t bitmap_words_factor;/* numeric dense window when words(span) <= factor * distinct words */
   int32_t group_for_len[4];   /* lanes per row for mean B-row length <= 8,16,32,inf (0 = default) */
-  int32_t reserved[8];
+  int32_t heavy_window_cols;  /* column-window cap of the heavy-row bin (0 = as wide as LDS allows) */
+  int32_t reserved[7];
 } appt_tuning;
 
 typedef struct appt_options {

@@ -85,7 +85,7 @@ def test_rmat_small(eng, op):
 
 @pytest.mark.parametrize("op", OPS)
 def test_heavy_rows(eng, op):
-    # arrow matrix: one dense row and column -> workgroup and global bins
+    # arrow matrix: one dense row and column -> workgroup and heavy-row bins
     n = 20000
     rng = np.random.default_rng(7)
     rows = [np.arange(n)]
@@ -101,6 +101,44 @@ def test_heavy_rows(eng, op):
     m = host(n, rp, c, rng.uniform(0, 1, len(c)))
     say("case arrow n=20000", op)
     got = eng.multiply(m, op)
+    assert_product(got, ref(m, op), exact=False)
+
+
+@pytest.fixture(scope="module")
+def eng_heavy():
+    # workgroup bins capped at 1 KB of LDS and 96-column windows: every row
+    # wider than a few hundred columns takes the heavy-row bin, in many windows
+    e = ap.Engine(0, tuning={"wg_lds_bytes": 1024, "heavy_window_cols": 96})
+    yield e
+    e.close()
+
+
+def shuffled_rows(m, seed):
+    """Same matrix with the entries of every row in random order."""
+    rng = np.random.default_rng(seed)
+    col = m.col.copy(); val = m.val.copy()
+    for i in range(m.n):
+        s, e = m.rowptr[i], m.rowptr[i + 1]
+        p = s + rng.permutation(e - s)
+        col[s:e] = m.col[p]; val[s:e] = m.val[p]
+    return host(m.n, m.rowptr, col, val)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_heavy_windows(eng_heavy, op):
+    # cursor mode (sorted B rows) over many windows
+    for m in (ap.rmat(11, 16, seed=3), ap.erdos_renyi(3000, 40.0, 8), ap.banded(2500, 200, 0.5, 9)):
+        say("case heavy windows", m.meta, op)
+        got = eng_heavy.multiply(m, op)
+        assert_product(got, ref(m, op), exact=False)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_heavy_windows_unsorted(eng_heavy, op):
+    # rescan mode: B rows unsorted, every window filters by column
+    m = shuffled_rows(ap.erdos_renyi(2000, 30.0, 10), 11)
+    say("case heavy windows unsorted", op)
+    got = eng_heavy.multiply(m, op)
     assert_product(got, ref(m, op), exact=False)
 
 
