@@ -73,6 +73,17 @@ class HostCsr(C.Structure):
                 ("col", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
 
 
+class BatchBuffers(C.Structure):
+    _fields_ = [("n_total", C.c_int32), ("nnz", C.c_int64), ("rowptr", C.c_void_p),
+                ("col", C.c_void_p), ("val", C.c_void_p)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [("count", C.c_int32), ("n_total", C.c_int32), ("nnz_a", C.c_int64),
+                ("nnz_c", C.c_int64), ("products", C.c_int64), ("graph", C.c_int32),
+                ("bins_used", C.c_int32), ("tuning", Tuning)]
+
+
 P = C.POINTER
 _SIGNATURES = {
     "appt_last_error": (C.c_char_p, []),
@@ -98,6 +109,17 @@ _SIGNATURES = {
                                 C.c_uint64, P(HostCsr)]),
     "appt_gen_banded": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_uint64, P(HostCsr)]),
     "appt_gen_erdos_renyi": (C.c_int, [C.c_int32, C.c_double, C.c_uint64, P(HostCsr)]),
+    "appt_batch_create": (C.c_int, [C.c_void_p, C.c_int32, P(C.c_int32), P(C.c_int64), P(C.c_int),
+                                    P(C.c_void_p)]),
+    "appt_batch_destroy": (C.c_int, [C.c_void_p]),
+    "appt_batch_inputs": (C.c_int, [C.c_void_p, P(BatchBuffers)]),
+    "appt_batch_upload": (C.c_int, [C.c_void_p, P(HostCsr)]),
+    "appt_batch_gather": (C.c_int, [C.c_void_p, P(CsrView)]),
+    "appt_batch_plan": (C.c_int, [C.c_void_p, C.c_int32]),
+    "appt_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, P(Tuning)]),
+    "appt_batch_execute": (C.c_int, [C.c_void_p]),
+    "appt_batch_result": (C.c_int, [C.c_void_p, C.c_int32, P(CsrResult)]),
+    "appt_batch_info_get": (C.c_int, [C.c_void_p, P(BatchInfo)]),
     "appt_device_upload": (C.c_int, [C.c_void_p, P(HostCsr), P(CsrView)]),
     "appt_device_free_view": (C.c_int, [C.c_void_p, P(CsrView)]),
     "appt_csr_download": (C.c_int, [C.c_void_p, P(CsrResult), P(C.c_int64), P(C.c_int32),

@@ -35,8 +35,8 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__",
              f"-I{ROCM / 'include'}"]
 
-HIP_SOURCES = ["kernels.hip", "descriptors.hip"]
-CXX_SOURCES = ["engine.cpp", "capi.cpp", "gen.cpp"]
+HIP_SOURCES = ["kernels.hip", "descriptors.hip", "batch.hip"]
+CXX_SOURCES = ["engine.cpp", "batch.cpp", "capi.cpp", "gen.cpp"]
 
 
 def _hipcc() -> str:

@@ -18,6 +18,9 @@ using appt::Error;
 struct appt_engine {
   std::unique_ptr<Engine> impl;
 };
+struct appt_batch {
+  std::unique_ptr<appt::Batch> impl;
+};
 
 template <typename F>
 static appt_status guard(F&& f) {
@@ -129,6 +132,67 @@ appt_status appt_spgemm_run_timed(const char* task, const int32_t* dOff, const i
 appt_status appt_dispatch_descriptors(appt_engine* eng, const appt_csr_view* A, appt_op op, appt_descriptors* out) {
   if (!eng || !A || !out) return APPT_ERR_INVALID;
   return guard([&] { eng->impl->descriptors(*A, op, out); });
+}
+
+appt_status appt_batch_create(appt_engine* eng, int32_t count, const int32_t* n_each, const int64_t* nnz_each,
+                              const appt_op* ops, appt_batch** out) {
+  if (!eng || !out) return APPT_ERR_INVALID;
+  *out = nullptr;
+  return guard([&] {
+    auto* b = new appt_batch;
+    try {
+      b->impl.reset(new appt::Batch(*eng->impl, count, n_each, nnz_each, ops));
+    } catch (...) {
+      delete b;
+      throw;
+    }
+    *out = b;
+  });
+}
+
+appt_status appt_batch_destroy(appt_batch* b) {
+  if (!b) return APPT_OK;
+  return guard([&] { delete b; });
+}
+
+appt_status appt_batch_inputs(appt_batch* b, appt_batch_buffers* out) {
+  if (!b || !out) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->inputs(out); });
+}
+
+appt_status appt_batch_upload(appt_batch* b, const appt_host_csr* packed) {
+  if (!b || !packed) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->upload(*packed); });
+}
+
+appt_status appt_batch_gather(appt_batch* b, const appt_csr_view* each) {
+  if (!b) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->gather(each); });
+}
+
+appt_status appt_batch_plan(appt_batch* b, int32_t use_graph) {
+  if (!b) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->plan(use_graph != 0); });
+}
+
+appt_status appt_batch_autotune(appt_batch* b, int32_t reps, appt_tuning* chosen) {
+  if (!b) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->autotune(reps, chosen); });
+}
+
+appt_status appt_batch_execute(appt_batch* b) {
+  if (!b) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->execute(); });
+}
+
+appt_status appt_batch_result(const appt_batch* b, int32_t k, appt_csr_result* out) {
+  if (!b || !out) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->result(k, out); });
+}
+
+appt_status appt_batch_info_get(const appt_batch* b, appt_batch_info* out) {
+  if (!b || !out) return APPT_ERR_INVALID;
+  return guard([&] { b->impl->info(out); });
 }
 
 appt_status appt_device_upload(appt_engine* eng, const appt_host_csr* h, appt_csr_view* v) {

@@ -146,7 +146,12 @@ void Engine::descriptors(const appt_csr_view& A, appt_op op, appt_descriptors* o
   out->psym = nnz > 0 ? (double)hacc[7] / (double)nnz : 0.0;
   const double fill = op == APPT_OP_AA ? out->fill_aa : out->fill_aat;
   out->pick_ours = pick_ours_rule(out->deg_cv, fill, nnz) ? 1 : 0;
-  out->plan_bins = plan_bins(A, op);
+  Operands O;
+  Plan P;
+  stage_operands(ws_, A, op, O);
+  stage_prepass(ws_, O, cfg_);
+  read_sym(ws_, P);
+  out->plan_bins = P.bins_used;
 }
 
 }  // namespace appt

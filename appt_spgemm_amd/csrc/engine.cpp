@@ -13,6 +13,10 @@
 //                                          lower entries pulled from the upper part
 //   cub::DeviceRadixSort (:195)            not needed: rows come out sorted
 // Host synchronisations: 2 (bin sizes; nnz(C)).
+//
+// The pipeline is split into stages that only enqueue kernels; compute()
+// runs them with the two synchronisations between, and a batch replays them
+// inside a HIP graph with the launch shapes of its first run.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -83,7 +87,35 @@ static BinParams default_params() {
   P.cls_bytes[0] = 16 * 1024; P.cls_bytes[1] = 24 * 1024; P.cls_bytes[2] = 32 * 1024;
   P.glen_max[0] = 8; P.glen_max[1] = 16; P.glen_max[2] = 32;
   P.dense_ratio = 4;
+  P.lds_budget = 64 * 1024;
   return P;
+}
+
+Config make_config(const appt_tuning* t, int64_t lds_max) {
+  Config c;
+  c.sym = default_params();
+  c.num = default_params();
+  int64_t group_budget = 64 * 1024;
+  // workgroup-per-row bins may use the device's LDS per block minus static LDS
+  int32_t wg = (int32_t)(lds_max - 8192);
+  if (t) {
+    int nc = 0;
+    while (nc < APPT_MAX_CLASSES && nc < kNumCls && t->f_class_max[nc] > 0) nc++;
+    if (nc > 0) {
+      c.sym.ncls = c.num.ncls = nc;
+      for (int k = 0; k < nc; k++) c.sym.cls_max[k] = c.num.cls_max[k] = t->f_class_max[k];
+    }
+    if (t->lds_budget_bytes > 0) group_budget = t->lds_budget_bytes;
+    if (t->bitmap_words_factor > 0) c.num.dense_ratio = t->bitmap_words_factor;
+    for (int g = 0; g < 3; g++)
+      if (t->group_for_len[g] > 0) c.sym.glen_max[g] = c.num.glen_max[g] = t->group_for_len[g];
+    if (t->wg_lds_bytes > 0) wg = std::min<int32_t>(wg, t->wg_lds_bytes);
+    if (t->heavy_window_cols > 0) c.heavy_cap = t->heavy_window_cols;
+  }
+  if (group_budget > lds_max) group_budget = lds_max;
+  c.sym.lds_budget = c.num.lds_budget = (int32_t)group_budget;
+  c.sym.wg_bytes = c.num.wg_bytes = wg;
+  return c;
 }
 
 Engine::Engine(const appt_options* opt) : pool_(0) {
@@ -118,43 +150,13 @@ Engine::Engine(const appt_options* opt) : pool_(0) {
   set_lds_per_cu(lds_cu > 0 ? lds_cu : lds_max_);
   APPT_HIP(hipHostMalloc((void**)&hs_, sizeof(HostStats), hipHostMallocDefault));
   for (auto& e : ev_) APPT_HIP(hipEventCreate(&e));
-
-  sym_params = default_params();
-  num_params = default_params();
-  int64_t group_budget = 64 * 1024;
-  if (opt) {
-    const appt_tuning& t = opt->tuning;
-    int nc = 0;
-    while (nc < APPT_MAX_CLASSES && nc < kNumCls && t.f_class_max[nc] > 0) nc++;
-    if (nc > 0) {
-      sym_params.ncls = num_params.ncls = nc;
-      for (int c = 0; c < nc; c++) { sym_params.cls_max[c] = t.f_class_max[c]; num_params.cls_max[c] = t.f_class_max[c]; }
-    }
-    if (t.lds_budget_bytes > 0) group_budget = t.lds_budget_bytes;
-    if (t.bitmap_words_factor > 0) num_params.dense_ratio = t.bitmap_words_factor;
-    for (int g = 0; g < 3; g++)
-      if (t.group_for_len[g] > 0) sym_params.glen_max[g] = num_params.glen_max[g] = t.group_for_len[g];
-  }
-  if (group_budget > lds_max_) group_budget = lds_max_;
-  set_lds_budget(group_budget);
-  // workgroup-per-row bins may use the device's LDS per block minus static LDS
-  const int32_t wg = (int32_t)(lds_max_ - 8192);
-  sym_params.wg_bytes = num_params.wg_bytes = wg;
-  if (opt && opt->tuning.wg_lds_bytes > 0) {
-    // cap the workgroup bins (rows above go to the heavy-row bin)
-    sym_params.wg_bytes = num_params.wg_bytes = std::min<int32_t>(wg, opt->tuning.wg_lds_bytes);
-  }
-  if (opt && opt->tuning.heavy_window_cols > 0) heavy_cap_ = opt->tuning.heavy_window_cols;
+  cfg_ = make_config(opt ? &opt->tuning : nullptr, lds_max_);
 }
 
 Engine::~Engine() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  Scratch* all[] = {&ws_.bptr,   &ws_.bcol,     &ws_.bval,    &ws_.cursor, &ws_.bmin,   &ws_.bmax,
-                    &ws_.rowf,   &ws_.rowlo,    &ws_.rowhi,   &ws_.symbin, &ws_.numbin, &ws_.rowcnt,
-                    &ws_.rowoff, &ws_.symrows,  &ws_.numrows, &ws_.scan_tmp, &ws_.stats, &ws_.rowwords,
-                    &ws_.wlist,  &ws_.statpart, &ws_.ctl,     &ws_.cur};
-  for (Scratch* s : all) free_scratch(*s);
+  free_workspace(ws_);
   pool_.trim();
   for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
   if (hs_) (void)hipHostFree(hs_);
@@ -171,10 +173,12 @@ void* Engine::ensure(Scratch& s, size_t bytes) {
   return s.p;
 }
 
-void Engine::free_scratch(Scratch& s) {
-  if (s.p) pool_.put(s.p);
-  s.p = nullptr;
-  s.cap = 0;
+void Engine::free_workspace(Workspace& W) {
+  W.each([&](Scratch& s) {
+    if (s.p) pool_.put(s.p);
+    s.p = nullptr;
+    s.cap = 0;
+  });
 }
 
 void Engine::record(int idx) {
@@ -191,60 +195,210 @@ void Engine::validate(const appt_csr_view& A) {
 template <typename T>
 static T* as(void* p) { return static_cast<T*>(p); }
 
-void Engine::prepass_stage(const appt_csr_view& A, appt_op op, const int32_t*& bp, const int32_t*& bc,
-                           const double*& bv) {
+static size_t stat_words() { return (size_t)kNumBins * kStatFields; }
+// stats layout: [sym stats | num stats | products (u64) | sym cursors | num cursors]
+static size_t stats_ints() { return stat_words() * 2 + 2 + 2 * kNumBins; }
+
+// ---------------------------------------------------------------------------
+// stages
+// ---------------------------------------------------------------------------
+void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Operands& O) {
   const int32_t n = A.n;
   const int64_t nnz = A.nnz;
   hipStream_t s = stream_;
-  bp = A.rowptr;
-  bc = A.col;
-  bv = A.val;
+  O = Operands{};
+  O.n = n;
+  O.annz = nnz;
+  O.ap = A.rowptr;
+  O.ac = A.col;
+  O.av = A.val;
+  O.bc = A.col;
+  O.bv = A.val;
+  O.bnnz = nnz;
+  O.upper = op == APPT_OP_AATS ? 1 : 0;
+  const int32_t* bptr = A.rowptr;
   if (op != APPT_OP_AA) {
-    int32_t* tp = as<int32_t>(ensure(ws_.bptr, sizeof(int32_t) * ((size_t)n + 1)));
-    int32_t* tc = as<int32_t>(ensure(ws_.bcol, sizeof(int32_t) * (size_t)nnz));
-    double* tv = as<double>(ensure(ws_.bval, sizeof(double) * (size_t)nnz));
-    int32_t* cnt = as<int32_t>(ensure(ws_.cursor, sizeof(int32_t) * ((size_t)n + 1)));
-    int64_t* off64 = as<int64_t>(ensure(ws_.rowoff, sizeof(int64_t) * ((size_t)n + 1)));
-    int64_t* tmp = as<int64_t>(ensure(ws_.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
+    // B = A^T: column histogram, scan, atomic-cursor scatter
+    int32_t* tp = as<int32_t>(ensure(W.bptr, sizeof(int32_t) * ((size_t)n + 1)));
+    int32_t* tc = as<int32_t>(ensure(W.bcol, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    double* tv = as<double>(ensure(W.bval, sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
+    int32_t* cnt = as<int32_t>(ensure(W.cursor, sizeof(int32_t) * ((size_t)n + 1)));
+    int64_t* off64 = as<int64_t>(ensure(W.rowoff, sizeof(int64_t) * ((size_t)n + 1)));
+    int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
     launch_fill_i32(cnt, n, 0, s);
     launch_colcount(nnz, A.col, cnt, s);
     launch_scan(cnt, n, off64, tp, tmp, s);
-    if (n > 0) APPT_HIP(hipMemcpyAsync(cnt, tp, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    launch_i64_to_i32(off64, n, cnt, s);
     launch_transpose_scatter(n, A.rowptr, A.col, A.val, cnt, tc, tv, s);
-    bp = tp; bc = tc; bv = tv;
+    O.bc = tc;
+    O.bv = tv;
+    bptr = tp;
   }
-  record(1);
-  int4* bmeta = as<int4>(ensure(ws_.bmin, sizeof(int4) * (size_t)std::max(n, 1)));
-  int64_t* rowf = as<int64_t>(ensure(ws_.rowf, sizeof(int64_t) * (size_t)std::max(n, 1)));
-  int32_t* rowlo = as<int32_t>(ensure(ws_.rowlo, sizeof(int32_t) * (size_t)std::max(n, 1)));
-  int32_t* rowhi = as<int32_t>(ensure(ws_.rowhi, sizeof(int32_t) * (size_t)std::max(n, 1)));
-  uint8_t* symbin = as<uint8_t>(ensure(ws_.symbin, (size_t)std::max(n, 1)));
-  // stats layout: [sym stats | num stats | products(u64) | cursors]
-  const size_t stat_words = (size_t)kNumBins * kStatFields;
-  int32_t* dstats = as<int32_t>(ensure(ws_.stats, sizeof(int32_t) * (stat_words * 2 + 4 + kNumBins)));
-  unsigned long long* dprod = reinterpret_cast<unsigned long long*>(dstats + stat_words * 2);
-  launch_fill_i32(dstats, (int64_t)(stat_words * 2 + 4 + kNumBins), 0, s);
-  const int upper = op == APPT_OP_AATS ? 1 : 0;
+  stage_bmeta(W, O, bptr, nullptr, nullptr, 0);
+}
+
+void Engine::stage_bmeta(Workspace& W, const Operands& O, const int32_t* ptr, const int32_t* ptr2,
+                         const uint8_t* sel, int64_t off2) {
+  hipStream_t s = stream_;
+  int4* bmeta = as<int4>(ensure(W.bmeta, sizeof(int4) * (size_t)std::max(O.n, 1)));
   // control words: [0] / [1] heavy-row claim counters, [2] B-unsorted flag
-  int32_t* ctl = as<int32_t>(ensure(ws_.ctl, sizeof(int32_t) * 4));
+  int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * 4));
   launch_fill_i32(ctl, 4, 0, s);
-  launch_rowstats(n, bp, bc, bmeta, ctl + 2, s);
-  int32_t* part = as<int32_t>(ensure(ws_.statpart, sizeof(int32_t) * (size_t)stats_part_elems() +
+  launch_rowstats(O.n, ptr, ptr2, sel, off2, O.bc, bmeta, ctl + 2, s);
+}
+
+void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
+  const int32_t n = O.n;
+  hipStream_t s = stream_;
+  const size_t nn = (size_t)std::max(n, 1);
+  const int4* bmeta = as<int4>(W.bmeta.p);
+  int64_t* rowf = as<int64_t>(ensure(W.rowf, sizeof(int64_t) * nn));
+  int32_t* rowlo = as<int32_t>(ensure(W.rowlo, sizeof(int32_t) * nn));
+  int32_t* rowhi = as<int32_t>(ensure(W.rowhi, sizeof(int32_t) * nn));
+  uint8_t* symbin = as<uint8_t>(ensure(W.symbin, nn));
+  int32_t* symrows = as<int32_t>(ensure(W.symrows, sizeof(int32_t) * nn));
+  int32_t* rowcnt = as<int32_t>(ensure(W.rowcnt, sizeof(int32_t) * nn));
+  int32_t* rowwords = as<int32_t>(ensure(W.rowwords, sizeof(int32_t) * nn));
+  int32_t* dstats = as<int32_t>(ensure(W.stats, sizeof(int32_t) * stats_ints()));
+  unsigned long long* dprod = reinterpret_cast<unsigned long long*>(dstats + stat_words() * 2);
+  int32_t* dcursor = dstats + stat_words() * 2 + 2;
+  int32_t* part = as<int32_t>(ensure(W.statpart, sizeof(int32_t) * (size_t)stats_part_elems() +
                                                       sizeof(unsigned long long) * kStatsMaxBlocks));
-  launch_prepass(n, A.rowptr, A.col, bmeta, upper, sym_params, rowf, rowlo, rowhi, symbin, dstats, dprod, part, s);
-  APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, sizeof(int32_t) * stat_words, hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipMemcpyAsync(&hs_->products, dprod, sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  launch_fill_i32(dstats, (int64_t)stats_ints(), 0, s);
+  launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part, s);
+  launch_bin_cursors(dstats, dcursor, s);
+  launch_bin_scatter(n, symbin, dcursor, symrows, s);
+  launch_fill_i32(rowcnt, n, 0, s);
+  launch_fill_i32(rowwords, n, 0, s);
+}
+
+void Engine::read_sym(Workspace& W, Plan& P) {
+  hipStream_t s = stream_;
+  const int32_t* dstats = as<int32_t>(W.stats.p);
+  APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, sizeof(int32_t) * stat_words(), hipMemcpyDeviceToHost, s));
+  APPT_HIP(hipMemcpyAsync(&hs_->products, dstats + stat_words() * 2, sizeof(unsigned long long),
+                          hipMemcpyDeviceToHost, s));
   APPT_HIP(hipStreamSynchronize(s));
+  std::memcpy(P.sym, hs_->sym, sizeof(P.sym));
+  P.products = (int64_t)hs_->products;
+  P.bins_used = 0;
+  for (int b = 1; b < kNumBins; b++) P.bins_used += P.sym[b * kStatFields] > 0 ? 1 : 0;
 }
 
-int Engine::plan_bins(const appt_csr_view& A, appt_op op) {
-  const int32_t* bp; const int32_t* bc; const double* bv;
-  prepass_stage(A, op, bp, bc, bv);
-  int used = 0;
-  for (int b = 1; b < kNumBins; b++) used += hs_->sym[b * kStatFields] > 0 ? 1 : 0;
-  return used;
+HeavyArgs Engine::heavy_args(Workspace& W, const Operands& O, const Config& cfg) {
+  HeavyArgs h{};
+  // dynamic LDS: the block's LDS less the heavy kernel's static staging (~21 KB)
+  const int64_t dyn = lds_max_ - 24 * 1024;
+  h.wc_sym = heavy_window_cols(false, dyn, cfg.heavy_cap);
+  h.wc_num = heavy_window_cols(true, dyn, cfg.heavy_cap);
+  h.work = as<int32_t>(W.ctl.p);
+  h.bflag = h.work + 2;
+  h.cur = as<int32_t>(ensure(W.cur, sizeof(int32_t) * (size_t)std::max<int64_t>(O.annz, 1)));
+  return h;
 }
 
+static RowArgs row_args(const Workspace& W, const Operands& O, const int32_t* rows, int32_t cnt, int upper,
+                        const BinParams& bp) {
+  RowArgs ra{};
+  ra.rows = rows;
+  ra.nrows = cnt;
+  ra.ap = O.ap;
+  ra.ac = O.ac;
+  ra.av = O.av;
+  ra.bc = O.bc;
+  ra.bv = O.bv;
+  ra.rowf = as<const int64_t>(W.rowf.p);
+  ra.rowlo = as<const int32_t>(W.rowlo.p);
+  ra.rowhi = as<const int32_t>(W.rowhi.p);
+  ra.rowwords = as<int32_t>(W.rowwords.p);
+  ra.upper = upper;
+  ra.bnnz = O.bnnz;
+  ra.bmeta = as<const int4>(W.bmeta.p);
+  ra.wlist = as<uint2>(W.wlist.p);
+  ra.lds_budget = bp.lds_budget;
+  return ra;
+}
+
+void Engine::stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, const Plan& P) {
+  hipStream_t s = stream_;
+  (void)ensure(W.wlist, sizeof(uint2) * kWordListCap * (size_t)std::max(O.n, 1));
+  const int32_t* symrows = as<int32_t>(W.symrows.p);
+  int32_t* rowcnt = as<int32_t>(W.rowcnt.p);
+  HeavyArgs heavy{};
+  if (P.sym[kBinHeavy * kStatFields] > 0) heavy = heavy_args(W, O, cfg);
+  int32_t off = 0;
+  for (int b = 1; b < kNumBins; b++) {
+    const int32_t cnt = P.sym[b * kStatFields];
+    if (cnt == 0) continue;
+    const int32_t mmax = P.sym[b * kStatFields + 1];
+    const int32_t wmax = P.sym[b * kStatFields + 2];
+    const int64_t rb = symbolic_row_lds_bytes(b, mmax, wmax);
+    if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "symbolic row exceeds LDS");
+    // structure passes compute full rows (AAtS mirrors values only)
+    RowArgs ra = row_args(W, O, symrows + off, cnt, 0, cfg.sym);
+    launch_symbolic_bin(b, ra, wmax, rowcnt, heavy, s);
+    off += cnt;
+  }
+}
+
+void Engine::stage_offsets(Workspace& W, const Operands& O, const Config& cfg, int32_t* rp32) {
+  const int32_t n = O.n;
+  hipStream_t s = stream_;
+  const size_t nn = (size_t)std::max(n, 1);
+  int64_t* rowoff = as<int64_t>(ensure(W.rowoff, sizeof(int64_t) * ((size_t)n + 1)));
+  int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
+  uint8_t* numbin = as<uint8_t>(ensure(W.numbin, nn));
+  int32_t* numrows = as<int32_t>(ensure(W.numrows, sizeof(int32_t) * nn));
+  int32_t* dstats = as<int32_t>(W.stats.p);
+  int32_t* dnum = dstats + stat_words();
+  int32_t* dcursor = dstats + stat_words() * 2 + 2 + kNumBins;
+  launch_scan(as<int32_t>(W.rowcnt.p), n, rowoff, rp32, tmp, s);
+  launch_numbin(n, O.ap, as<int64_t>(W.rowf.p), as<int32_t>(W.rowcnt.p), as<int32_t>(W.rowwords.p),
+                as<int32_t>(W.rowlo.p), as<int32_t>(W.rowhi.p), cfg.num, numbin, dnum, as<int32_t>(W.statpart.p), s);
+  launch_bin_cursors(dnum, dcursor, s);
+  launch_bin_scatter(n, numbin, dcursor, numrows, s);
+}
+
+void Engine::read_num(Workspace& W, const Operands& O, Plan& P) {
+  hipStream_t s = stream_;
+  const int32_t* dstats = as<int32_t>(W.stats.p);
+  APPT_HIP(hipMemcpyAsync(hs_->num, dstats + stat_words(), sizeof(int32_t) * stat_words(), hipMemcpyDeviceToHost, s));
+  APPT_HIP(hipMemcpyAsync(&hs_->total, as<int64_t>(W.rowoff.p) + O.n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  APPT_HIP(hipStreamSynchronize(s));
+  std::memcpy(P.num, hs_->num, sizeof(P.num));
+  P.nnz = hs_->total;
+}
+
+void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, const Plan& P, int32_t* ccol,
+                           double* cval) {
+  hipStream_t s = stream_;
+  const int32_t* numrows = as<int32_t>(W.numrows.p);
+  const int64_t* rowoff = as<int64_t>(W.rowoff.p);
+  HeavyArgs heavy{};
+  if (P.num[kBinHeavy * kStatFields] > 0) heavy = heavy_args(W, O, cfg);
+  int32_t off = 0;
+  for (int b = 1; b < kNumBins; b++) {
+    const int32_t cnt = P.num[b * kStatFields];
+    if (cnt == 0) continue;
+    const int32_t cmax = P.num[b * kStatFields + 1];
+    const int32_t wmax = P.num[b * kStatFields + 2];
+    const int64_t rb = numeric_row_lds_bytes(b, cmax, wmax);
+    if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "numeric row exceeds LDS");
+    // AAtS: values are formed for j >= i only; the structure is the full row
+    RowArgs ra = row_args(W, O, numrows + off, cnt, O.upper, cfg.num);
+    launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, heavy, s);
+    off += cnt;
+  }
+}
+
+void Engine::stage_mirror(Workspace& W, const Operands& O, const int32_t* ccol, double* cval) {
+  // AAtS: lower entries take the value of their upper mirror image
+  if (O.upper) launch_mirror_pull(O.n, as<int64_t>(W.rowoff.p), ccol, cval, stream_);
+}
+
+// ---------------------------------------------------------------------------
+// one call
+// ---------------------------------------------------------------------------
 void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   validate(A);
   if (!C) throw Error(APPT_ERR_INVALID, "result pointer is NULL");
@@ -254,129 +408,41 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
   times_ = appt_phase_times{};
   const int32_t n = A.n;
   hipStream_t s = stream_;
-  int syncs = 0;
+  Workspace& W = ws_;
+  Operands O;
+  Plan P;
   record(0);
-
-  const int32_t* bp = nullptr;
-  const int32_t* bc = nullptr;
-  const double* bv = nullptr;
-  prepass_stage(A, op, bp, bc, bv);
-  syncs++;
-  const int upper = op == APPT_OP_AATS ? 1 : 0;
-  int32_t* rowcnt = as<int32_t>(ensure(ws_.rowcnt, sizeof(int32_t) * (size_t)std::max(n, 1)));
-  int32_t* rowwords = as<int32_t>(ensure(ws_.rowwords, sizeof(int32_t) * (size_t)std::max(n, 1)));
-  int64_t* rowoff = as<int64_t>(ensure(ws_.rowoff, sizeof(int64_t) * ((size_t)n + 1)));
-  int32_t* symrows = as<int32_t>(ensure(ws_.symrows, sizeof(int32_t) * (size_t)std::max(n, 1)));
-  int32_t* numrows = as<int32_t>(ensure(ws_.numrows, sizeof(int32_t) * (size_t)std::max(n, 1)));
-  int64_t* tmp = as<int64_t>(ensure(ws_.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
-  const int64_t* rowf = as<int64_t>(ws_.rowf.p);
-  const int32_t* rowlo = as<int32_t>(ws_.rowlo.p);
-  const int32_t* rowhi = as<int32_t>(ws_.rowhi.p);
-  const uint8_t* symbin = as<uint8_t>(ws_.symbin.p);
-  const int4* bmeta = as<int4>(ws_.bmin.p);
-  uint2* wlist = as<uint2>(ensure(ws_.wlist, sizeof(uint2) * kWordListCap * (size_t)std::max(n, 1)));
-  uint8_t* numbin = as<uint8_t>(ensure(ws_.numbin, (size_t)std::max(n, 1)));
-  const size_t stat_words = (size_t)kNumBins * kStatFields;
-  int32_t* dstats = as<int32_t>(ws_.stats.p);
-  int32_t* dnum = dstats + stat_words;
-  int32_t* dcursor = dstats + stat_words * 2 + 4;
-  const int64_t products = (int64_t)hs_->products;
-  int32_t symoff[kNumBins + 1];
-  {
-    int32_t run = 0;
-    for (int b = 0; b < kNumBins; b++) {
-      symoff[b] = run;
-      hs_->cursor[b] = run;
-      if (b > 0) run += hs_->sym[b * kStatFields];
-    }
-    symoff[kNumBins] = run;
-  }
-  APPT_HIP(hipMemcpyAsync(dcursor, hs_->cursor, sizeof(int32_t) * kNumBins, hipMemcpyHostToDevice, s));
-  launch_bin_scatter(n, symbin, dcursor, symrows, s);
-  launch_fill_i32(rowcnt, n, 0, s);
-  launch_fill_i32(rowwords, n, 0, s);
+  stage_operands(W, A, op, O);
+  record(1);
+  stage_prepass(W, O, cfg_);
+  read_sym(W, P);
   record(2);
-
-  // heavy-row bin: LDS column windows, a cursor per A entry
-  HeavyArgs heavy{};
-  auto setup_heavy = [&]() {
-    if (heavy.work) return;
-    // dynamic LDS: the block's LDS less the kernel's static staging (~21 KB)
-    const int64_t dyn = lds_max_ - 24 * 1024;
-    heavy.wc_sym = heavy_window_cols(false, dyn, heavy_cap_);
-    heavy.wc_num = heavy_window_cols(true, dyn, heavy_cap_);
-    heavy.work = as<int32_t>(ws_.ctl.p);
-    heavy.bflag = heavy.work + 2;
-    heavy.cur = as<int32_t>(ensure(ws_.cur, sizeof(int32_t) * (size_t)std::max<int64_t>(A.nnz, 1)));
-  };
-  if (hs_->sym[kBinHeavy * kStatFields] > 0) setup_heavy();
-
-  // ---- symbolic -------------------------------------------------------------
-  int nbins_used = 0;
-  for (int b = 1; b < kNumBins; b++) {
-    const int32_t cnt = hs_->sym[b * kStatFields];
-    if (cnt == 0) continue;
-    nbins_used++;
-    const int32_t mmax = hs_->sym[b * kStatFields + 1];
-    const int32_t wmax = hs_->sym[b * kStatFields + 2];
-    RowArgs ra{symrows + symoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, 0, A.nnz, bmeta, wlist};
-    const int64_t rb = symbolic_row_lds_bytes(b, mmax, wmax);
-    if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "symbolic row exceeds LDS");
-    launch_symbolic_bin(b, ra, wmax, rowcnt, heavy, s);
-  }
+  stage_symbolic(W, O, cfg_, P);
   record(3);
-
-  // ---- row offsets, numeric binning ----------------------------------------
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)n + 1)));
-  launch_scan(rowcnt, n, rowoff, rp32, tmp, s);
-  launch_numbin(n, A.rowptr, rowf, rowcnt, rowwords, rowlo, rowhi, num_params, numbin, dnum,
-                as<int32_t>(ws_.statpart.p), s);
-  APPT_HIP(hipMemcpyAsync(hs_->num, dnum, sizeof(int32_t) * stat_words, hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipMemcpyAsync(&hs_->total, rowoff + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipStreamSynchronize(s));
-  syncs++;
-  const int64_t unnz = hs_->total;
-  int32_t numoff[kNumBins + 1];
-  {
-    int32_t run = 0;
-    for (int b = 0; b < kNumBins; b++) {
-      numoff[b] = run;
-      hs_->cursor[b] = run;
-      if (b > 0) run += hs_->num[b * kStatFields];
-    }
-    numoff[kNumBins] = run;
+  int32_t* ccol = nullptr;
+  double* cval = nullptr;
+  try {
+    stage_offsets(W, O, cfg_, rp32);
+    read_num(W, O, P);
+    record(4);
+    // output buffers (the symbolic pass sized C exactly)
+    ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(P.nnz, 1)));
+    cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(P.nnz, 1)));
+    stage_numeric(W, O, cfg_, P, ccol, cval);
+    record(5);
+    stage_mirror(W, O, ccol, cval);
+  } catch (...) {
+    pool_.put(rp32);
+    pool_.put(ccol);
+    pool_.put(cval);
+    throw;
   }
-  APPT_HIP(hipMemcpyAsync(dcursor, hs_->cursor, sizeof(int32_t) * kNumBins, hipMemcpyHostToDevice, s));
-  launch_bin_scatter(n, numbin, dcursor, numrows, s);
-  record(4);
-
-  // output buffers (the symbolic pass sized C exactly)
-  int32_t* ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(unnz, 1)));
-  double* cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(unnz, 1)));
-
-  // ---- numeric --------------------------------------------------------------
-  // AAtS: values are formed for j >= i only; the structure is the full row
-  for (int b = 1; b < kNumBins; b++) {
-    const int32_t cnt = hs_->num[b * kStatFields];
-    if (cnt == 0) continue;
-    const int32_t cmax = hs_->num[b * kStatFields + 1];
-    const int32_t wmax = hs_->num[b * kStatFields + 2];
-    RowArgs ra{numrows + numoff[b], cnt, A.rowptr, A.col, A.val, bp, bc, bv, rowf, rowlo, rowhi, rowwords, upper, A.nnz, bmeta, wlist};
-    const int64_t rb = numeric_row_lds_bytes(b, cmax, wmax);
-    if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "numeric row exceeds LDS");
-    if (b == kBinHeavy) setup_heavy();   // rows may become heavy only in the numeric binning
-    launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, heavy, s);
-  }
-  record(5);
-
-  // ---- AAtS: mirror the upper triangle into the lower entries -------------
-  if (upper) launch_mirror_pull(n, rowoff, ccol, cval, s);
-  const int64_t cnnz = unnz;
   void* rowptr_out = rp32;
-  if (cnnz > INT32_MAX) {
+  if (P.nnz > INT32_MAX) {
     pool_.put(rp32);
     int64_t* rp64 = as<int64_t>(pool_.get(sizeof(int64_t) * ((size_t)n + 1)));
-    launch_copy_i64(rowoff, (int64_t)n + 1, rp64, s);
+    launch_copy_i64(as<int64_t>(W.rowoff.p), (int64_t)n + 1, rp64, s);
     rowptr_out = rp64;
   }
   record(6);
@@ -385,17 +451,17 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
 
   Result* R = new Result{rowptr_out, ccol, cval};
   C->n = n;
-  C->nnz = cnnz;
-  C->rowptr_is_64 = cnnz > INT32_MAX ? 1 : 0;
+  C->nnz = P.nnz;
+  C->rowptr_is_64 = P.nnz > INT32_MAX ? 1 : 0;
   C->rowptr = rowptr_out;
   C->col = ccol;
   C->val = cval;
   C->_priv = R;
 
-  times_.products = products;
-  times_.nnz_c = cnnz;
-  times_.n_bins_used = nbins_used;
-  times_.sync_points = syncs + 1;
+  times_.products = P.products;
+  times_.nnz_c = P.nnz;
+  times_.n_bins_used = P.bins_used;
+  times_.sync_points = 3;
   if (timing_) {
     float ms[6];
     for (int k = 0; k < 6; k++) APPT_HIP(hipEventElapsedTime(&ms[k], ev_[k], ev_[k + 1]));

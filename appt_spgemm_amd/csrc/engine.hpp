@@ -53,9 +53,50 @@ struct Scratch {
   size_t cap = 0;
 };
 
+// Bin thresholds and launch shapes of one pipeline: the engine's defaults,
+// or a batch's own (autotuned) copy.
+struct Config {
+  BinParams sym{}, num{};
+  int32_t heavy_cap = 0;  // column-window cap of the heavy-row bin (0: as wide as LDS allows)
+};
+
+// Device scratch of one pipeline.  Buffers only grow, so a HIP graph that
+// captured their addresses stays valid while the owner lives.
 struct Workspace {
-  Scratch bptr, bcol, bval, cursor, bmin, bmax, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff,
-      symrows, numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur;
+  Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
+      numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur;
+  template <typename F>
+  void each(F&& f) {
+    Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
+                      &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
+                      &cur};
+    for (Scratch* s : all) f(*s);
+  }
+};
+
+// Operands of one pipeline run.  A side: output row i owns entries p in
+// [ap[i], ap[i+1]) with B-row id ac[p] and value av[p].  B side: row k is
+// bc/bv[bmeta[k].x, bmeta[k].y), bmeta built by the operand stage.
+struct Operands {
+  int32_t n = 0;  // output rows (= B rows)
+  int64_t annz = 0;
+  const int32_t* ap = nullptr;
+  const int32_t* ac = nullptr;
+  const double* av = nullptr;
+  const int32_t* bc = nullptr;
+  const double* bv = nullptr;
+  int64_t bnnz = 0;
+  int upper = 0;  // AAtS: values formed for j >= i, lower part mirrored
+};
+
+// Bin statistics read back at the two host synchronisations; they fix the
+// launch shapes of every later replay of the same structure.
+struct Plan {
+  int32_t sym[kNumBins * kStatFields] = {};
+  int32_t num[kNumBins * kStatFields] = {};
+  int64_t products = 0;
+  int64_t nnz = 0;
+  int bins_used = 0;
 };
 
 struct Result {  // what appt_csr_result::_priv points to
@@ -69,7 +110,6 @@ struct HostStats {  // pinned staging for the small device->host reads
   int32_t num[kNumBins * kStatFields];
   unsigned long long products;
   int64_t total;
-  int32_t cursor[kNumBins];
 };
 
 class Engine {
@@ -87,34 +127,101 @@ class Engine {
   Pool& pool() { return pool_; }
   void set_timing(bool t) { timing_ = t; }
   const appt_phase_times& times() const { return times_; }
+  const Config& config() const { return cfg_; }
+  void set_config(const Config& c) { cfg_ = c; }
+  int64_t lds_max() const { return lds_max_; }
+  static void validate(const appt_csr_view& A);
 
-  BinParams sym_params, num_params;
+  void* ensure(Scratch& s, size_t bytes);
+  void free_workspace(Workspace& W);
+
+  // ---- pipeline stages: kernel launches only.  No host synchronisation and,
+  // once the workspace has grown to the problem, no allocation, so a replay
+  // of the same structure can be captured into a HIP graph.
+  // B = A or A^T for one matrix, and the B-row metadata.
+  void stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Operands& O);
+  // B-row metadata from B's row starts/ends: row k is [ptr[k], ptr[k+1]) or,
+  // where sel[k] != 0, [off2 + ptr2[k], off2 + ptr2[k+1]).
+  void stage_bmeta(Workspace& W, const Operands& O, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel,
+                   int64_t off2);
+  void stage_prepass(Workspace& W, const Operands& O, const Config& cfg);
+  void stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, const Plan& P);
+  void stage_offsets(Workspace& W, const Operands& O, const Config& cfg, int32_t* rp32);
+  void stage_numeric(Workspace& W, const Operands& O, const Config& cfg, const Plan& P, int32_t* ccol,
+                     double* cval);
+  void stage_mirror(Workspace& W, const Operands& O, const int32_t* ccol, double* cval);
+  // host synchronisations: bin statistics of the symbolic / numeric binning
+  void read_sym(Workspace& W, Plan& P);
+  void read_num(Workspace& W, const Operands& O, Plan& P);
 
  private:
-  void* ensure(Scratch& s, size_t bytes);
-  void free_scratch(Scratch& s);
   void record(int idx);
-  void validate(const appt_csr_view& A);
-  // transpose (if op needs it), B-row spans, pre-pass and symbolic binning;
-  // ends with one host synchronisation (bin statistics in hs_->sym).
-  void prepass_stage(const appt_csr_view& A, appt_op op, const int32_t*& bp, const int32_t*& bc,
-                     const double*& bv);
-  int plan_bins(const appt_csr_view& A, appt_op op);
+  HeavyArgs heavy_args(Workspace& W, const Operands& O, const Config& cfg);
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
   Pool pool_;
   Workspace ws_;
+  Config cfg_;
   HostStats* hs_ = nullptr;
   bool timing_ = false;
   hipEvent_t ev_[8] = {};
   appt_phase_times times_ = {};
   int64_t lds_max_ = 65536;
-  int32_t heavy_cap_ = 0;  // column-window cap of the heavy-row bin (0: LDS-limited)
 };
 
-void set_lds_budget(int64_t b);
+Config make_config(const appt_tuning* t, int64_t lds_max);
+appt_tuning tuning_of(const Config& c);
+
+// Many independent matrices as one block-diagonal product (A*A or A*A^T per
+// matrix): one pipeline over the packed rows, planned once with the two
+// synchronisations, then replayed from a captured HIP graph.
+class Batch {
+ public:
+  Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz_each, const appt_op* ops);
+  ~Batch();
+  void inputs(appt_batch_buffers* out) const;
+  void upload(const appt_host_csr& packed);
+  void gather(const appt_csr_view* each);
+  void plan(bool use_graph);
+  void execute();
+  void result(int32_t k, appt_csr_result* out) const;
+  void autotune(int32_t reps, appt_tuning* chosen);
+  void info(appt_batch_info* out) const;
+
+ private:
+  void* alloc(size_t bytes);
+  void operands();           // kernels: B-row ids, transposes of A*A^T blocks, B-row metadata
+  void stages();             // kernels: operands .. numeric, per-matrix row pointers
+  void run_planning();       // stages with the synchronisations; sizes C
+  void drop_graph();
+  float time_eager(int reps);
+
+  Engine& E_;
+  Config cfg_;
+  Workspace W_;
+  Plan plan_{};
+  Operands O_{};
+  int32_t K_ = 0, N_ = 0;
+  int64_t nnz_a_ = 0, nnz_t_ = 0;
+  std::vector<int32_t> blk_row_h_;
+  std::vector<int64_t> blk_nnz_h_, blkoff_h_;
+  std::vector<void*> owned_;
+  // device
+  int32_t *in_rp_ = nullptr, *in_col_ = nullptr, *blk_row_ = nullptr, *rowbase_ = nullptr, *rowblk_ = nullptr;
+  int32_t *acg_ = nullptr, *tcnt_ = nullptr, *tp32_ = nullptr, *tcur_ = nullptr, *rp32_ = nullptr, *crp_ = nullptr;
+  int32_t* ccol_ = nullptr;
+  double *in_val_ = nullptr, *cval_ = nullptr;
+  int64_t *blk_nnz_ = nullptr, *toff_ = nullptr, *tscan_ = nullptr, *blkoff_ = nullptr;
+  uint8_t *blk_op_ = nullptr, *rowop_ = nullptr;
+  BatchSrc* src_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t exec_ = nullptr;
+  bool planned_ = false;
+  bool use_graph_ = true;
+};
+
 void set_last_error(const std::string& m);
 
 }  // namespace appt

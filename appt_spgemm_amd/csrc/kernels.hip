@@ -61,14 +61,22 @@ void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s) 
 // One 16-byte gather per A entry then serves the pre-pass and both product
 // passes.
 // ---------------------------------------------------------------------------
-__global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+__global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ ptr2,
+                           const uint8_t* __restrict__ sel, int64_t off2, const int32_t* __restrict__ col,
                            int4* __restrict__ meta, int32_t* __restrict__ flag) {
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
   bool unsorted = false;
   for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb) {
-    const int s = ptr[i], e = ptr[i + 1];
+    int s, e;
+    if (sel && sel[i]) {
+      s = (int)(off2 + ptr2[i]);
+      e = (int)(off2 + ptr2[i + 1]);
+    } else {
+      s = ptr[i];
+      e = ptr[i + 1];
+    }
     int mn = INT32_MAX, mx = -1;
     for (int p = s + lane; p < e; p += G) {
       const int c = col[p];
@@ -83,11 +91,12 @@ __global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int
   // one plain store per wave that saw a descent (all writers store 1)
   if (__ballot(unsorted) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
 }
-void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, int32_t* flag, hipStream_t s) {
+void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
+                     const int32_t* col, int4* meta, int32_t* flag, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_rowstats, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, meta, flag);
+  hipLaunchKernelGGL(k_rowstats, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, ptr2, sel, off2, col, meta, flag);
 }
 
 // ---------------------------------------------------------------------------
@@ -557,6 +566,22 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int32_t n, const uint8_t* _
     __syncthreads();
   }
 }
+// cursor[b] = first slot of bin b in the row list: exclusive scan of the
+// bin counts stats[b * kStatFields] over bins 1.. (bin 0 is not listed).
+__global__ void k_bin_cursors(const int32_t* __restrict__ stats, int32_t* __restrict__ cursor) {
+  if (threadIdx.x == 0) {
+    int32_t run = 0;
+    cursor[0] = 0;
+    for (int b = 1; b < kNumBins; b++) {
+      cursor[b] = run;
+      run += stats[b * kStatFields];
+    }
+  }
+}
+void launch_bin_cursors(const int32_t* stats, int32_t* cursor, hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_cursors, dim3(1), dim3(64), 0, s, stats, cursor);
+}
+
 void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n + 1023) / 1024;
@@ -611,8 +636,9 @@ __device__ __forceinline__ void for_each_product(int i, int lane, const RowArgs&
     double a = 0.0;
     if (p < e) {
       const int k = A.ac[p];
-      bs = A.bp[k];
-      be = A.bp[k + 1];
+      const int4 bm = A.bmeta[k];
+      bs = bm.x;
+      be = bm.y;
       if (VALS) a = A.av[p];
     }
     const int m = (e - base) < G ? (e - base) : G;
@@ -1547,14 +1573,12 @@ static int rows_per_block(int G, int64_t row_bytes, int64_t budget) {
   return R;
 }
 
-static int64_t g_lds_budget = 64 * 1024;
-void set_lds_budget(int64_t b) { g_lds_budget = b; }
 
 template <int G, bool DENSE>
 static void sym_lds(const RowArgs& a, int32_t words_max, int32_t* rowcnt, hipStream_t s) {
   const int32_t row_words = DENSE ? words_max : (int32_t)(sym_slots(words_max) * 2);
   const int64_t rb = (int64_t)row_words * 4;
-  const int R = G > kWave ? 1 : rows_per_block(G, rb, g_lds_budget);
+  const int R = G > kWave ? 1 : rows_per_block(G, rb, a.lds_budget);
   const size_t lds = (size_t)R * rb;
   const unsigned blocks = (unsigned)((a.nrows + R - 1) / R);
   (void)hipFuncSetAttribute((const void*)k_symbolic_lds<G, DENSE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1565,7 +1589,7 @@ template <int G, bool DENSE>
 static void num_lds(const RowArgs& a, int32_t words_max, int32_t c_max, const int64_t* rowoff, int32_t* ccol,
                     double* cval, hipStream_t s) {
   const int64_t rb = DENSE ? dense_row_bytes(words_max, c_max) : whash_row_bytes(words_max, c_max);
-  const int R = G > kWave ? 1 : rows_per_block(G, rb, g_lds_budget);
+  const int R = G > kWave ? 1 : rows_per_block(G, rb, a.lds_budget);
   const size_t lds = (size_t)R * rb;
   const unsigned blocks = (unsigned)((a.nrows + R - 1) / R);
   (void)hipFuncSetAttribute((const void*)k_numeric_lds<G, DENSE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1577,9 +1601,10 @@ static int64_t g_lds_per_cu = 160 * 1024;
 void set_lds_per_cu(int64_t b) { g_lds_per_cu = b > 0 ? b : 65536; }
 
 // Groups per block for a window kernel and the persistent grid.
-static void window_shape(int G, int64_t group_bytes, int32_t nrows, int* R_out, unsigned* blocks_out, int* chunk_out) {
+static void window_shape(int G, int64_t group_bytes, int32_t nrows, int64_t lds_budget, int* R_out,
+                         unsigned* blocks_out, int* chunk_out) {
   int64_t budget = g_lds_per_cu / 2;   // two blocks per CU
-  if (budget > g_lds_budget * 2) budget = g_lds_budget * 2;
+  if (budget > lds_budget * 2) budget = lds_budget * 2;
   int R = (int)(budget / (group_bytes > 0 ? group_bytes : 1));
   const int per_wave = kWave / G;
   if (R > 256 / G) R = 256 / G;
@@ -1601,7 +1626,7 @@ template <int G, bool UPPER, bool W64>
 static void sym_window_launch(const RowArgs& a, int32_t W, int32_t* rowcnt, hipStream_t s) {
   const int64_t gb = sym_window_bytes(G, W);
   int R; unsigned blocks; int chunk;
-  window_shape(G, gb, a.nrows, &R, &blocks, &chunk);
+  window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks, &chunk);
   const size_t lds = (size_t)R * gb;
   (void)hipFuncSetAttribute((const void*)k_sym_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((k_sym_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, chunk, rowcnt);
@@ -1611,7 +1636,7 @@ static void num_window_launch(const RowArgs& a, int32_t W, int32_t Cm, const int
                               double* cval, hipStream_t s) {
   const int64_t gb = num_window_bytes(G, W, Cm);
   int R; unsigned blocks; int chunk;
-  window_shape(G, gb, a.nrows, &R, &blocks, &chunk);
+  window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks, &chunk);
   const size_t lds = (size_t)R * gb;
   (void)hipFuncSetAttribute((const void*)k_num_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((k_num_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, Cm, chunk, rowoff,

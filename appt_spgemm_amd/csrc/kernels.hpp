@@ -32,6 +32,7 @@ struct BinParams {
   int32_t wg_bytes;               // LDS bytes of a workgroup-per-row bin
   int32_t glen_max[3];            // mean B-row length bounds for G = 8, 16, 32
   int32_t dense_ratio;            // numeric: dense if words(span) <= max(ratio * distinct words, 64)
+  int32_t lds_budget;             // LDS bytes a block of the wave-level bins may take
 };
 
 inline int bin_group(int bin) { return 8 << ((bin - 1) % 4); }
@@ -49,10 +50,11 @@ inline int64_t num_slots(int64_t c) { return np2_host((c * 3) / 2 + 1); }
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 // B-row metadata {start, end, min column, max column} (min = INT32_MAX,
 // max = -1 for an empty row).
-// flag[0] is set to 1 when some row's columns are not in nondecreasing order
-// (flag must be zero beforehand).
-void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* col, int4* meta, int32_t* flag,
-                     hipStream_t s);
+// Row k is [ptr[k], ptr[k+1]) of col, or [off2 + ptr2[k], off2 + ptr2[k+1])
+// where sel (optional) is non-zero.  flag[0] is set to 1 when some row's
+// columns are not in nondecreasing order (flag must be zero beforehand).
+void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
+                     const int32_t* col, int4* meta, int32_t* flag, hipStream_t s);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
                               int32_t* cursor, int32_t* bc, double* bv, hipStream_t s);
@@ -69,19 +71,23 @@ void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4*
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
                    const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
                    uint8_t* bin, int32_t* stats, int32_t* part, hipStream_t s);
+// cursor[b] = start of bin b in the row list (exclusive scan of the counts
+// in a stats array of kNumBins * kStatFields).
+void launch_bin_cursors(const int32_t* stats, int32_t* cursor, hipStream_t s);
 void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s);
 
 struct RowArgs {
   const int32_t* rows;   // row ids of this bin
   int32_t nrows;
   const int32_t* ap; const int32_t* ac; const double* av;
-  const int32_t* bp; const int32_t* bc; const double* bv;
+  const int32_t* bc; const double* bv;   // B rows are addressed through bmeta
   const int64_t* rowf; const int32_t* rowlo; const int32_t* rowhi;
   int32_t* rowwords;     // distinct 32-column words per output row (symbolic writes, numeric reads)
   int upper;
   int64_t bnnz;          // nnz of B (selects 32- or 64-bit gather offsets)
   const int4* bmeta;     // per B row: {start, end, min column, max column}
   uint2* wlist;          // per output row: up to 32 sorted (word, mask) pairs from the symbolic pass
+  int32_t lds_budget;    // LDS bytes a block of the wave-level bins may take
 };
 constexpr int kWordListCap = 32;
 
@@ -112,6 +118,27 @@ int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max);
 // AAtS: fill the lower entries of the complete, sorted C from row r's upper
 // part (C's structure is symmetric; values were formed for j >= i only).
 void launch_mirror_pull(int32_t n, const int64_t* off, const int32_t* ccol, double* cval, hipStream_t s);
+
+// ---- batched path (block-diagonal packing of many matrices) ----
+struct BatchSrc {  // one separate device matrix to pack
+  const int32_t* rowptr;
+  const int32_t* col;
+  const double* val;
+  int32_t n;
+  int64_t nnz;
+};
+void launch_batch_rows(int32_t N, int32_t K, const int32_t* blk_row, const uint8_t* blk_op, int32_t* rowbase,
+                       int32_t* rowblk, uint8_t* rowop, hipStream_t s);
+void launch_batch_acg(int32_t N, const int32_t* rp, const int32_t* col, const int32_t* rowbase,
+                      const uint8_t* rowop, int32_t* acg, int32_t* tcnt, hipStream_t s);
+void launch_batch_tscatter(int32_t N, const int32_t* rp, const int32_t* acg, const int32_t* rowbase,
+                           const uint8_t* rowop, int32_t* cursor, int64_t nnz_a, int32_t* bcol, double* bval,
+                           hipStream_t s);
+void launch_batch_local_rowptr(int32_t N, int32_t K, const int64_t* rowoff, const int32_t* rowbase,
+                               const int32_t* rowblk, const int32_t* blk_row, int32_t* crp, int64_t* blkoff,
+                               hipStream_t s);
+void launch_batch_gather(int32_t K, const BatchSrc* src, const int32_t* blk_row, const int64_t* blk_nnz, int32_t* rp,
+                         int32_t* col, double* val, hipStream_t s);
 
 void set_num_cus(int n);
 void set_lds_per_cu(int64_t b);

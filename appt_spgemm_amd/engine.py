@@ -16,8 +16,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import (OPS, CsrResult, CsrView, Descriptors, EngResult, HostCsr, Options, PhaseTimes,
-                   Tuning, check, lib)
+from ._abi import (OPS, BatchBuffers, BatchInfo, CsrResult, CsrView, Descriptors, EngResult, HostCsr,
+                   Options, PhaseTimes, Tuning, check, lib)
 
 
 # ---------------------------------------------------------------------------
@@ -243,6 +243,90 @@ class Engine:
         check(lib().appt_dispatch_descriptors(self.handle, C.byref(A.view), OPS[op], C.byref(d)),
               "dispatch_descriptors")
         return {f: getattr(d, f) for f, _ in Descriptors._fields_}
+
+
+def _tuning_dict(t: Tuning) -> dict:
+    return {"f_class_max": [int(x) for x in t.f_class_max if x > 0], "wg_lds_bytes": int(t.wg_lds_bytes),
+            "lds_budget_bytes": int(t.lds_budget_bytes), "bitmap_words_factor": int(t.bitmap_words_factor),
+            "group_for_len": [int(x) for x in t.group_for_len[:3]], "heavy_window_cols": int(t.heavy_window_cols)}
+
+
+def pack(mats: list) -> HostMatrix:
+    """Packed batch layout: global row pointers, column indices local to each matrix."""
+    nnz = np.array([m.nnz for m in mats], dtype=np.int64)
+    base = np.concatenate([[0], np.cumsum(nnz)])
+    parts = [np.asarray(m.rowptr[:-1], np.int64) + base[k] for k, m in enumerate(mats)]
+    rp = np.concatenate(parts + [np.array([base[-1]], np.int64)]) if mats else np.zeros(1, np.int64)
+    col = np.concatenate([np.asarray(m.col, np.int32) for m in mats]) if mats else np.zeros(0, np.int32)
+    val = np.concatenate([np.asarray(m.val, np.float64) for m in mats]) if mats else np.zeros(0)
+    return HostMatrix(int(sum(m.n for m in mats)), rp.astype(np.int32), col, val)
+
+
+class Batch:
+    """Many matrices (A_k*A_k or A_k*A_k^T each) as one block-diagonal product:
+    planned once, then replayed from a captured HIP graph (``appt_batch_*``)."""
+
+    def __init__(self, engine: Engine, ns, nnzs, ops):
+        self._engine = engine
+        self.count = len(ns)
+        n_arr = (C.c_int32 * max(self.count, 1))(*[int(x) for x in ns])
+        z_arr = (C.c_int64 * max(self.count, 1))(*[int(x) for x in nnzs])
+        o_arr = (C.c_int * max(self.count, 1))(*[OPS[o] for o in ops])
+        h = C.c_void_p()
+        check(lib().appt_batch_create(engine.handle, self.count, n_arr, z_arr, o_arr, C.byref(h)),
+              "appt_batch_create")
+        self.handle = h
+
+    @classmethod
+    def from_host(cls, engine: Engine, mats: list, ops: list) -> "Batch":
+        b = cls(engine, [m.n for m in mats], [m.nnz for m in mats], ops)
+        b.upload(pack(mats))
+        return b
+
+    def upload(self, packed: HostMatrix) -> None:
+        h = packed._as_c()
+        check(lib().appt_batch_upload(self.handle, C.byref(h)), "appt_batch_upload")
+
+    def gather(self, mats: list) -> None:
+        arr = (CsrView * max(len(mats), 1))(*[m.view for m in mats])
+        check(lib().appt_batch_gather(self.handle, arr), "appt_batch_gather")
+
+    def plan(self, graph: bool = True) -> None:
+        check(lib().appt_batch_plan(self.handle, 1 if graph else 0), "appt_batch_plan")
+
+    def autotune(self, reps: int = 2) -> dict:
+        t = Tuning()
+        check(lib().appt_batch_autotune(self.handle, reps, C.byref(t)), "appt_batch_autotune")
+        return _tuning_dict(t)
+
+    def execute(self) -> None:
+        check(lib().appt_batch_execute(self.handle), "appt_batch_execute")
+
+    def result(self, k: int) -> DeviceResult:
+        r = CsrResult()
+        check(lib().appt_batch_result(self.handle, k, C.byref(r)), "appt_batch_result")
+        return DeviceResult(self._engine, r)
+
+    def download(self, k: int) -> HostMatrix:
+        return self.result(k).download()
+
+    def info(self) -> dict:
+        i = BatchInfo()
+        check(lib().appt_batch_info_get(self.handle, C.byref(i)), "appt_batch_info_get")
+        d = {f: getattr(i, f) for f, _ in BatchInfo._fields_ if f != "tuning"}
+        d["tuning"] = _tuning_dict(i.tuning)
+        return d
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().appt_batch_destroy(self.handle), "appt_batch_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 def run_ours(task: str, dOff: int, dCol: int, dVal: int, n: int, nnz: int, reps: int) -> EngResult:

@@ -166,6 +166,60 @@ appt_status appt_dispatch_descriptors(appt_engine* eng, const appt_csr_view* A, 
 /* Parse "AA" | "AAt" | "AAtS" (src/spgemm_engine.cuh:134). */
 appt_status appt_op_from_task(const char* task, appt_op* out);
 
+/* ---- batched path --------------------------------------------------------------
+ * Many independent matrices (each C_k = A_k*A_k or A_k*A_k^T) run as one
+ * block-diagonal product: one pipeline over all packed rows, planned once (two
+ * host synchronisations), then replayed as a captured HIP graph with no host
+ * synchronisation inside.  The structure of the inputs is fixed at plan time;
+ * values may change between executions.  No counterpart in the reference,
+ * which runs one matrix per call (src/spgemm_engine.cuh:132). */
+typedef struct appt_batch appt_batch;
+
+/* Packed device input of a batch, written by the caller (or appt_batch_gather):
+ * matrix k owns rows [row0_k, row0_k + n_k) with row0 the prefix sum of n_each,
+ * rowptr holds global entry offsets (n_total + 1 entries), col holds indices
+ * local to each matrix (0 .. n_k-1), val the values. */
+typedef struct appt_batch_buffers {
+  int32_t n_total;
+  int64_t nnz;
+  int32_t* rowptr;
+  int32_t* col;
+  double* val;
+} appt_batch_buffers;
+
+typedef struct appt_batch_info {
+  int32_t count;
+  int32_t n_total;
+  int64_t nnz_a;             /* packed input entries */
+  int64_t nnz_c;             /* sum of the results' nnz */
+  int64_t products;          /* sum of the matrices' product counts (2*products = FLOPs) */
+  int32_t graph;             /* 1 when execute replays a captured HIP graph */
+  int32_t bins_used;
+  appt_tuning tuning;        /* bin configuration in use */
+} appt_batch_info;
+
+/* ops[k] is APPT_OP_AA or APPT_OP_AAT; sum of nnz_each < 2^31, sum of n_each + count < 2^31. */
+appt_status appt_batch_create(appt_engine* eng, int32_t count, const int32_t* n_each, const int64_t* nnz_each,
+                              const appt_op* ops, appt_batch** out);
+appt_status appt_batch_destroy(appt_batch* b);
+appt_status appt_batch_inputs(appt_batch* b, appt_batch_buffers* out);
+/* Copy a packed host batch (same layout, n = n_total, nnz = sum of nnz_each) into the packed input. */
+struct appt_host_csr;
+appt_status appt_batch_upload(appt_batch* b, const struct appt_host_csr* packed);
+/* Copy `count` separate device matrices (each rowptr starting at 0) into the packed input. */
+appt_status appt_batch_gather(appt_batch* b, const appt_csr_view* each);
+/* Planning run (sizes, C buffers); use_graph != 0 then captures the replay graph. */
+appt_status appt_batch_plan(appt_batch* b, int32_t use_graph);
+/* Time candidate bin configurations (reps executions each) on this batch,
+ * keep the fastest and re-plan with it. */
+appt_status appt_batch_autotune(appt_batch* b, int32_t reps, appt_tuning* chosen);
+/* Run the planned batch; synchronous on return. */
+appt_status appt_batch_execute(appt_batch* b);
+/* View of C_k inside the batch's buffers (int32 rowptr, local columns); valid
+ * until the batch is re-planned or destroyed; not to be passed to appt_csr_release. */
+appt_status appt_batch_result(const appt_batch* b, int32_t k, appt_csr_result* out);
+appt_status appt_batch_info_get(const appt_batch* b, appt_batch_info* out);
+
 /* ---- host CSR, CSR1 files and synthetic generators --------------------------- */
 /* Host CSR owned by the library (malloc'd); free with appt_host_csr_free. */
 typedef struct appt_host_csr {
