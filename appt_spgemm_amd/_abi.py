@@ -84,6 +84,28 @@ class BatchInfo(C.Structure):
                 ("bins_used", C.c_int32), ("tuning", Tuning)]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("slots", C.c_int32), ("hbm_budget_bytes", C.c_int64),
+                ("batch_max_jobs", C.c_int32), ("batch_row_limit", C.c_int32),
+                ("admission_wait_ms", C.c_int32), ("tuning", Tuning)]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("group", C.c_int32),
+                ("queue_ms", C.c_double), ("run_ms", C.c_double), ("latency_ms", C.c_double),
+                ("products", C.c_int64), ("reserved_bytes", C.c_int64), ("C", CsrResult)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("refused", C.c_int64),
+                ("failed", C.c_int64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
+                ("mean_ms", C.c_double), ("gflops", C.c_double), ("jobs_per_s", C.c_double),
+                ("bytes_reserved", C.c_int64), ("peak_bytes_reserved", C.c_int64),
+                ("groups_run", C.c_int64)]
+
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "released"}
+
 P = C.POINTER
 _SIGNATURES = {
     "appt_last_error": (C.c_char_p, []),
@@ -120,6 +142,13 @@ _SIGNATURES = {
     "appt_batch_execute": (C.c_int, [C.c_void_p]),
     "appt_batch_result": (C.c_int, [C.c_void_p, C.c_int32, P(CsrResult)]),
     "appt_batch_info_get": (C.c_int, [C.c_void_p, P(BatchInfo)]),
+    "appt_server_create": (C.c_int, [P(ServerOptions), P(C.c_void_p)]),
+    "appt_server_destroy": (C.c_int, [C.c_void_p]),
+    "appt_server_submit": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, P(C.c_uint64)]),
+    "appt_server_poll": (C.c_int, [C.c_void_p, C.c_uint64, P(JobInfo)]),
+    "appt_server_wait": (C.c_int, [C.c_void_p, C.c_uint64, P(JobInfo)]),
+    "appt_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "appt_server_stats_get": (C.c_int, [C.c_void_p, P(ServerStats)]),
     "appt_device_upload": (C.c_int, [C.c_void_p, P(HostCsr), P(CsrView)]),
     "appt_device_free_view": (C.c_int, [C.c_void_p, P(CsrView)]),
     "appt_csr_download": (C.c_int, [C.c_void_p, P(CsrResult), P(C.c_int64), P(C.c_int32),

@@ -36,7 +36,7 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__",
              f"-I{ROCM / 'include'}"]
 
 HIP_SOURCES = ["kernels.hip", "descriptors.hip", "batch.hip"]
-CXX_SOURCES = ["engine.cpp", "batch.cpp", "capi.cpp", "gen.cpp"]
+CXX_SOURCES = ["engine.cpp", "batch.cpp", "server.cpp", "capi.cpp", "gen.cpp"]
 
 
 def _hipcc() -> str:

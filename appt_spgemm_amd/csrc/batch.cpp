@@ -80,6 +80,7 @@ Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz
   rp32_ = as<int32_t>(alloc(sizeof(int32_t) * n1));
   crp_ = as<int32_t>(alloc(sizeof(int32_t) * ((size_t)N_ + K_ + 1)));
   blkoff_ = as<int64_t>(alloc(sizeof(int64_t) * ((size_t)K_ + 1)));
+  blkprod_ = as<int64_t>(alloc(sizeof(int64_t) * ((size_t)K_ + 1)));
   src_ = as<BatchSrc>(alloc(sizeof(BatchSrc) * (size_t)std::max(K_, 1)));
   APPT_HIP(hipMemcpyAsync(blk_row_, blk_row_h_.data(), sizeof(int32_t) * ((size_t)K_ + 1), hipMemcpyHostToDevice, s));
   APPT_HIP(hipMemcpyAsync(blk_nnz_, blk_nnz_h_.data(), sizeof(int64_t) * ((size_t)K_ + 1), hipMemcpyHostToDevice, s));
@@ -183,6 +184,7 @@ void Batch::run_planning() {
   E_.stage_symbolic(W_, O_, cfg_, plan_);
   E_.stage_offsets(W_, O_, cfg_, rp32_);
   E_.read_num(W_, O_, plan_);
+  if (E_.admission()) E_.admission()(result_bytes(N_, plan_.nnz) + 4 * (int64_t)K_);
   const size_t cap = (size_t)std::max<int64_t>(plan_.nnz, 1);
   if (ccol_) E_.pool().put(ccol_);
   if (cval_) E_.pool().put(cval_);
@@ -192,8 +194,12 @@ void Batch::run_planning() {
   cval_ = as<double>(E_.pool().get(sizeof(double) * cap));
   E_.stage_numeric(W_, O_, cfg_, plan_, ccol_, cval_);
   launch_batch_local_rowptr(N_, K_, as<int64_t>(W_.rowoff.p), rowbase_, rowblk_, blk_row_, crp_, blkoff_, s);
+  launch_batch_block_sum(K_, blk_row_, as<int64_t>(W_.rowf.p), blkprod_, s);
   blkoff_h_.assign((size_t)K_ + 1, 0);
+  blkprod_h_.assign((size_t)K_ + 1, 0);
   APPT_HIP(hipMemcpyAsync(blkoff_h_.data(), blkoff_, sizeof(int64_t) * ((size_t)K_ + 1), hipMemcpyDeviceToHost, s));
+  if (K_ > 0)
+    APPT_HIP(hipMemcpyAsync(blkprod_h_.data(), blkprod_, sizeof(int64_t) * (size_t)K_, hipMemcpyDeviceToHost, s));
   APPT_HIP(hipGetLastError());
   APPT_HIP(hipStreamSynchronize(s));
   for (int32_t k = 0; k < K_; k++)

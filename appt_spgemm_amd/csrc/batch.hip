@@ -128,6 +128,24 @@ void launch_batch_local_rowptr(int32_t N, int32_t K, const int64_t* rowoff, cons
                      blk_row, crp, blkoff);
 }
 
+// out[b] = sum of rowf over matrix b's rows (its product count).
+__global__ void k_batch_block_sum(int32_t K, const int32_t* __restrict__ blk_row, const int64_t* __restrict__ rowf,
+                                  int64_t* __restrict__ out) {
+  __shared__ int64_t red[17];
+  for (int b = blockIdx.x; b < K; b += gridDim.x) {
+    int64_t v = 0;
+    for (int r = blk_row[b] + threadIdx.x; r < blk_row[b + 1]; r += blockDim.x) v += rowf[r];
+    int64_t tot;
+    (void)block_scan_excl<int64_t>(v, red, &tot);
+    if (threadIdx.x == 0) out[b] = tot;
+  }
+}
+void launch_batch_block_sum(int32_t K, const int32_t* blk_row, const int64_t* rowf, int64_t* out, hipStream_t s) {
+  if (K <= 0) return;
+  const unsigned blocks = (unsigned)(K < 4096 ? K : 4096);
+  hipLaunchKernelGGL(k_batch_block_sum, dim3(blocks), dim3(256), 0, s, K, blk_row, rowf, out);
+}
+
 // Copy K separate device matrices into the packed buffers, one block per
 // matrix at a time: rows get global entry offsets, entries are copied as is.
 __global__ void k_batch_gather(int32_t K, const BatchSrc* __restrict__ src, const int32_t* __restrict__ blk_row,

@@ -2,6 +2,7 @@
 // exceptions and returns a status; nothing here exits the process (the
 // reference's CK macro exits, src/spgemm_engine.cuh:18).
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 
@@ -41,6 +42,10 @@ static appt_status guard(F&& f) {
     return APPT_ERR_HIP;
   }
 }
+
+namespace appt {
+appt_status guard_call(const std::function<void()>& f) { return guard(f); }
+}  // namespace appt
 
 extern "C" {
 
@@ -230,9 +235,9 @@ appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v) {
 
 appt_status appt_csr_download(appt_engine* eng, const appt_csr_result* C, int64_t* rowptr, int32_t* col,
                               double* val) {
-  if (!eng || !C) return APPT_ERR_INVALID;
+  if (!C) return APPT_ERR_INVALID;
   return guard([&] {
-    APPT_HIP(hipSetDevice(eng->impl->device()));
+    if (eng) APPT_HIP(hipSetDevice(eng->impl->device()));
     const size_t n1 = (size_t)C->n + 1;
     if (rowptr) {
       if (C->rowptr_is_64) {

@@ -28,11 +28,15 @@ namespace appt {
 // ---------------------------------------------------------------------------
 // Pool
 // ---------------------------------------------------------------------------
-Pool::~Pool() { trim(); for (auto& kv : size_of_) (void)hipFree(kv.first); }
+Pool::~Pool() {
+  trim_locked();
+  for (auto& kv : size_of_) (void)hipFree(kv.first);
+}
 
 void* Pool::get(size_t bytes) {
   if (bytes == 0) bytes = 256;
   bytes = (bytes + 255) & ~(size_t)255;
+  std::lock_guard<std::mutex> lk(mu_);
   auto it = free_.lower_bound(bytes);
   if (it != free_.end() && it->first <= 2 * bytes) {
     void* p = it->second;
@@ -40,7 +44,7 @@ void* Pool::get(size_t bytes) {
     return p;
   }
   if (budget_ > 0 && allocated_ + (int64_t)bytes > budget_) {
-    trim();
+    trim_locked();
     if (allocated_ + (int64_t)bytes > budget_)
       throw Error(APPT_ERR_OOM, "engine HBM budget exceeded: need " + std::to_string(bytes) + " B, " +
                                     std::to_string(allocated_) + " of " + std::to_string(budget_) + " B in use");
@@ -49,7 +53,7 @@ void* Pool::get(size_t bytes) {
   hipError_t e = hipMalloc(&p, bytes);
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    trim();
+    trim_locked();
     e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       (void)hipGetLastError();
@@ -63,12 +67,18 @@ void* Pool::get(size_t bytes) {
 
 void Pool::put(void* p) {
   if (!p) return;
+  std::lock_guard<std::mutex> lk(mu_);
   auto it = size_of_.find(p);
   if (it == size_of_.end()) return;
   free_.emplace(it->second, p);
 }
 
 void Pool::trim() {
+  std::lock_guard<std::mutex> lk(mu_);
+  trim_locked();
+}
+
+void Pool::trim_locked() {
   for (auto& kv : free_) {
     (void)hipFree(kv.second);
     allocated_ -= (int64_t)size_of_[kv.second];
@@ -194,6 +204,10 @@ void Engine::validate(const appt_csr_view& A) {
 
 template <typename T>
 static T* as(void* p) { return static_cast<T*>(p); }
+
+int64_t result_bytes(int32_t n, int64_t nnz) {
+  return (nnz > INT32_MAX ? 8 : 4) * ((int64_t)n + 1) + 12 * nnz;
+}
 
 static size_t stat_words() { return (size_t)kNumBins * kStatFields; }
 // stats layout: [sym stats | num stats | products (u64) | sym cursors | num cursors]
@@ -426,6 +440,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     stage_offsets(W, O, cfg_, rp32);
     read_num(W, O, P);
     record(4);
+    if (admit_) admit_(result_bytes(n, P.nnz));
     // output buffers (the symbolic pass sized C exactly)
     ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(P.nnz, 1)));
     cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(P.nnz, 1)));

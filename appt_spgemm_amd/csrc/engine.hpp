@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <functional>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -28,7 +30,7 @@ struct Error : std::runtime_error {
 
 // Device memory pool: exact-capacity buffers kept on a free list and reused
 // when a request fits in [bytes, 2*bytes].  Accounts every byte against the
-// engine budget.
+// engine budget.  Thread-safe (a server releases results from other threads).
 class Pool {
  public:
   explicit Pool(int64_t budget) : budget_(budget) {}
@@ -41,6 +43,8 @@ class Pool {
   void set_budget(int64_t b) { budget_ = b; }
 
  private:
+  void trim_locked();
+  std::mutex mu_;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> size_of_;
   int64_t allocated_ = 0;
@@ -134,6 +138,11 @@ class Engine {
 
   void* ensure(Scratch& s, size_t bytes);
   void free_workspace(Workspace& W);
+  // Called with the exact bytes of C once the symbolic phase has sized it,
+  // before C is allocated; may block, or throw to refuse the call.
+  using Admission = std::function<void(int64_t)>;
+  void set_admission(Admission f) { admit_ = std::move(f); }
+  const Admission& admission() const { return admit_; }
 
   // ---- pipeline stages: kernel launches only.  No host synchronisation and,
   // once the workspace has grown to the problem, no allocation, so a replay
@@ -169,9 +178,12 @@ class Engine {
   hipEvent_t ev_[8] = {};
   appt_phase_times times_ = {};
   int64_t lds_max_ = 65536;
+  Admission admit_;
 };
 
 Config make_config(const appt_tuning* t, int64_t lds_max);
+// Device bytes of a CSR result with n rows and nnz entries.
+int64_t result_bytes(int32_t n, int64_t nnz);
 appt_tuning tuning_of(const Config& c);
 
 // Many independent matrices as one block-diagonal product (A*A or A*A^T per
@@ -189,6 +201,7 @@ class Batch {
   void result(int32_t k, appt_csr_result* out) const;
   void autotune(int32_t reps, appt_tuning* chosen);
   void info(appt_batch_info* out) const;
+  int64_t products(int32_t k) const { return blkprod_h_.at((size_t)k); }
 
  private:
   void* alloc(size_t bytes);
@@ -206,14 +219,14 @@ class Batch {
   int32_t K_ = 0, N_ = 0;
   int64_t nnz_a_ = 0, nnz_t_ = 0;
   std::vector<int32_t> blk_row_h_;
-  std::vector<int64_t> blk_nnz_h_, blkoff_h_;
+  std::vector<int64_t> blk_nnz_h_, blkoff_h_, blkprod_h_;
   std::vector<void*> owned_;
   // device
   int32_t *in_rp_ = nullptr, *in_col_ = nullptr, *blk_row_ = nullptr, *rowbase_ = nullptr, *rowblk_ = nullptr;
   int32_t *acg_ = nullptr, *tcnt_ = nullptr, *tp32_ = nullptr, *tcur_ = nullptr, *rp32_ = nullptr, *crp_ = nullptr;
   int32_t* ccol_ = nullptr;
   double *in_val_ = nullptr, *cval_ = nullptr;
-  int64_t *blk_nnz_ = nullptr, *toff_ = nullptr, *tscan_ = nullptr, *blkoff_ = nullptr;
+  int64_t *blk_nnz_ = nullptr, *toff_ = nullptr, *tscan_ = nullptr, *blkoff_ = nullptr, *blkprod_ = nullptr;
   uint8_t *blk_op_ = nullptr, *rowop_ = nullptr;
   BatchSrc* src_ = nullptr;
   hipGraph_t graph_ = nullptr;

@@ -137,6 +137,7 @@ void launch_batch_tscatter(int32_t N, const int32_t* rp, const int32_t* acg, con
 void launch_batch_local_rowptr(int32_t N, int32_t K, const int64_t* rowoff, const int32_t* rowbase,
                                const int32_t* rowblk, const int32_t* blk_row, int32_t* crp, int64_t* blkoff,
                                hipStream_t s);
+void launch_batch_block_sum(int32_t K, const int32_t* blk_row, const int64_t* rowf, int64_t* out, hipStream_t s);
 void launch_batch_gather(int32_t K, const BatchSrc* src, const int32_t* blk_row, const int64_t* blk_nnz, int32_t* rp,
                          int32_t* col, double* val, hipStream_t s);
 

@@ -16,8 +16,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import (OPS, BatchBuffers, BatchInfo, CsrResult, CsrView, Descriptors, EngResult, HostCsr,
-                   Options, PhaseTimes, Tuning, check, lib)
+from ._abi import (JOB_STATES, OPS, STATUS_NAMES, BatchBuffers, BatchInfo, CsrResult, CsrView, Descriptors,
+                   EngResult, HostCsr, JobInfo, Options, PhaseTimes, ServerOptions, ServerStats, Tuning, check,
+                   lib)
 
 
 # ---------------------------------------------------------------------------
@@ -147,7 +148,7 @@ class DeviceResult:
         rp = np.empty(n + 1, dtype=np.int64)
         col = np.empty(max(nnz, 1), dtype=np.int32)
         val = np.empty(max(nnz, 1), dtype=np.float64)
-        check(lib().appt_csr_download(self._engine.handle, C.byref(self.res),
+        check(lib().appt_csr_download(self._engine.handle if self._engine else None, C.byref(self.res),
                                       rp.ctypes.data_as(C.POINTER(C.c_int64)),
                                       col.ctypes.data_as(C.POINTER(C.c_int32)),
                                       val.ctypes.data_as(C.POINTER(C.c_double))), "csr_download")
@@ -320,6 +321,87 @@ class Batch:
     def close(self) -> None:
         if self.handle:
             check(lib().appt_batch_destroy(self.handle), "appt_batch_destroy")
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _tuning_struct(tuning: dict | None) -> Tuning:
+    t = Tuning()
+    for k, v in (tuning or {}).items():
+        if k in ("f_class_max", "group_for_len"):
+            arr = getattr(t, k)
+            for i, x in enumerate(v):
+                arr[i] = int(x)
+        else:
+            setattr(t, k, int(v))
+    return t
+
+
+class Server:
+    """Request loop over device-resident jobs (``appt_server_*``): memory
+    admission by exact nnz(C) after the symbolic phase, several jobs in flight
+    on their own streams, small waiting jobs run together as one batch."""
+
+    def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 200 * 10 ** 9,
+                 batch_max_jobs: int = 64, batch_row_limit: int = 65536, admission_wait_ms: int = 10000,
+                 tuning: dict | None = None):
+        o = ServerOptions()
+        o.device = device
+        o.slots = slots
+        o.hbm_budget_bytes = hbm_budget_bytes
+        o.batch_max_jobs = batch_max_jobs
+        o.batch_row_limit = batch_row_limit
+        o.admission_wait_ms = admission_wait_ms
+        o.tuning = _tuning_struct(tuning)
+        h = C.c_void_p()
+        check(lib().appt_server_create(C.byref(o), C.byref(h)), "appt_server_create")
+        self.handle = h
+
+    def submit(self, A: DeviceMatrix, op: str = "AA") -> int:
+        j = C.c_uint64()
+        check(lib().appt_server_submit(self.handle, C.byref(A.view), OPS[op], C.byref(j)), "appt_server_submit")
+        return int(j.value)
+
+    @staticmethod
+    def _info(i: JobInfo) -> dict:
+        d = {f: getattr(i, f) for f, _ in JobInfo._fields_ if f != "C"}
+        d["state"] = JOB_STATES.get(i.state, str(i.state))
+        d["status_name"] = STATUS_NAMES.get(i.status, str(i.status))
+        d["nnz"] = int(i.C.nnz)
+        return d
+
+    def poll(self, job: int) -> dict:
+        i = JobInfo()
+        check(lib().appt_server_poll(self.handle, job, C.byref(i)), "appt_server_poll")
+        return self._info(i)
+
+    def wait(self, job: int) -> dict:
+        i = JobInfo()
+        check(lib().appt_server_wait(self.handle, job, C.byref(i)), "appt_server_wait")
+        return self._info(i)
+
+    def download(self, job: int) -> HostMatrix:
+        i = JobInfo()
+        check(lib().appt_server_wait(self.handle, job, C.byref(i)), "appt_server_wait")
+        check(i.status, f"job {job}")
+        return DeviceResult(None, i.C).download()
+
+    def release(self, job: int) -> None:
+        check(lib().appt_server_release(self.handle, job), "appt_server_release")
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        check(lib().appt_server_stats_get(self.handle, C.byref(s)), "appt_server_stats_get")
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().appt_server_destroy(self.handle), "appt_server_destroy")
             self.handle = None
 
     def __enter__(self):

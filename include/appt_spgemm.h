@@ -220,6 +220,64 @@ appt_status appt_batch_execute(appt_batch* b);
 appt_status appt_batch_result(const appt_batch* b, int32_t k, appt_csr_result* out);
 appt_status appt_batch_info_get(const appt_batch* b, appt_batch_info* out);
 
+/* ---- serving -------------------------------------------------------------------
+ * A request loop over device-resident jobs (C = A*A or A*A^T).  A job is
+ * admitted by memory: once its symbolic phase has fixed nnz(C), the exact
+ * result size plus a working-set estimate must fit the HBM budget beside what
+ * admitted jobs hold; otherwise it waits for memory to free up, or is refused
+ * (APPT_ERR_ADMISSION) when it could never fit.  Up to `slots` jobs run at
+ * once, each on its own engine and default-priority stream, so their kernels
+ * overlap on the device; small jobs waiting together are run as one
+ * block-diagonal batch.  No counterpart in the reference. */
+typedef struct appt_server appt_server;
+
+typedef struct appt_server_options {
+  int32_t device;
+  int32_t slots;              /* jobs (or job groups) in flight at once; 0 = 4 */
+  int64_t hbm_budget_bytes;   /* admission budget; 0 = 200 GB */
+  int32_t batch_max_jobs;     /* most small jobs run as one group; 0 = 64, 1 = no grouping */
+  int32_t batch_row_limit;    /* a job is small when n <= this; 0 = 65536 */
+  int32_t admission_wait_ms;  /* longest wait for memory before refusing; 0 = 10000 */
+  appt_tuning tuning;
+} appt_server_options;
+
+typedef enum appt_job_state {
+  APPT_JOB_QUEUED = 0,
+  APPT_JOB_RUNNING = 1,
+  APPT_JOB_DONE = 2,          /* finished; status says whether it succeeded */
+  APPT_JOB_RELEASED = 3
+} appt_job_state;
+
+typedef struct appt_job_info {
+  int32_t state;              /* appt_job_state */
+  int32_t status;             /* appt_status of a finished job */
+  int32_t group;              /* jobs it ran with (1 = alone) */
+  double queue_ms, run_ms, latency_ms;  /* submit->start, start->done, submit->done */
+  int64_t products;
+  int64_t reserved_bytes;     /* admission bytes held by the result */
+  appt_csr_result C;          /* device result of a successful job until appt_server_release */
+} appt_job_info;
+
+typedef struct appt_server_stats {
+  int64_t submitted, completed, refused, failed;
+  double p50_ms, p99_ms, mean_ms;  /* latency of completed jobs */
+  double gflops;                   /* 2 * sum(products) of completed jobs / (last done - first submit) */
+  double jobs_per_s;
+  int64_t bytes_reserved, peak_bytes_reserved;
+  int64_t groups_run;              /* launches that ran more than one job */
+} appt_server_stats;
+
+appt_status appt_server_create(const appt_server_options* opt, appt_server** out);
+/* Finishes running jobs, fails queued ones, frees every result. */
+appt_status appt_server_destroy(appt_server* s);
+/* Queue C = op(A) (op AA or AAT); A is read until the job finishes. */
+appt_status appt_server_submit(appt_server* s, const appt_csr_view* A, appt_op op, uint64_t* job);
+appt_status appt_server_poll(appt_server* s, uint64_t job, appt_job_info* out);
+appt_status appt_server_wait(appt_server* s, uint64_t job, appt_job_info* out);
+/* Free a finished job's result and its admission bytes. */
+appt_status appt_server_release(appt_server* s, uint64_t job);
+appt_status appt_server_stats_get(appt_server* s, appt_server_stats* out);
+
 /* ---- host CSR, CSR1 files and synthetic generators --------------------------- */
 /* Host CSR owned by the library (malloc'd); free with appt_host_csr_free. */
 typedef struct appt_host_csr {
@@ -258,7 +316,8 @@ appt_status appt_gen_erdos_renyi(int32_t n, double mean_deg, uint64_t seed, appt
 /* ---- host <-> device helpers (for callers without their own allocator) ------- */
 appt_status appt_device_upload(appt_engine* eng, const appt_host_csr* h, appt_csr_view* out_dev);
 appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v);
-/* Copy a device result to caller-provided host arrays (rowptr as int64). */
+/* Copy a device result to caller-provided host arrays (rowptr as int64);
+ * eng may be NULL (the calling thread's current device is used). */
 appt_status appt_csr_download(appt_engine* eng, const appt_csr_result* C, int64_t* rowptr,
                               int32_t* col, double* val);
 
