@@ -44,7 +44,7 @@ def serve_job(rng: np.random.Generator, nmin: int = 1000, nmax: int = 2_000_000,
         g = max(2, int(round(n ** (1.0 / 3.0))))
         return stencil27(g, 0.1 if rng.random() < 0.5 else 0.0, seed), op, "stencil27"
     scale = max(6, int(np.ceil(np.log2(n))))
-    return rmat(scale, 16, seed=seed), op, "rmat"
+    return truncate(rmat(scale, 16, seed=seed), int(n)), op, "rmat"
 
 
 def truncate(m: HostMatrix, n: int) -> HostMatrix:
