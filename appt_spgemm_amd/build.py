@@ -18,7 +18,8 @@ CSRC = PKG / "csrc"
 OUT = PKG / "_build"
 LIB = PKG / "libappt_spgemm.so"
 BIN = PKG / "bin"
-CLI_TOOLS = ["appt_spgemm", "appt_dispatch"]
+CLI_TOOLS = ["appt_spgemm", "appt_dispatch", "appt_rocsparse"]
+CLI_LIBS = {"appt_rocsparse": ["-lrocsparse"]}  # vendor-library context timing only
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 ARCH = "gfx950"
 
@@ -81,7 +82,7 @@ def build(verbose: bool = False) -> Path:
         exe = BIN / tool
         cli_jobs.append(["g++", *CXX_FLAGS, str(CSRC / "cli" / f"{tool}.cpp"), "-o", str(exe) + ".tmp",
                          f"-L{PKG}", "-lappt_spgemm", f"-Wl,-rpath,$ORIGIN/..", f"-L{ROCM / 'lib'}",
-                         "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
+                         *CLI_LIBS.get(tool, []), "-lamdhip64", f"-Wl,-rpath,{ROCM / 'lib'}"])
     with cf.ThreadPoolExecutor(len(cli_jobs)) as ex:
         for f in [ex.submit(_run, j) for j in cli_jobs]:
             f.result()

@@ -43,6 +43,46 @@ def make_workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+GEN_SPEC = {"stencil27_96": "gen:stencil27:96:0:1", "stencil27_96_drop": "gen:stencil27:96:0.1:1",
+            "laplace2d_64": "gen:laplace2d:64"}
+
+
+def rocsparse_context(workload: str, op: str, reps: int, timeout_s: float = 240.0):
+    """rocSPARSE generic SpGEMM on the same matrix, timed as the reference
+    times its vendor path (per call: transpose if any, all stages and
+    allocations); context only, never the measured value."""
+    import subprocess
+    spec = GEN_SPEC.get(workload)
+    if spec is None and workload.startswith("rmat"):
+        spec = f"gen:rmat:{int(workload[4:] or 16)}:16:1"
+    exe = os.path.join(ROOT, "appt_spgemm_amd", "bin", "appt_rocsparse")
+    if spec is None or not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe, "ctx", spec, op, str(reps)], capture_output=True, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        return {"error": f"timed out after {timeout_s:.0f} s"}
+    line = [l for l in r.stdout.splitlines() if l.startswith("RESULT,")]
+    if r.returncode != 0 or not line:
+        return {"error": (r.stderr.strip().splitlines() or ["failed"])[-1][:200]}
+    f = line[-1].split(",")
+    return {"ms": float(f[6]), "nnz_c": int(f[5]), "reps": reps,
+            "scope": "csr2csc (A*A^T) + buffer size + nnz + compute + allocations, per call"}
+
+
+def pmc_traffic(workload: str, op: str):
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary for this workload (each
+    counter scaled by its reading for a device copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{op}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return int(round(1024 * (t["fetch_kib_per_dispatch"] * t["fetch_scale"] +
+                             t["write_kib_per_dispatch"] * t["write_scale"])))
+
+
 def bytes_csr(n: int, nnz: int, rowptr_bytes: int = 4) -> int:
     return rowptr_bytes * (n + 1) + 12 * nnz
 
@@ -250,11 +290,25 @@ def main() -> int:
     ap_.add_argument("--serve-pool", type=int, default=24)
     ap_.add_argument("--serve-jobs", type=int, default=200)
     ap_.add_argument("--serve-nmax", type=float, default=2e6)
+    ap_.add_argument("--copy-probe", type=int, default=0, help="MiB; run only a device copy (counter calibration)")
+    ap_.add_argument("--rocsparse", type=int, default=1, help="also time rocSPARSE on the same matrix (context)")
     args = ap_.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
               "this engine has no multi-GPU path", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        # calibration for rocprofv3 FETCH_SIZE / WRITE_SIZE: one device copy of
+        # a known size (reads and writes copy_probe MiB each)
+        import torch
+        n = args.copy_probe * (1 << 20) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_MiB": args.copy_probe, "dispatches": 3}), flush=True)
+        return 0
     if args.workload in ("batch4096", "serve"):
         res = bench_batch(args) if args.workload == "batch4096" else bench_serve(args)
         out = {"metric": res["metric"], "value": res["value"], "unit": res["unit"], "n_gpus": 1,
@@ -299,6 +353,10 @@ def main() -> int:
     call_gbs = b_alg / (ms * 1e-3) / 1e9
     A.free()
     eng.close()
+    roc = rocsparse_context(args.workload, args.op, 5) if args.rocsparse else None
+    if roc and "ms" in roc:
+        roc["GFLOPs"] = round(2.0 * products / (roc["ms"] * 1e-3) / 1e9, 3)
+        roc["nnz_c_matches"] = roc["nnz_c"] == nnz_c
     cpu = None if args.no_cpu else cpu_baseline(m, args.op, args.cpu_seconds)
     out = {
         "metric": "SpGEMM GFLOP/s (FP64, structural C=A*A) with HBM GB/s vs roofline",
@@ -316,10 +374,10 @@ def main() -> int:
         "config": {"workload": desc, "op": args.op, "n": m.n, "nnz_a": m.nnz, "nnz_c": nnz_c,
                    "products": products, "flops": 2 * products, "bytes_alg": b_alg,
                    "call_GBps": round(call_gbs, 2), "call_roofline_frac": round(call_gbs / HBM_PEAK_GBS, 4),
-                   "phase_ms": phases},
+                   "phase_ms": phases, "rocsparse": roc},
         "roofline": {"bound": "hbm", "kernel": dom.replace("_ms", ""), "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": pmc_traffic(args.workload, args.op) if dom == "numeric_ms" else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
