@@ -2,8 +2,8 @@
 for int32/fp64 CSR, behind the C ABI in include/appt_spgemm.h."""
 from ._abi import ApptError, lib  # noqa: F401
 from .engine import (Batch, DeviceMatrix, DeviceResult, Engine, HostMatrix, Server, banded,  # noqa: F401
-                     csr1_read, csr1_write, erdos_renyi, laplace2d, pack, rmat, run_ours, stencil27)
+                     csr1_read, csr1_write, erdos_renyi, laplace2d, pack, panels, rmat, run_ours, stencil27)
 
 __all__ = ["ApptError", "Batch", "DeviceMatrix", "DeviceResult", "Engine", "HostMatrix", "Server", "banded",
-           "csr1_read", "csr1_write", "erdos_renyi", "laplace2d", "lib", "pack", "rmat", "run_ours",
+           "csr1_read", "csr1_write", "erdos_renyi", "laplace2d", "lib", "pack", "panels", "rmat", "run_ours",
            "stencil27"]

@@ -118,6 +118,9 @@ _SIGNATURES = {
     "appt_engine_device_bytes": (C.c_int64, [C.c_void_p]),
     "appt_spgemm_compute": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, P(CsrResult)]),
     "appt_csr_release": (C.c_int, [C.c_void_p, P(CsrResult)]),
+    "appt_spgemm_compute_rows": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, C.c_int32, C.c_int32,
+                                           P(CsrResult)]),
+    "appt_spgemm_symbolic_rowptr": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, P(C.c_int64)]),
     "appt_spgemm_run_timed": (C.c_int, [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.c_int32, C.c_int64, C.c_int32, P(EngResult)]),
     "appt_engine_run_timed": (C.c_int, [C.c_void_p, C.c_char_p, P(CsrView), C.c_int32, P(EngResult)]),
@@ -151,6 +154,8 @@ _SIGNATURES = {
     "appt_server_stats_get": (C.c_int, [C.c_void_p, P(ServerStats)]),
     "appt_device_upload": (C.c_int, [C.c_void_p, P(HostCsr), P(CsrView)]),
     "appt_device_free_view": (C.c_int, [C.c_void_p, P(CsrView)]),
+    "appt_csr_download_rows": (C.c_int, [C.c_void_p, P(CsrResult), C.c_int32, C.c_int32, P(C.c_int64),
+                                          P(C.c_int32), P(C.c_double)]),
     "appt_csr_download": (C.c_int, [C.c_void_p, P(CsrResult), P(C.c_int64), P(C.c_int32),
                                      P(C.c_double)]),
 }

@@ -146,6 +146,7 @@ void Batch::operands() {
   hipStream_t s = E_.stream();
   O_ = Operands{};
   O_.n = N_;
+  O_.nb = N_;
   O_.annz = nnz_a_;
   O_.ap = in_rp_;
   O_.ac = acg_;
@@ -154,12 +155,15 @@ void Batch::operands() {
   O_.bv = in_val_;
   O_.bnnz = nnz_a_ + nnz_t_;
   O_.upper = 0;
+  (void)E_.reset_ctl(W_);
   launch_fill_i32(tcnt_, (int64_t)N_ + 1, 0, s);
   launch_batch_acg(N_, in_rp_, in_col_, rowbase_, rowop_, acg_, tcnt_, s);
   if (nnz_t_ > 0) {
     launch_scan(tcnt_, N_, toff_, tp32_, tscan_, s);
     launch_i64_to_i32(toff_, N_, tcur_, s);
     launch_batch_tscatter(N_, in_rp_, acg_, rowbase_, rowop_, tcur_, nnz_a_, in_col_, in_val_, s);
+    int32_t* slist = static_cast<int32_t*>(E_.ensure(W_.sortlist, sizeof(int32_t) * (size_t)sort_rows_scratch_elems(N_)));
+    launch_sort_rows(N_, tp32_, in_col_ + nnz_a_, in_val_ + nnz_a_, slist, nullptr, s);
   } else {
     launch_fill_i32(tp32_, (int64_t)N_ + 1, 0, s);
   }

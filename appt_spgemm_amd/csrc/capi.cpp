@@ -104,6 +104,17 @@ appt_status appt_spgemm_compute(appt_engine* eng, const appt_csr_view* A, appt_o
   return guard([&] { eng->impl->compute(*A, op, C); });
 }
 
+appt_status appt_spgemm_compute_rows(appt_engine* eng, const appt_csr_view* A, appt_op op, int32_t row_begin,
+                                     int32_t row_end, appt_csr_result* C) {
+  if (!eng || !A || !C) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->compute_rows(*A, op, row_begin, row_end, C); });
+}
+
+appt_status appt_spgemm_symbolic_rowptr(appt_engine* eng, const appt_csr_view* A, appt_op op, int64_t* rowptr) {
+  if (!eng || !A || !rowptr) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->symbolic_rowptr(*A, op, rowptr); });
+}
+
 appt_status appt_csr_release(appt_engine* eng, appt_csr_result* C) {
   if (!eng || !C) return APPT_ERR_INVALID;
   return guard([&] { eng->impl->release(C); });
@@ -230,6 +241,27 @@ appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v) {
     (void)hipFree((void*)v->col);
     (void)hipFree((void*)v->val);
     std::memset(v, 0, sizeof(*v));
+  });
+}
+
+appt_status appt_csr_download_rows(appt_engine* eng, const appt_csr_result* C, int32_t row_begin, int32_t row_end,
+                                   int64_t* rowptr, int32_t* col, double* val) {
+  if (!C || !rowptr) return APPT_ERR_INVALID;
+  if (row_begin < 0 || row_end < row_begin || row_end > C->n) return APPT_ERR_INVALID;
+  return guard([&] {
+    if (eng) APPT_HIP(hipSetDevice(eng->impl->device()));
+    const size_t n1 = (size_t)(row_end - row_begin) + 1;
+    if (C->rowptr_is_64) {
+      APPT_HIP(hipMemcpy(rowptr, (const int64_t*)C->rowptr + row_begin, sizeof(int64_t) * n1, hipMemcpyDeviceToHost));
+    } else {
+      std::unique_ptr<int32_t[]> t(new int32_t[n1]);
+      APPT_HIP(hipMemcpy(t.get(), (const int32_t*)C->rowptr + row_begin, sizeof(int32_t) * n1, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n1; i++) rowptr[i] = t[i];
+    }
+    const int64_t base = rowptr[0], cnt = rowptr[n1 - 1] - rowptr[0];
+    for (size_t i = 0; i < n1; i++) rowptr[i] -= base;
+    if (col && cnt) APPT_HIP(hipMemcpy(col, C->col + base, sizeof(int32_t) * (size_t)cnt, hipMemcpyDeviceToHost));
+    if (val && cnt) APPT_HIP(hipMemcpy(val, C->val + base, sizeof(double) * (size_t)cnt, hipMemcpyDeviceToHost));
   });
 }
 

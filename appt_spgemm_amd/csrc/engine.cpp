@@ -222,6 +222,7 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
   hipStream_t s = stream_;
   O = Operands{};
   O.n = n;
+  O.nb = n;
   O.annz = nnz;
   O.ap = A.rowptr;
   O.ac = A.col;
@@ -231,19 +232,27 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
   O.bnnz = nnz;
   O.upper = op == APPT_OP_AATS ? 1 : 0;
   const int32_t* bptr = A.rowptr;
+  int32_t* ctl = reset_ctl(W);
   if (op != APPT_OP_AA) {
-    // B = A^T: column histogram, scan, atomic-cursor scatter
+    // B = A^T with sorted rows (heavy rows then carry per-entry cursors
+    // across windows): column histogram and scan; a symmetric pattern is
+    // gathered in place of A's layout, any other takes the atomic-cursor
+    // scatter and a per-row sort (both skipped when the gather succeeded)
     int32_t* tp = as<int32_t>(ensure(W.bptr, sizeof(int32_t) * ((size_t)n + 1)));
     int32_t* tc = as<int32_t>(ensure(W.bcol, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
     double* tv = as<double>(ensure(W.bval, sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
     int32_t* cnt = as<int32_t>(ensure(W.cursor, sizeof(int32_t) * ((size_t)n + 1)));
     int64_t* off64 = as<int64_t>(ensure(W.rowoff, sizeof(int64_t) * ((size_t)n + 1)));
     int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
+    int32_t* slist = as<int32_t>(ensure(W.sortlist, sizeof(int32_t) * (size_t)sort_rows_scratch_elems(n)));
+    int32_t* asym = ctl + 3;
     launch_fill_i32(cnt, n, 0, s);
     launch_colcount(nnz, A.col, cnt, s);
     launch_scan(cnt, n, off64, tp, tmp, s);
+    launch_transpose_gather(n, A.rowptr, A.col, A.val, tc, tv, asym, s);
     launch_i64_to_i32(off64, n, cnt, s);
-    launch_transpose_scatter(n, A.rowptr, A.col, A.val, cnt, tc, tv, s);
+    launch_transpose_scatter(n, A.rowptr, A.col, A.val, cnt, tc, tv, asym, s);
+    launch_sort_rows(n, tp, tc, tv, slist, asym, s);
     O.bc = tc;
     O.bv = tv;
     bptr = tp;
@@ -251,14 +260,17 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
   stage_bmeta(W, O, bptr, nullptr, nullptr, 0);
 }
 
+int32_t* Engine::reset_ctl(Workspace& W) {
+  int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * 4));
+  launch_fill_i32(ctl, 4, 0, stream_);
+  return ctl;
+}
+
 void Engine::stage_bmeta(Workspace& W, const Operands& O, const int32_t* ptr, const int32_t* ptr2,
                          const uint8_t* sel, int64_t off2) {
-  hipStream_t s = stream_;
-  int4* bmeta = as<int4>(ensure(W.bmeta, sizeof(int4) * (size_t)std::max(O.n, 1)));
-  // control words: [0] / [1] heavy-row claim counters, [2] B-unsorted flag
-  int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * 4));
-  launch_fill_i32(ctl, 4, 0, s);
-  launch_rowstats(O.n, ptr, ptr2, sel, off2, O.bc, bmeta, ctl + 2, s);
+  int4* bmeta = as<int4>(ensure(W.bmeta, sizeof(int4) * (size_t)std::max(O.nb, 1)));
+  int32_t* ctl = as<int32_t>(W.ctl.p);
+  launch_rowstats(O.nb, ptr, ptr2, sel, off2, O.bc, bmeta, ctl + 2, stream_);
 }
 
 void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
@@ -405,28 +417,63 @@ void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, c
   }
 }
 
-void Engine::stage_mirror(Workspace& W, const Operands& O, const int32_t* ccol, double* cval) {
+void Engine::stage_mirror(Workspace& W, const Operands& O, const Plan& P, const int32_t* ccol, double* cval) {
   // AAtS: lower entries take the value of their upper mirror image
-  if (O.upper) launch_mirror_pull(O.n, as<int64_t>(W.rowoff.p), ccol, cval, stream_);
+  if (O.upper) launch_mirror_pull(O.n, P.nnz, as<int64_t>(W.rowoff.p), ccol, cval, stream_);
 }
 
 // ---------------------------------------------------------------------------
 // one call
 // ---------------------------------------------------------------------------
 void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
+  compute_rows(A, op, 0, A.n, C);
+}
+
+void Engine::symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr_host) {
+  validate(A);
+  if (!rowptr_host) throw Error(APPT_ERR_INVALID, "rowptr pointer is NULL");
+  if (op != APPT_OP_AA && op != APPT_OP_AAT && op != APPT_OP_AATS) throw Error(APPT_ERR_INVALID, "bad op");
+  APPT_HIP(hipSetDevice(device_));
+  Workspace& W = ws_;
+  Operands O;
+  Plan P;
+  stage_operands(W, A, op, O);
+  stage_prepass(W, O, cfg_);
+  read_sym(W, P);
+  stage_symbolic(W, O, cfg_, P);
+  int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)A.n + 1)));
+  try {
+    stage_offsets(W, O, cfg_, rp32);
+    APPT_HIP(hipMemcpyAsync(rowptr_host, W.rowoff.p, sizeof(int64_t) * ((size_t)A.n + 1), hipMemcpyDeviceToHost,
+                            stream_));
+    APPT_HIP(hipStreamSynchronize(stream_));
+  } catch (...) {
+    pool_.put(rp32);
+    throw;
+  }
+  pool_.put(rp32);
+}
+
+void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_t r1, appt_csr_result* C) {
   validate(A);
   if (!C) throw Error(APPT_ERR_INVALID, "result pointer is NULL");
   if (op != APPT_OP_AA && op != APPT_OP_AAT && op != APPT_OP_AATS) throw Error(APPT_ERR_INVALID, "bad op");
+  if (r0 < 0 || r1 < r0 || r1 > A.n) throw Error(APPT_ERR_INVALID, "row range outside [0, n]");
+  if (op == APPT_OP_AATS && (r0 != 0 || r1 != A.n))
+    throw Error(APPT_ERR_UNSUPPORTED, "AAtS mirrors across all rows; row panels take AA or AAt");
   APPT_HIP(hipSetDevice(device_));
   std::memset(C, 0, sizeof(*C));
   times_ = appt_phase_times{};
-  const int32_t n = A.n;
+  const int32_t n = r1 - r0;
   hipStream_t s = stream_;
   Workspace& W = ws_;
   Operands O;
   Plan P;
   record(0);
   stage_operands(W, A, op, O);
+  // a panel: output rows r0..r1-1 of A against all of B
+  O.ap = A.rowptr + r0;
+  O.n = n;
   record(1);
   stage_prepass(W, O, cfg_);
   read_sym(W, P);
@@ -446,7 +493,7 @@ void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
     cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(P.nnz, 1)));
     stage_numeric(W, O, cfg_, P, ccol, cval);
     record(5);
-    stage_mirror(W, O, ccol, cval);
+    stage_mirror(W, O, P, ccol, cval);
   } catch (...) {
     pool_.put(rp32);
     pool_.put(ccol);

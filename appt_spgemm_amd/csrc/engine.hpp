@@ -68,12 +68,12 @@ struct Config {
 // captured their addresses stays valid while the owner lives.
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
-      numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur;
+      numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist;
   template <typename F>
   void each(F&& f) {
     Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
-                      &cur};
+                      &cur,    &sortlist};
     for (Scratch* s : all) f(*s);
   }
 };
@@ -82,8 +82,9 @@ struct Workspace {
 // [ap[i], ap[i+1]) with B-row id ac[p] and value av[p].  B side: row k is
 // bc/bv[bmeta[k].x, bmeta[k].y), bmeta built by the operand stage.
 struct Operands {
-  int32_t n = 0;  // output rows (= B rows)
-  int64_t annz = 0;
+  int32_t n = 0;   // output rows (A-side rows; a panel of A's rows)
+  int32_t nb = 0;  // B rows
+  int64_t annz = 0;  // entries addressed by A-side positions (cursor array size)
   const int32_t* ap = nullptr;
   const int32_t* ac = nullptr;
   const double* av = nullptr;
@@ -122,6 +123,10 @@ class Engine {
   ~Engine();
 
   void compute(const appt_csr_view& A, appt_op op, appt_csr_result* C);
+  // Rows [r0, r1) of op(A) (a row panel): C has r1 - r0 rows, global columns.
+  void compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_t r1, appt_csr_result* C);
+  // Exact row pointers of op(A) (symbolic phase only), copied to the host.
+  void symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr_host);
   void release(appt_csr_result* C);
   void run_timed(const char* task, const appt_csr_view& A, int reps, appt_eng_result* out);
   void descriptors(const appt_csr_view& A, appt_op op, appt_descriptors* out);
@@ -149,6 +154,9 @@ class Engine {
   // of the same structure can be captured into a HIP graph.
   // B = A or A^T for one matrix, and the B-row metadata.
   void stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Operands& O);
+  // Control words [0] / [1] heavy-row claim counters, [2] B-unsorted flag,
+  // [3] asymmetric-pattern flag of the transpose; zeroed here.
+  int32_t* reset_ctl(Workspace& W);
   // B-row metadata from B's row starts/ends: row k is [ptr[k], ptr[k+1]) or,
   // where sel[k] != 0, [off2 + ptr2[k], off2 + ptr2[k+1]).
   void stage_bmeta(Workspace& W, const Operands& O, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel,
@@ -158,7 +166,7 @@ class Engine {
   void stage_offsets(Workspace& W, const Operands& O, const Config& cfg, int32_t* rp32);
   void stage_numeric(Workspace& W, const Operands& O, const Config& cfg, const Plan& P, int32_t* ccol,
                      double* cval);
-  void stage_mirror(Workspace& W, const Operands& O, const int32_t* ccol, double* cval);
+  void stage_mirror(Workspace& W, const Operands& O, const Plan& P, const int32_t* ccol, double* cval);
   // host synchronisations: bin statistics of the symbolic / numeric binning
   void read_sym(Workspace& W, Plan& P);
   void read_num(Workspace& W, const Operands& O, Plan& P);

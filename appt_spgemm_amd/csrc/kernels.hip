@@ -116,9 +116,54 @@ void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t 
   hipLaunchKernelGGL(k_colcount, dim3((unsigned)blocks), dim3(256), 0, s, nnz, col, cnt);
 }
 
+// Symmetric-pattern transpose: when every row is strictly sorted and each
+// stored (k, i) has a stored (i, k), B = A^T has A's pattern, so B row k is
+// A row k with values a_ik found by binary search in row i.  Any miss,
+// duplicate or descent sets *asym and the scatter path below runs instead.
+__global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
+                                   const double* __restrict__ av, int32_t* __restrict__ bc, double* __restrict__ bv,
+                                   int32_t* __restrict__ asym) {
+  constexpr int G = 16;
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  for (int64_t k = (int64_t)blockIdx.x * gpb + threadIdx.x / G; k < n; k += (int64_t)gridDim.x * gpb) {
+    if (__hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const int s = ap[k], e = ap[k + 1];
+    bool miss = false;
+    for (int p = s + lane; p < e; p += G) {
+      const int i = ac[p];
+      if (p > s && ac[p - 1] >= i) miss = true;   // descent or duplicate
+      int lo = ap[i], hi = ap[i + 1];
+      const int end = hi;
+      while (lo < hi) {
+        const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+        if (ac[mid] < (int)k) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < end && ac[lo] == (int)k) {
+        bc[p] = i;
+        bv[p] = av[lo];
+      } else {
+        miss = true;
+      }
+    }
+    if (miss) *asym = 1;   // every writer stores 1
+  }
+}
+void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, int32_t* bc,
+                             double* bv, int32_t* asym, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_transpose_gather, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, av, bc, bv, asym);
+}
+
+// gate (optional): the scatter and the row sort run only when *gate != 0
 __global__ void k_transpose_scatter(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
                                     const double* __restrict__ av, int32_t* __restrict__ cursor,
-                                    int32_t* __restrict__ bc, double* __restrict__ bv) {
+                                    int32_t* __restrict__ bc, double* __restrict__ bv,
+                                    const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
@@ -133,12 +178,12 @@ __global__ void k_transpose_scatter(int32_t n, const int32_t* __restrict__ ap, c
   }
 }
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
-                              int32_t* cursor, int32_t* bc, double* bv, hipStream_t s) {
+                              int32_t* cursor, int32_t* bc, double* bv, const int32_t* gate, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_transpose_scatter, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, av, cursor,
-                     bc, bv);
+                     bc, bv, gate);
 }
 
 // ---------------------------------------------------------------------------
@@ -813,6 +858,118 @@ __device__ __forceinline__ void lds_sort_pairs(int32_t* keys, V* vals, int len, 
       gsync<G>();
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Sorted transpose: the atomic-cursor scatter leaves each B row in arbitrary
+// order; every row is then sorted by its column (the source row index), the
+// values moving with their keys.  Rows of at most 64 entries: one wave, in
+// registers; up to kSortLds entries: one block in LDS; longer rows: one
+// block, in place in global memory.
+// ---------------------------------------------------------------------------
+constexpr int kSortLds = 4096;
+
+__global__ void k_sort_rows_wave(int32_t n, const int32_t* __restrict__ bp, int32_t* __restrict__ bc,
+                                 double* __restrict__ bv, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int lane = threadIdx.x % kWave;
+  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n; r += nw) {
+    const int s = bp[r], len = bp[r + 1] - s;
+    if (len < 2 || len > kWave) continue;  // uniform across the wave
+    int key = lane < len ? bc[s + lane] : INT32_MAX;
+    double v = lane < len ? bv[s + lane] : 0.0;
+#pragma unroll
+    for (int k = 2; k <= kWave; k <<= 1) {
+#pragma unroll
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int pk = __shfl_xor(key, j, kWave);
+        const double pv = __shfl_xor(v, j, kWave);
+        const bool up = (lane & k) == 0, lower = (lane & j) == 0;
+        // lower lane of an ascending pair keeps the minimum, and so on
+        if (lower == up ? pk < key : pk > key) {
+          key = pk;
+          v = pv;
+        }
+      }
+    }
+    if (lane < len) {
+      bc[s + lane] = key;
+      bv[s + lane] = v;
+    }
+  }
+}
+
+// Rows longer than a wave, listed for balanced sorting: lengths in
+// (kWave, kSortLds] from the front of `list`, longer ones from the back;
+// count[0] / count[1] (zero beforehand) receive the two list lengths.
+__global__ void k_select_rows(int32_t n, const int32_t* __restrict__ bp, int32_t* __restrict__ list,
+                              int32_t* __restrict__ count, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int len = bp[r + 1] - bp[r];
+    if (len > kSortLds) list[n - 1 - atomicAdd(&count[1], 1)] = (int32_t)r;
+    else if (len > kWave) list[atomicAdd(&count[0], 1)] = (int32_t)r;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sort_rows_lds(int32_t n, const int32_t* __restrict__ bp,
+                                                       int32_t* __restrict__ bc, double* __restrict__ bv,
+                                                       const int32_t* __restrict__ list,
+                                                       const int32_t* __restrict__ count,
+                                                       const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  constexpr int T = 256;
+  __shared__ int32_t keys[kSortLds];
+  __shared__ double vals[kSortLds];
+  const int m = count[0];
+  for (int q = blockIdx.x; q < m; q += gridDim.x) {
+    const int r = list[q];
+    const int s = bp[r], len = bp[r + 1] - s;
+    for (int t = threadIdx.x; t < len; t += T) {
+      keys[t] = bc[s + t];
+      vals[t] = bv[s + t];
+    }
+    __syncthreads();
+    lds_sort_pairs<T, double>(keys, vals, len, threadIdx.x);
+    for (int t = threadIdx.x; t < len; t += T) {
+      bc[s + t] = keys[t];
+      bv[s + t] = vals[t];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_sort_rows_long(int32_t n, const int32_t* __restrict__ bp,
+                                                         int32_t* __restrict__ bc, double* __restrict__ bv,
+                                                         const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ count,
+                                                         const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int m = count[1];
+  for (int q = blockIdx.x; q < m; q += gridDim.x) {
+    const int r = list[n - 1 - q];
+    const int s = bp[r], len = bp[r + 1] - s;
+    lds_sort_pairs<1024, double>(bc + s, bv + s, len, threadIdx.x);
+    __syncthreads();
+  }
+}
+
+int64_t sort_rows_scratch_elems(int32_t n) { return (int64_t)n + 2; }
+
+void launch_sort_rows(int32_t n, const int32_t* bp, int32_t* bc, double* bv, int32_t* scratch, const int32_t* gate,
+                      hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n * kWave + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_sort_rows_wave, dim3((unsigned)blocks), dim3(256), 0, s, n, bp, bc, bv, gate);
+  int32_t* count = scratch + n;
+  launch_fill_i32(count, 2, 0, s);
+  int64_t b1 = ((int64_t)n + 255) / 256;
+  if (b1 > 4096) b1 = 4096;
+  hipLaunchKernelGGL(k_select_rows, dim3((unsigned)b1), dim3(256), 0, s, n, bp, scratch, count, gate);
+  hipLaunchKernelGGL(k_sort_rows_lds, dim3(2048), dim3(256), 0, s, n, bp, bc, bv, scratch, count, gate);
+  hipLaunchKernelGGL(k_sort_rows_long, dim3(256), dim3(1024), 0, s, n, bp, bc, bv, scratch, count, gate);
 }
 
 // Group-wide exclusive scan (G <= 64 shuffles, else block scan).
@@ -1744,36 +1901,90 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
 // and sorted; the numeric pass formed values only for j >= i.  Each lower
 // entry (c, r), r < c, takes the value of (r, c) from row r's upper part,
 // found by binary search (columns are ascending).  No atomics, no sort.
+// Short rows are dealt by rows; long rows (power-law hubs, ~10^4-10^5 lower
+// entries) by entries.
 // ---------------------------------------------------------------------------
-__global__ void k_mirror_pull(int32_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ ccol,
+constexpr int kMirrorChunk = 4096;  // C entries per block of the long-row kernel
+constexpr int kMirrorLong = 1024;   // rows longer than this go to the entry-parallel kernel
+
+// Value of (r, c) from row r (columns ascending), or 0 when absent.
+__device__ __forceinline__ double mirror_value(const int64_t* off, const int32_t* ccol, const double* cval, int r,
+                                               int c) {
+  int64_t lo = off[r], hi = off[r + 1];
+  const int64_t end = hi;
+  while (lo < hi) {  // first position of row r with column >= c
+    const int64_t mid = (lo + hi) >> 1;
+    if (ccol[mid] < c) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && ccol[lo] == c) ? cval[lo] : 0.0;
+}
+
+// Rows of at most kMirrorLong entries: 16 lanes per row over its lower part.
+__global__ void k_mirror_rows(int32_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ ccol,
                               double* __restrict__ cval) {
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
   for (int64_t c = (int64_t)blockIdx.x * gpb + threadIdx.x / G; c < n; c += (int64_t)gridDim.x * gpb) {
     const int64_t s = off[c], e = off[c + 1];
+    if (e - s > kMirrorLong) continue;
     for (int64_t b = s; b < e; b += G) {
-      if (ccol[b] >= c) break;                 // lower part is a prefix of the row
+      if (ccol[b] >= c) break;  // the lower part is a prefix of the row
       const int64_t p = b + lane;
       if (p < e) {
         const int r = ccol[p];
-        if (r < c) {
-          int64_t lo = off[r], hi = off[r + 1];
-          while (lo < hi) {                    // first position with column >= c
-            const int64_t mid = (lo + hi) >> 1;
-            if (ccol[mid] < c) lo = mid + 1; else hi = mid;
-          }
-          cval[p] = (lo < off[r + 1] && ccol[lo] == c) ? cval[lo] : 0.0;
-        }
+        if (r < c) cval[p] = mirror_value(off, ccol, cval, r, (int)c);
       }
     }
   }
 }
-void launch_mirror_pull(int32_t n, const int64_t* off, const int32_t* ccol, double* cval, hipStream_t s) {
-  if (n <= 0) return;
+
+// Longer rows, entry-parallel: a block owns kMirrorChunk consecutive entries
+// of C, so one long row is spread over many blocks.
+__global__ void __launch_bounds__(256) k_mirror_long(int32_t n, int64_t nnz, const int64_t* __restrict__ off,
+                                                     const int32_t* __restrict__ ccol, double* __restrict__ cval) {
+  __shared__ int32_t rows[2];
+  const int64_t e0 = (int64_t)blockIdx.x * kMirrorChunk;
+  const int64_t e1 = (e0 + kMirrorChunk) < nnz ? (e0 + kMirrorChunk) : nnz;
+  if (threadIdx.x < 2) {
+    // last row whose first entry is at or before e0 (thread 0) / e1 - 1 (thread 1)
+    const int64_t e = threadIdx.x == 0 ? e0 : e1 - 1;
+    int l = 0, h = n - 1;
+    while (l < h) {
+      const int mid = (int)(((int64_t)l + h + 1) >> 1);
+      if (off[mid] <= e) l = mid;
+      else h = mid - 1;
+    }
+    rows[threadIdx.x] = l;
+  }
+  __syncthreads();
+  const int r0 = rows[0], r1 = rows[1];
+  // skip the block when none of its rows is long (checked in one pass
+  // when it spans few rows)
+  if (r1 - r0 < (int)blockDim.x) {
+    const int c = r0 + (int)threadIdx.x;
+    const bool lng = c <= r1 && off[c + 1] - off[c] > kMirrorLong;
+    if (!__syncthreads_or(lng)) return;
+  }
+  int c = r0;
+  for (int64_t p = e0 + threadIdx.x; p < e1; p += blockDim.x) {
+    while (off[c + 1] <= p) c++;  // the row holding entry p
+    if (off[c + 1] - off[c] <= kMirrorLong) continue;
+    const int r = ccol[p];
+    if (r >= c) continue;         // upper part: formed by the numeric pass
+    cval[p] = mirror_value(off, ccol, cval, r, c);
+  }
+}
+
+void launch_mirror_pull(int32_t n, int64_t nnz, const int64_t* off, const int32_t* ccol, double* cval,
+                        hipStream_t s) {
+  if (n <= 0 || nnz <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_mirror_pull, dim3((unsigned)blocks), dim3(256), 0, s, n, off, ccol, cval);
+  hipLaunchKernelGGL(k_mirror_rows, dim3((unsigned)blocks), dim3(256), 0, s, n, off, ccol, cval);
+  const int64_t b2 = (nnz + kMirrorChunk - 1) / kMirrorChunk;
+  hipLaunchKernelGGL(k_mirror_long, dim3((unsigned)b2), dim3(256), 0, s, n, nnz, off, ccol, cval);
 }
 
 }  // namespace appt

@@ -56,8 +56,18 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
                      const int32_t* col, int4* meta, int32_t* flag, hipStream_t s);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
+// Symmetric-pattern transpose into (bc, bv) laid out like A; sets *asym
+// (zero beforehand) when A is not strictly sorted with a symmetric pattern.
+void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, int32_t* bc,
+                             double* bv, int32_t* asym, hipStream_t s);
+// Atomic-cursor scatter; gate (optional): runs only when *gate != 0.
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
-                              int32_t* cursor, int32_t* bc, double* bv, hipStream_t s);
+                              int32_t* cursor, int32_t* bc, double* bv, const int32_t* gate, hipStream_t s);
+// Sort every row of (bp, bc, bv) by column, values moving with their keys;
+// scratch: sort_rows_scratch_elems(n) int32; gate as above.
+int64_t sort_rows_scratch_elems(int32_t n);
+void launch_sort_rows(int32_t n, const int32_t* bp, int32_t* bc, double* bv, int32_t* scratch, const int32_t* gate,
+                      hipStream_t s);
 // Exclusive scan of n int32 counts into out64[0..n] (out64[n] = total); out32
 // (optional) receives the same values truncated to int32.  `tmp` needs
 // scan_tmp_elems(n) int64.
@@ -117,7 +127,8 @@ int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max);
 
 // AAtS: fill the lower entries of the complete, sorted C from row r's upper
 // part (C's structure is symmetric; values were formed for j >= i only).
-void launch_mirror_pull(int32_t n, const int64_t* off, const int32_t* ccol, double* cval, hipStream_t s);
+void launch_mirror_pull(int32_t n, int64_t nnz, const int64_t* off, const int32_t* ccol, double* cval,
+                        hipStream_t s);
 
 // ---- batched path (block-diagonal packing of many matrices) ----
 struct BatchSrc {  // one separate device matrix to pack

@@ -154,6 +154,21 @@ class DeviceResult:
                                       val.ctypes.data_as(C.POINTER(C.c_double))), "csr_download")
         return HostMatrix(n, rp, col[:nnz], val[:nnz])
 
+    def download_rows(self, r0: int, r1: int) -> HostMatrix:
+        """Rows [r0, r1) as a host matrix (r1 - r0 rows, global columns)."""
+        L = lib()
+        h = self._engine.handle if self._engine else None
+        rp = np.empty(r1 - r0 + 1, dtype=np.int64)
+        check(L.appt_csr_download_rows(h, C.byref(self.res), r0, r1, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       None, None), "csr_download_rows")
+        cnt = int(rp[-1])
+        col = np.empty(max(cnt, 1), dtype=np.int32)
+        val = np.empty(max(cnt, 1), dtype=np.float64)
+        check(L.appt_csr_download_rows(h, C.byref(self.res), r0, r1, rp.ctypes.data_as(C.POINTER(C.c_int64)),
+                                       col.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       val.ctypes.data_as(C.POINTER(C.c_double))), "csr_download_rows")
+        return HostMatrix(r1 - r0, rp, col[:cnt], val[:cnt])
+
     def release(self) -> None:
         if self.res._priv:
             check(lib().appt_csr_release(self._engine.handle, C.byref(self.res)), "csr_release")
@@ -221,6 +236,20 @@ class Engine:
               f"appt_spgemm_compute({op})")
         return DeviceResult(self, r)
 
+    def compute_rows(self, A: DeviceMatrix, op: str, r0: int, r1: int) -> DeviceResult:
+        """Row panel [r0, r1) of op(A) (global column indices)."""
+        r = CsrResult()
+        check(lib().appt_spgemm_compute_rows(self.handle, C.byref(A.view), OPS[op], r0, r1, C.byref(r)),
+              f"appt_spgemm_compute_rows({op}, {r0}, {r1})")
+        return DeviceResult(self, r)
+
+    def symbolic_rowptr(self, A: DeviceMatrix, op: str = "AA") -> np.ndarray:
+        """Exact int64 row pointers of op(A) from the symbolic phase."""
+        rp = np.empty(A.n + 1, dtype=np.int64)
+        check(lib().appt_spgemm_symbolic_rowptr(self.handle, C.byref(A.view), OPS[op],
+                                                rp.ctypes.data_as(C.POINTER(C.c_int64))), "symbolic_rowptr")
+        return rp
+
     def multiply(self, m: HostMatrix, op: str = "AA") -> HostMatrix:
         """Upload, compute, download, release: C = op(m) as a host matrix."""
         A = self.upload(m)
@@ -250,6 +279,27 @@ def _tuning_dict(t: Tuning) -> dict:
     return {"f_class_max": [int(x) for x in t.f_class_max if x > 0], "wg_lds_bytes": int(t.wg_lds_bytes),
             "lds_budget_bytes": int(t.lds_budget_bytes), "bitmap_words_factor": int(t.bitmap_words_factor),
             "group_for_len": [int(x) for x in t.group_for_len[:3]], "heavy_window_cols": int(t.heavy_window_cols)}
+
+
+def panels(rowptr: np.ndarray, max_bytes: int) -> list:
+    """Split rows into contiguous panels whose C (4 or 8 B per row pointer,
+    12 B per entry) stays within max_bytes; a single row larger than that is
+    a panel of its own."""
+    rowptr = np.asarray(rowptr, np.int64)
+    n = len(rowptr) - 1
+    out, r0 = [], 0
+    while r0 < n:
+        # largest r1 with 12 * (rowptr[r1] - rowptr[r0]) + 8 * (r1 - r0 + 1) <= max_bytes
+        lo, hi = r0 + 1, n
+        while lo < hi:
+            mid = (lo + hi + 1) // 2
+            if 12 * (rowptr[mid] - rowptr[r0]) + 8 * (mid - r0 + 1) <= max_bytes:
+                lo = mid
+            else:
+                hi = mid - 1
+        out.append((r0, lo))
+        r0 = lo
+    return out
 
 
 def pack(mats: list) -> HostMatrix:

@@ -147,6 +147,16 @@ appt_status appt_spgemm_compute(appt_engine* eng, const appt_csr_view* A, appt_o
                                 appt_csr_result* C);
 appt_status appt_csr_release(appt_engine* eng, appt_csr_result* C);
 
+/* Row panel: rows [row_begin, row_end) of op(A) (op AA or AAT) as a CSR with
+ * row_end - row_begin rows and global column indices.  For products whose C
+ * does not fit the device at once; no counterpart in the reference, whose
+ * int32 arena overflows first (src/spgemm_engine.cuh:156-162). */
+appt_status appt_spgemm_compute_rows(appt_engine* eng, const appt_csr_view* A, appt_op op, int32_t row_begin,
+                                     int32_t row_end, appt_csr_result* C);
+/* Exact row pointers of op(A) (n+1 int64, host memory) from the symbolic
+ * phase alone: sizes row panels before any numeric work. */
+appt_status appt_spgemm_symbolic_rowptr(appt_engine* eng, const appt_csr_view* A, appt_op op, int64_t* rowptr);
+
 /* run_ours(task, dOff, dCol, dVal, n, nnz, reps) (src/spgemm_engine.cuh:132-133):
  * one checked call (checksums from the sorted values on the host, :194-196),
  * then `reps` timed calls; ms is the mean per call over the timed loop
@@ -316,6 +326,11 @@ appt_status appt_gen_erdos_renyi(int32_t n, double mean_deg, uint64_t seed, appt
 /* ---- host <-> device helpers (for callers without their own allocator) ------- */
 appt_status appt_device_upload(appt_engine* eng, const appt_host_csr* h, appt_csr_view* out_dev);
 appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v);
+/* Rows [row_begin, row_end) of a device result to host arrays: rowptr gets
+ * row_end - row_begin + 1 int64 offsets relative to the first row, col / val
+ * that range's entries (sized by the caller from the row pointers). */
+appt_status appt_csr_download_rows(appt_engine* eng, const appt_csr_result* C, int32_t row_begin, int32_t row_end,
+                                   int64_t* rowptr, int32_t* col, double* val);
 /* Copy a device result to caller-provided host arrays (rowptr as int64);
  * eng may be NULL (the calling thread's current device is used). */
 appt_status appt_csr_download(appt_engine* eng, const appt_csr_result* C, int64_t* rowptr,

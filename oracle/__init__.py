@@ -98,9 +98,10 @@ def transpose(n, rowptr, col, val):
     return tp, tc[:len(cl)], tv[:len(cl)]
 
 
-def rows_range(n, rowptr, col, val, r0, r1, op="AA", threads=0, b=None):
+def rows_range(n, rowptr, col, val, r0, r1, op="AA", threads=0, b=None, want_scale=False):
     """Rows [r0, r1) of op(A) for op in {AA, AAt} (the CPU-baseline sample).
-    ``b`` may pass a prebuilt (rowptr, col, val) of B."""
+    ``b`` may pass a prebuilt (rowptr, col, val) of B.  Returns (rowptr, col,
+    val) or, with want_scale, (rowptr, col, val, sum |a*b| per entry)."""
     rp, cl, vl = _arr(rowptr, col, val)
     if b is None:
         b = (rp, cl, vl) if op == "AA" else transpose(n, rp, cl, vl)
@@ -117,9 +118,12 @@ def rows_range(n, rowptr, col, val, r0, r1, op="AA", threads=0, b=None):
     cols = np.empty(max(m, 1), np.int32)
     vals = np.empty(max(m, 1), np.float64)
     start = off[:-1].copy()
+    scale = np.empty(max(m, 1), np.float64) if want_scale else None
     if L.oracle_gustavson_range(*args, _p(start, C.c_int64), _p(cols, C.c_int32),
-                                _p(vals, C.c_double), None, threads):
+                                _p(vals, C.c_double), _p(scale, C.c_double) if want_scale else None, threads):
         raise MemoryError("oracle_gustavson_range failed")
+    if want_scale:
+        return off, cols[:m], vals[:m], scale[:m]
     return off, cols[:m], vals[:m]
 
 

@@ -123,3 +123,14 @@ def test_generator_argument_errors():
     assert e.value.status_name == "APPT_ERR_INVALID"
     with pytest.raises(ApptError):
         ap.rmat(12, 16, a=0.9, b=0.2, c=0.2)
+
+
+def test_panel_split():
+    # panels never exceed the byte bound unless a single row does
+    rp = np.concatenate([[0], np.cumsum(np.array([5, 0, 300, 7, 7, 7, 1, 0, 0, 40], np.int64))])
+    parts = ap.panels(rp, max_bytes=12 * 20 + 8 * 4)
+    assert parts[0][0] == 0 and parts[-1][1] == len(rp) - 1
+    assert all(a < b for a, b in parts) and all(parts[k][1] == parts[k + 1][0] for k in range(len(parts) - 1))
+    for a, b in parts:
+        size = 12 * (rp[b] - rp[a]) + 8 * (b - a + 1)
+        assert size <= 12 * 20 + 8 * 4 or b == a + 1

@@ -296,3 +296,41 @@ def test_full_size_stencil_c2(eng):
                     np.testing.assert_array_equal(got.val[lo:hi], v2)
         finally:
             A.free()
+
+
+@pytest.mark.parametrize("op", ["AA", "AAt"])
+def test_row_panels(eng, op):
+    # symbolic row pointers, then C in row panels of bounded bytes
+    m = ap.rmat(12, 16, seed=2)
+    say("case row panels rmat12", op)
+    r = ref(m, op)
+    A = eng.upload(m)
+    try:
+        rp = eng.symbolic_rowptr(A, op)
+        np.testing.assert_array_equal(rp, r[0])
+        parts = ap.panels(rp, max_bytes=400_000)
+        assert len(parts) > 3
+        cols, vals, rps = [], [], [np.zeros(1, np.int64)]
+        for r0, r1 in parts:
+            R = eng.compute_rows(A, op, r0, r1)
+            try:
+                h = R.download()
+            finally:
+                R.release()
+            assert h.n == r1 - r0
+            cols.append(h.col); vals.append(h.val)
+            rps.append(np.asarray(h.rowptr[1:], np.int64) + rps[-1][-1])
+        got = ap.HostMatrix(m.n, np.concatenate(rps), np.concatenate(cols), np.concatenate(vals))
+        assert_product(got, r, exact=False)
+    finally:
+        A.free()
+
+
+def test_row_panels_reject_aats(eng):
+    m = ap.laplace2d(16)
+    A = eng.upload(m)
+    try:
+        with pytest.raises(ap.ApptError):
+            eng.compute_rows(A, "AAtS", 0, 10)
+    finally:
+        A.free()
