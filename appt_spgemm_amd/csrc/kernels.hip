@@ -289,6 +289,11 @@ __host__ __device__ __forceinline__ int64_t sym_window_bytes(int G, int64_t W) {
 }
 // numeric window LDS per group: meta (G int2 + G f64) | bits (W + G u32) | pre (W u16) |
 // vals (Cm + G f64) | cols (Cm i32)
+// numeric list-window LDS per group: meta (G int2 + G f64) | word index (W + G u8) |
+// listed words (kWordCap x (mask, first position)) | vals (Cm + G f64)
+__host__ __device__ __forceinline__ int64_t num_list_bytes(int G, int64_t W, int64_t Cm) {
+  return align16(G * 16) + align16(W + G) + align16(32 * 8) + align16((Cm + G) * 8);
+}
 __host__ __device__ __forceinline__ int64_t num_window_bytes(int G, int64_t W, int64_t Cm) {
   return align16(G * 16) + align16((W + G) * 4) + align16(W * 2) + align16((Cm + G) * 8) + align16(Cm * 4);
 }
@@ -332,7 +337,7 @@ __host__ __device__ __forceinline__ int64_t whash_bytes_hd(int64_t w, int64_t c)
 
 // Numeric bin from row nnz c, span words, distinct words w and L.
 __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t words, int64_t w, int64_t f,
-                                          int64_t nA, int64_t* stat) {
+                                          int64_t nA, bool listed, int64_t* stat) {
   *stat = 0;
   if (c <= 0 || words <= 0) return 0;
   const int gidx = group_index(P, f, nA);
@@ -341,6 +346,7 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
   const int64_t dense_bytes = ((words * 8 + c * 12) + 15) & ~(int64_t)15;
   for (int cl = 0; cl < P.ncls; cl++)
     if (c <= P.cls_max[cl]) {
+      if (listed && num_list_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) { *stat = words; return kBinList0 + cl * 4 + gidx; }
       if (num_window_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) { *stat = words; return 1 + (cl * 2 + 0) * 4 + gidx; }
       if (whash_bytes_hd(w, c) <= P.cls_bytes[cl]) { *stat = w; return 1 + (cl * 2 + 1) * 4 + gidx; }
       break;
@@ -542,7 +548,9 @@ __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_
       c = rowcnt[i];
       const int64_t words = span_words(rowlo[i], rowhi[i]);
       const int64_t nA = ap[i + 1] - ap[i];
-      b = num_bin_of(P, c, words, rowwords[i] & kWordsMask, rowf[i], nA, &stat);
+      const int rw = rowwords[i];
+      const bool listed = (rw & kNoWordList) == 0 && (rw & kWordsMask) <= kWordListCap;
+      b = num_bin_of(P, c, words, rw & kWordsMask, rowf[i], nA, listed, &stat);
       bin[i] = (uint8_t)b;
     }
     stats_add_wave(st, b, c, stat);
@@ -1408,16 +1416,25 @@ __device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool ln
 
 // Value pass for one staged chunk (mbe, ma): a_ik * b_kj added at its final
 // sorted position, in A-row storage order per output column.
-template <int G, bool UPPER, bool W64>
+template <int G, bool UPPER, bool W64, bool LIST>
 __device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool lng, const RowArgs& A,
                                             const int2* mbe, const double* ma, const uint32_t* bits,
-                                            const uint16_t* pre, double* vals, int Cm, int W) {
+                                            const uint16_t* pre, const uint8_t* widx, const uint2* wtab,
+                                            double* vals, int Cm, int W) {
   constexpr int B = G < 16 ? G : 16;
   auto add = [&](int j, double prod, bool ok) {
     const bool use = ok && (!UPPER || j >= i);
     const uint32_t bb = use ? (uint32_t)(j - lo) : 0u;
     const uint32_t w = (bb >> 5) < (uint32_t)W ? (bb >> 5) : 0u;  // in range by construction
-    const uint32_t pos = (uint32_t)pre[w] + __popc(bits[w] & ((1u << (bb & 31u)) - 1u));
+    uint32_t pos;
+    if constexpr (LIST) {
+      // word -> listed entry (mask, first position); indices < kWordCap by construction
+      const uint32_t x = widx[w];
+      const uint2 te = wtab[x < (uint32_t)kWordCap ? x : 0u];
+      pos = te.y + __popc(te.x & ((1u << (bb & 31u)) - 1u));
+    } else {
+      pos = (uint32_t)pre[w] + __popc(bits[w] & ((1u << (bb & 31u)) - 1u));
+    }
     // positions are < c <= Cm by construction; the clamp keeps a damaged
     // input from writing outside this group's LDS region
     const uint32_t slot = use ? (pos < (uint32_t)Cm ? pos : (uint32_t)Cm) : (uint32_t)(Cm + lane);
@@ -1542,21 +1559,35 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
   }
 }
 
-template <int G, bool UPPER, bool W64>
+template <int G, bool UPPER, bool W64, bool LIST>
 __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_t Cm, int32_t chunk,
                                                     const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
                                                     double* __restrict__ cval) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int R = blockDim.x / G;
   const int gi = threadIdx.x / G, lane = threadIdx.x % G;
-  unsigned char* base = lds_raw + (size_t)gi * num_window_bytes(G, W, Cm);
+  unsigned char* base = lds_raw + (size_t)gi * (LIST ? num_list_bytes(G, W, Cm) : num_window_bytes(G, W, Cm));
   int2* mbe = (int2*)base;
   double* ma = (double*)(base + G * 8);
-  uint32_t* bits = (uint32_t*)(base + align16(G * 16));
-  uint16_t* pre = (uint16_t*)((unsigned char*)bits + align16((W + G) * 4));
-  double* vals = (double*)((unsigned char*)pre + align16(W * 2));
-  int32_t* cols = (int32_t*)((unsigned char*)vals + align16((Cm + G) * 8));
-  for (int t = lane; t < W + G; t += G) bits[t] = 0u;
+  // dense layout: bitmap | prefix counts | vals | cols
+  // list layout:  word index | listed words | vals (columns go straight to C)
+  uint32_t* bits = nullptr;
+  uint16_t* pre = nullptr;
+  int32_t* cols = nullptr;
+  uint8_t* widx = nullptr;
+  uint2* wtab = nullptr;
+  double* vals;
+  if constexpr (LIST) {
+    widx = base + align16(G * 16);
+    wtab = (uint2*)(widx + align16(W + G));
+    vals = (double*)((unsigned char*)wtab + align16(32 * 8));
+  } else {
+    bits = (uint32_t*)(base + align16(G * 16));
+    pre = (uint16_t*)((unsigned char*)bits + align16((W + G) * 4));
+    vals = (double*)((unsigned char*)pre + align16(W * 2));
+    cols = (int32_t*)((unsigned char*)vals + align16((Cm + G) * 8));
+    for (int t = lane; t < W + G; t += G) bits[t] = 0u;
+  }
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   const int r0 = lb * chunk;
   const int r1 = (r0 + chunk) < A.nrows ? (r0 + chunk) : A.nrows;
@@ -1620,18 +1651,27 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
         const int ex = gscan_excl<G, int>(__popc(e.y), lane, &tot);
         int rr = run + ex;
         if (u < nt && e.x < (uint32_t)W) {   // in range by construction
-          bits[e.x] = e.y;
-          pre[e.x] = (uint16_t)rr;
           uint32_t x = e.y;
-          while (x && rr < Cm) {
-            cols[rr++] = lo + (int)e.x * 32 + __builtin_ctz(x);
-            x &= x - 1u;
+          if constexpr (LIST) {
+            widx[e.x] = (uint8_t)u;
+            wtab[u] = make_uint2(e.y, (uint32_t)rr);
+            while (x && rr < c) {
+              ccol[off + rr++] = lo + (int)e.x * 32 + __builtin_ctz(x);
+              x &= x - 1u;
+            }
+          } else {
+            bits[e.x] = e.y;
+            pre[e.x] = (uint16_t)rr;
+            while (x && rr < Cm) {
+              cols[rr++] = lo + (int)e.x * 32 + __builtin_ctz(x);
+              x &= x - 1u;
+            }
           }
         }
         myw[k] = e.x;
         run += tot;
       }
-    } else {
+    } else if constexpr (!LIST) {
       gsync<G>();
       const int m0 = (q0.e - q0.s) < G ? (q0.e - q0.s) : G;
       or_chunk<G, UPPER, W64, false>(i, lo, lane, m0, lng, A, mbe, bits, W, nullptr, nullptr);
@@ -1668,21 +1708,23 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
     gsync<G>();
     {
       const int m0 = (q0.e - q0.s) < G ? (q0.e - q0.s) : G;
-      value_chunk<G, UPPER, W64>(i, lo, lane, m0, lng, A, mbe, ma, bits, pre, vals, Cm, W);
+      value_chunk<G, UPPER, W64, LIST>(i, lo, lane, m0, lng, A, mbe, ma, bits, pre, widx, wtab, vals, Cm, W);
       gsync<G>();
       for (int b0 = q0.s + G; b0 < q0.e; b0 += G) {                // long A rows
         const bool l2 = stage_entries<G, true>(b0, q0.e, lane, A, mbe, ma);
         gsync<G>();
         const int m = (q0.e - b0) < G ? (q0.e - b0) : G;
-        value_chunk<G, UPPER, W64>(i, lo, lane, m, l2, A, mbe, ma, bits, pre, vals, Cm, W);
+        value_chunk<G, UPPER, W64, LIST>(i, lo, lane, m, l2, A, mbe, ma, bits, pre, widx, wtab, vals, Cm, W);
         gsync<G>();
       }
     }
     for (int t = lane; t < c; t += G) {
-      ccol[off + t] = cols[t];
+      if constexpr (!LIST) ccol[off + t] = cols[t];
       cval[off + t] = vals[t];
     }
-    if (have_list) {
+    if constexpr (LIST) {
+      // word index entries are rewritten by the next row's placement
+    } else if (have_list) {
 #pragma unroll
       for (int k = 0; k < PL; k++)
         if (lane + k * G < nt && myw[k] < (uint32_t)W) bits[myw[k]] = 0u;
@@ -1716,6 +1758,7 @@ int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max) {
   if (bin == kBinHeavy || bin == 0) return 0;
   if (bin == kBinWgBitmap) return dense_row_bytes(words_max, c_max);
+  if (bin_is_list(bin)) return num_list_bytes(bin_group(bin), words_max, c_max);
   if (bin_dense(bin)) return num_window_bytes(bin_group(bin), words_max, c_max);
   return whash_row_bytes(words_max, c_max);
 }
@@ -1788,16 +1831,17 @@ static void sym_window_launch(const RowArgs& a, int32_t W, int32_t* rowcnt, hipS
   (void)hipFuncSetAttribute((const void*)k_sym_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   hipLaunchKernelGGL((k_sym_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, chunk, rowcnt);
 }
-template <int G, bool UPPER, bool W64>
+template <int G, bool UPPER, bool W64, bool LIST>
 static void num_window_launch(const RowArgs& a, int32_t W, int32_t Cm, const int64_t* rowoff, int32_t* ccol,
                               double* cval, hipStream_t s) {
-  const int64_t gb = num_window_bytes(G, W, Cm);
+  const int64_t gb = LIST ? num_list_bytes(G, W, Cm) : num_window_bytes(G, W, Cm);
   int R; unsigned blocks; int chunk;
   window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks, &chunk);
   const size_t lds = (size_t)R * gb;
-  (void)hipFuncSetAttribute((const void*)k_num_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_num_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, Cm, chunk, rowoff,
-                     ccol, cval);
+  (void)hipFuncSetAttribute((const void*)k_num_window<G, UPPER, W64, LIST>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_num_window<G, UPPER, W64, LIST>), dim3(blocks), dim3(R * G), lds, s, a, W, Cm, chunk,
+                     rowoff, ccol, cval);
 }
 template <int G>
 static void sym_window_any(const RowArgs& a, int32_t W, int32_t* rowcnt, hipStream_t s) {
@@ -1805,16 +1849,16 @@ static void sym_window_any(const RowArgs& a, int32_t W, int32_t* rowcnt, hipStre
   if (a.upper) { if (wide) sym_window_launch<G, true, true>(a, W, rowcnt, s); else sym_window_launch<G, true, false>(a, W, rowcnt, s); }
   else { if (wide) sym_window_launch<G, false, true>(a, W, rowcnt, s); else sym_window_launch<G, false, false>(a, W, rowcnt, s); }
 }
-template <int G>
+template <int G, bool LIST>
 static void num_window_any(const RowArgs& a, int32_t W, int32_t Cm, const int64_t* rowoff, int32_t* ccol,
                            double* cval, hipStream_t s) {
   const bool wide = a.bnnz >= kNarrowIdx;
   if (a.upper) {
-    if (wide) num_window_launch<G, true, true>(a, W, Cm, rowoff, ccol, cval, s);
-    else num_window_launch<G, true, false>(a, W, Cm, rowoff, ccol, cval, s);
+    if (wide) num_window_launch<G, true, true, LIST>(a, W, Cm, rowoff, ccol, cval, s);
+    else num_window_launch<G, true, false, LIST>(a, W, Cm, rowoff, ccol, cval, s);
   } else {
-    if (wide) num_window_launch<G, false, true>(a, W, Cm, rowoff, ccol, cval, s);
-    else num_window_launch<G, false, false>(a, W, Cm, rowoff, ccol, cval, s);
+    if (wide) num_window_launch<G, false, true, LIST>(a, W, Cm, rowoff, ccol, cval, s);
+    else num_window_launch<G, false, false, LIST>(a, W, Cm, rowoff, ccol, cval, s);
   }
 }
 
@@ -1879,18 +1923,27 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
     heavy_launch(true, a, h, h.work + 1, nullptr, rowoff, ccol, cval, s);
     return;
   }
+  if (bin_is_list(bin)) {
+    switch (bin_group(bin)) {
+      case 8: num_window_any<8, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+      case 16: num_window_any<16, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+      case 32: num_window_any<32, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+      default: num_window_any<64, true>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    }
+    return;
+  }
   if (bin == kBinWgBitmap) { num_lds<256, true>(a, words_max, c_max, rowoff, ccol, cval, s); return; }
   if (bin == kBinWgHash) { num_lds<256, false>(a, words_max, c_max, rowoff, ccol, cval, s); return; }
   const int G = bin_group(bin);
   const bool bm = bin_is_bitmap(bin);
   switch (G * 2 + (bm ? 1 : 0)) {
-    case 17: num_window_any<8>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 17: num_window_any<8, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 16: num_lds<8, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
-    case 33: num_window_any<16>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 33: num_window_any<16, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 32: num_lds<16, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
-    case 65: num_window_any<32>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 65: num_window_any<32, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 64: num_lds<32, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
-    case 129: num_window_any<64>(a, words_max, c_max, rowoff, ccol, cval, s); break;
+    case 129: num_window_any<64, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     case 128: num_lds<64, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     default: break;
   }

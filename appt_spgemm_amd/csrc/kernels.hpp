@@ -10,6 +10,10 @@
 //   25 / 26           one workgroup per row, dense / word hash in LDS
 //   27                heavy rows: one workgroup per row, the output span
 //                     walked in LDS column windows
+//   28 .. 39          numeric only: wave-level groups for rows whose symbolic
+//                     pass left a sorted word list, 28 + cls*4 + gidx; the
+//                     LDS holds a byte index per window word and the listed
+//                     words instead of a bitmap and prefix counts
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -17,8 +21,8 @@
 namespace appt {
 
 constexpr int kNumCls = 3;
-constexpr int kNumBins = 28;
-constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27;
+constexpr int kNumBins = 40;
+constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27, kBinList0 = 28;
 constexpr int kHeavyT = 1024;   // threads of a heavy-row workgroup
 constexpr int kStatFields = 3;  // count, max metric, max words
 constexpr int kStatsMaxBlocks = 4096;
@@ -35,9 +39,10 @@ struct BinParams {
   int32_t lds_budget;             // LDS bytes a block of the wave-level bins may take
 };
 
-inline int bin_group(int bin) { return 8 << ((bin - 1) % 4); }
-inline int bin_cls(int bin) { return (bin - 1) / 8; }
-inline bool bin_is_bitmap(int bin) { return ((bin - 1) / 4) % 2 == 0; }
+inline bool bin_is_list(int bin) { return bin >= kBinList0; }
+inline int bin_group(int bin) { return bin_is_list(bin) ? 8 << ((bin - kBinList0) % 4) : 8 << ((bin - 1) % 4); }
+inline int bin_cls(int bin) { return bin_is_list(bin) ? (bin - kBinList0) / 4 : (bin - 1) / 8; }
+inline bool bin_is_bitmap(int bin) { return !bin_is_list(bin) && ((bin - 1) / 4) % 2 == 0; }
 
 // Symbolic hash table slots for a row with f products (np2l((4f)/3+1) of the
 // reference sizing rule); numeric hash slots for a row with c outputs.
