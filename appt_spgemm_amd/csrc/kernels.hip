@@ -670,6 +670,27 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
+// Rows of a persistent window block.  The bin's row list is split into eight
+// contiguous ranges, one per XCD (block b runs on XCD b % 8), sized by the
+// XCD's block count; inside its range the blocks of one XCD take tiles of R
+// rows in turn (group gi owns row gi of a tile).  At any moment the blocks
+// of an XCD work on neighbouring rows, so the B rows they gather are shared
+// in that XCD's L2.
+struct RowWalk {
+  int first, step, end;
+};
+__device__ __forceinline__ RowWalk row_walk(int nrows, int R, int gi) {
+  constexpr int kXcd = 8;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b % kXcd, k = b / kXcd;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  const int nbx = per + (x < rem ? 1 : 0);
+  const int before = x * per + (x < rem ? x : rem);
+  const int x0 = (int)((int64_t)nrows * before / nb);
+  const int x1 = (int)((int64_t)nrows * (before + nbx) / nb);
+  return RowWalk{x0 + k * R + gi, nbx * R, x1};
+}
+
 // fn(j, prod) is called once per product with j = B column and prod = a*b
 // (rounded product; 0 when VALS is false).
 //
@@ -1309,8 +1330,8 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy(RowArgs A, int32_t wc, int32_
 }
 // ---------------------------------------------------------------------------
 // Window kernels (dense mode of the wave-level bins).  Persistent: each block
-// owns a contiguous chunk of the bin's row list (XCD-aware numbering), and
-// each group of G lanes walks rows of that chunk.  Per group, LDS holds a
+// walks rows of the bin's row list (row_walk: XCD-local ranges, tiles dealt
+// to the XCD's blocks in turn), one row per group of G lanes at a time.  Per group, LDS holds a
 // bitmap over the row's output window, kept all-zero between rows by
 // clearing only the words the row touched (recorded in a touched list), so
 // the per-row cost does not grow with the window width.  The product loop is
@@ -1421,7 +1442,10 @@ __device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool
                                             const int2* mbe, const double* ma, const uint32_t* bits,
                                             const uint16_t* pre, const uint8_t* widx, const uint2* wtab,
                                             double* vals, int Cm, int W) {
-  constexpr int B = G < 16 ? G : 16;
+  // gathers in flight per lane; the list variant trades depth for occupancy
+  // (fewer VGPRs: 4 waves per SIMD instead of 3)
+  constexpr int kDepth = LIST ? 8 : 16;
+  constexpr int B = G < kDepth ? G : kDepth;
   auto add = [&](int j, double prod, bool ok) {
     const bool use = ok && (!UPPER || j >= i);
     const uint32_t bb = use ? (uint32_t)(j - lo) : 0u;
@@ -1474,7 +1498,7 @@ __device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool
 // Symbolic window: row count, distinct words, and (when it fits) the row's
 // sorted word list (word, mask) for the numeric kernel.
 template <int G, bool UPPER, bool W64>
-__global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_t chunk, int32_t* __restrict__ rowcnt) {
+__global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_t* __restrict__ rowcnt) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int R = blockDim.x / G;
   const int gi = threadIdx.x / G, lane = threadIdx.x % G;
@@ -1484,26 +1508,25 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
   uint32_t* touched = (uint32_t*)((unsigned char*)bits + align16((W + G) * 4));
   int* ctl = (int*)((unsigned char*)touched + align16(kTouchCap * 4));
   for (int t = lane; t < W + G; t += G) bits[t] = 0u;
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = lb * chunk;
-  const int r1 = (r0 + chunk) < A.nrows ? (r0 + chunk) : A.nrows;
-  int r = r0 + gi;
+  const RowWalk walk = row_walk(A.nrows, R, gi);
+  const int S = walk.step, r1 = walk.end;
+  int r = walk.first;
   // pipeline prologue: rows r .. r+3R filled to hop depths 4, 3, 2, 1
   RowState q0{}, q1{}, q2{}, q3{};
   q0.k = q1.k = q2.k = q3.k = -1;
   if (r < r1) { q0.i = A.rows[r]; hop_scalars(q0, A); hop_entry(q0, lane, A); hop_bmeta<false>(q0, lane, A); }
-  if (r + R < r1) { q1.i = A.rows[r + R]; hop_scalars(q1, A); hop_entry(q1, lane, A); }
-  if (r + 2 * R < r1) { q2.i = A.rows[r + 2 * R]; hop_scalars(q2, A); }
-  if (r + 3 * R < r1) q3.i = A.rows[r + 3 * R];
+  if (r + S < r1) { q1.i = A.rows[r + S]; hop_scalars(q1, A); hop_entry(q1, lane, A); }
+  if (r + 2 * S < r1) { q2.i = A.rows[r + 2 * S]; hop_scalars(q2, A); }
+  if (r + 3 * S < r1) q3.i = A.rows[r + 3 * S];
   gsync<G>();
-  for (; r < r1; r += R) {
+  for (; r < r1; r += S) {
     // hops for the rows ahead (each uses values requested one iteration ago)
     RowState q4{};
     q4.k = -1;
-    if (r + 4 * R < r1) q4.i = A.rows[r + 4 * R];
-    if (r + 3 * R < r1) hop_scalars(q3, A);
-    if (r + 2 * R < r1) hop_entry(q2, lane, A);
-    if (r + R < r1) hop_bmeta<false>(q1, lane, A);
+    if (r + 4 * S < r1) q4.i = A.rows[r + 4 * S];
+    if (r + 3 * S < r1) hop_scalars(q3, A);
+    if (r + 2 * S < r1) hop_entry(q2, lane, A);
+    if (r + S < r1) hop_bmeta<false>(q1, lane, A);
     const int i = q0.i;
     const int lo = q0.lo;
     if (lane == 0) ctl[0] = 0;
@@ -1560,7 +1583,7 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
 }
 
 template <int G, bool UPPER, bool W64, bool LIST>
-__global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_t Cm, int32_t chunk,
+__global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_t Cm,
                                                     const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
                                                     double* __restrict__ cval) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
@@ -1588,9 +1611,8 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
     cols = (int32_t*)((unsigned char*)vals + align16((Cm + G) * 8));
     for (int t = lane; t < W + G; t += G) bits[t] = 0u;
   }
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int r0 = lb * chunk;
-  const int r1 = (r0 + chunk) < A.nrows ? (r0 + chunk) : A.nrows;
+  const RowWalk walk = row_walk(A.nrows, R, gi);
+  const int S = walk.step, r1 = walk.end;
   constexpr int PL = (kWordCap + G - 1) / G;
   // per-row output head: offsets, word count, sorted word list
   struct Head {
@@ -1607,7 +1629,7 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
     for (int k = 0; k < PL; k++)   // read without checking the count; used only if valid
       h.ent[k] = (lane + k * G) < kWordCap ? wl[lane + k * G] : make_uint2(0u, 0u);
   };
-  int r = r0 + gi;
+  int r = walk.first;
   // pipeline prologue: rows r .. r+3R filled to hop depths 4, 3, 2, 1
   RowState q0{}, q1{}, q2{}, q3{};
   Head h0{}, h1{}, h2{}, h3{};
@@ -1615,18 +1637,18 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
   if (r < r1) {
     q0.i = A.rows[r]; hop_scalars(q0, A); load_head(q0.i, h0); hop_entry(q0, lane, A); hop_bmeta<true>(q0, lane, A);
   }
-  if (r + R < r1) { q1.i = A.rows[r + R]; hop_scalars(q1, A); load_head(q1.i, h1); hop_entry(q1, lane, A); }
-  if (r + 2 * R < r1) { q2.i = A.rows[r + 2 * R]; hop_scalars(q2, A); load_head(q2.i, h2); }
-  if (r + 3 * R < r1) q3.i = A.rows[r + 3 * R];
+  if (r + S < r1) { q1.i = A.rows[r + S]; hop_scalars(q1, A); load_head(q1.i, h1); hop_entry(q1, lane, A); }
+  if (r + 2 * S < r1) { q2.i = A.rows[r + 2 * S]; hop_scalars(q2, A); load_head(q2.i, h2); }
+  if (r + 3 * S < r1) q3.i = A.rows[r + 3 * S];
   gsync<G>();
-  for (; r < r1; r += R) {
+  for (; r < r1; r += S) {
     // hops for the rows ahead (each uses values requested one iteration ago)
     RowState q4{};
     q4.k = -1;
-    if (r + 4 * R < r1) q4.i = A.rows[r + 4 * R];
-    if (r + 3 * R < r1) { hop_scalars(q3, A); load_head(q3.i, h3); }
-    if (r + 2 * R < r1) hop_entry(q2, lane, A);
-    if (r + R < r1) hop_bmeta<true>(q1, lane, A);
+    if (r + 4 * S < r1) q4.i = A.rows[r + 4 * S];
+    if (r + 3 * S < r1) { hop_scalars(q3, A); load_head(q3.i, h3); }
+    if (r + 2 * S < r1) hop_entry(q2, lane, A);
+    if (r + S < r1) hop_bmeta<true>(q1, lane, A);
     const int i = q0.i;
     const int lo = q0.lo;
     const int64_t off = h0.off;
@@ -1802,7 +1824,7 @@ void set_lds_per_cu(int64_t b) { g_lds_per_cu = b > 0 ? b : 65536; }
 
 // Groups per block for a window kernel and the persistent grid.
 static void window_shape(int G, int64_t group_bytes, int32_t nrows, int64_t lds_budget, int* R_out,
-                         unsigned* blocks_out, int* chunk_out) {
+                         unsigned* blocks_out) {
   int64_t budget = g_lds_per_cu / 2;   // two blocks per CU
   if (budget > lds_budget * 2) budget = lds_budget * 2;
   int R = (int)(budget / (group_bytes > 0 ? group_bytes : 1));
@@ -1817,7 +1839,6 @@ static void window_shape(int G, int64_t group_bytes, int32_t nrows, int64_t lds_
   if (blocks < 1) blocks = 1;
   *R_out = R;
   *blocks_out = (unsigned)blocks;
-  *chunk_out = (int)((nrows + blocks - 1) / blocks);
 }
 
 constexpr int64_t kNarrowIdx = (int64_t)1 << 29;  // 32-bit byte offsets into B's f64 values
@@ -1825,22 +1846,22 @@ constexpr int64_t kNarrowIdx = (int64_t)1 << 29;  // 32-bit byte offsets into B'
 template <int G, bool UPPER, bool W64>
 static void sym_window_launch(const RowArgs& a, int32_t W, int32_t* rowcnt, hipStream_t s) {
   const int64_t gb = sym_window_bytes(G, W);
-  int R; unsigned blocks; int chunk;
-  window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks, &chunk);
+  int R; unsigned blocks;
+  window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks);
   const size_t lds = (size_t)R * gb;
   (void)hipFuncSetAttribute((const void*)k_sym_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_sym_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, chunk, rowcnt);
+  hipLaunchKernelGGL((k_sym_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, rowcnt);
 }
 template <int G, bool UPPER, bool W64, bool LIST>
 static void num_window_launch(const RowArgs& a, int32_t W, int32_t Cm, const int64_t* rowoff, int32_t* ccol,
                               double* cval, hipStream_t s) {
   const int64_t gb = LIST ? num_list_bytes(G, W, Cm) : num_window_bytes(G, W, Cm);
-  int R; unsigned blocks; int chunk;
-  window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks, &chunk);
+  int R; unsigned blocks;
+  window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks);
   const size_t lds = (size_t)R * gb;
   (void)hipFuncSetAttribute((const void*)k_num_window<G, UPPER, W64, LIST>,
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_num_window<G, UPPER, W64, LIST>), dim3(blocks), dim3(R * G), lds, s, a, W, Cm, chunk,
+  hipLaunchKernelGGL((k_num_window<G, UPPER, W64, LIST>), dim3(blocks), dim3(R * G), lds, s, a, W, Cm,
                      rowoff, ccol, cval);
 }
 template <int G>
