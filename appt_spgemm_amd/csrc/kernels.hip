@@ -289,10 +289,13 @@ __host__ __device__ __forceinline__ int64_t sym_window_bytes(int G, int64_t W) {
 }
 // numeric window LDS per group: meta (G int2 + G f64) | bits (W + G u32) | pre (W u16) |
 // vals (Cm + G f64) | cols (Cm i32)
+// Longest word list a numeric group of G lanes takes (G >= 16: the full
+// list; 8-lane groups hold at most 4 entries per lane in registers).
+__host__ __device__ __forceinline__ constexpr int list_cap(int G) { return G >= 16 ? kWordListCap : 32; }
 // numeric list-window LDS per group: meta (G int2 + G f64) | word index (W + G u8) |
-// listed words (kWordCap x (mask, first position)) | vals (Cm + G f64)
+// listed words (list_cap(G) x (mask, first position)) | vals (Cm + G f64)
 __host__ __device__ __forceinline__ int64_t num_list_bytes(int G, int64_t W, int64_t Cm) {
-  return align16(G * 16) + align16(W + G) + align16(32 * 8) + align16((Cm + G) * 8);
+  return align16(G * 16) + align16(W + G) + align16(list_cap(G) * 8) + align16((Cm + G) * 8);
 }
 __host__ __device__ __forceinline__ int64_t num_window_bytes(int G, int64_t W, int64_t Cm) {
   return align16(G * 16) + align16((W + G) * 4) + align16(W * 2) + align16((Cm + G) * 8) + align16(Cm * 4);
@@ -337,7 +340,7 @@ __host__ __device__ __forceinline__ int64_t whash_bytes_hd(int64_t w, int64_t c)
 
 // Numeric bin from row nnz c, span words, distinct words w and L.
 __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t words, int64_t w, int64_t f,
-                                          int64_t nA, bool listed, int64_t* stat) {
+                                          int64_t nA, int64_t listed, int64_t* stat) {
   *stat = 0;
   if (c <= 0 || words <= 0) return 0;
   const int gidx = group_index(P, f, nA);
@@ -346,7 +349,11 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
   const int64_t dense_bytes = ((words * 8 + c * 12) + 15) & ~(int64_t)15;
   for (int cl = 0; cl < P.ncls; cl++)
     if (c <= P.cls_max[cl]) {
-      if (listed && num_list_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) { *stat = words; return kBinList0 + cl * 4 + gidx; }
+      // listed: the row's word-list length, or -1 without a list
+      if (listed >= 0 && listed <= list_cap(8 << gidx) && num_list_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) {
+        *stat = words;
+        return kBinList0 + cl * 4 + gidx;
+      }
       if (num_window_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) { *stat = words; return 1 + (cl * 2 + 0) * 4 + gidx; }
       if (whash_bytes_hd(w, c) <= P.cls_bytes[cl]) { *stat = w; return 1 + (cl * 2 + 1) * 4 + gidx; }
       break;
@@ -549,7 +556,7 @@ __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_
       const int64_t words = span_words(rowlo[i], rowhi[i]);
       const int64_t nA = ap[i + 1] - ap[i];
       const int rw = rowwords[i];
-      const bool listed = (rw & kNoWordList) == 0 && (rw & kWordsMask) <= kWordListCap;
+      const int64_t listed = (rw & kNoWordList) == 0 && (rw & kWordsMask) <= kWordListCap ? (rw & kWordsMask) : -1;
       b = num_bin_of(P, c, words, rw & kWordsMask, rowf[i], nA, listed, &stat);
       bin[i] = (uint8_t)b;
     }
@@ -1341,7 +1348,7 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy(RowArgs A, int32_t wc, int32_
 // ---------------------------------------------------------------------------
 
 constexpr int kBatch = 8;     // A entries whose B gathers are issued together
-constexpr int kWordCap = 32;  // words per row kept in the symbolic -> numeric word list
+constexpr int kWordCap = kWordListCap;  // words per row kept in the symbolic -> numeric word list
 
 // Stage G A entries of a row chunk: (bs, be) of their B rows (zero-padded so
 // a batch may read past the chunk end) and, for the value pass, a_ik.
@@ -1452,9 +1459,9 @@ __device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool
     const uint32_t w = (bb >> 5) < (uint32_t)W ? (bb >> 5) : 0u;  // in range by construction
     uint32_t pos;
     if constexpr (LIST) {
-      // word -> listed entry (mask, first position); indices < kWordCap by construction
+      // word -> listed entry (mask, first position); indices < list_cap(G) by construction
       const uint32_t x = widx[w];
-      const uint2 te = wtab[x < (uint32_t)kWordCap ? x : 0u];
+      const uint2 te = wtab[x < (uint32_t)list_cap(G) ? x : 0u];
       pos = te.y + __popc(te.x & ((1u << (bb & 31u)) - 1u));
     } else {
       pos = (uint32_t)pre[w] + __popc(bits[w] & ((1u << (bb & 31u)) - 1u));
@@ -1603,7 +1610,7 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
   if constexpr (LIST) {
     widx = base + align16(G * 16);
     wtab = (uint2*)(widx + align16(W + G));
-    vals = (double*)((unsigned char*)wtab + align16(32 * 8));
+    vals = (double*)((unsigned char*)wtab + align16(list_cap(G) * 8));
   } else {
     bits = (uint32_t*)(base + align16(G * 16));
     pre = (uint16_t*)((unsigned char*)bits + align16((W + G) * 4));
@@ -1613,21 +1620,27 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
   }
   const RowWalk walk = row_walk(A.nrows, R, gi);
   const int S = walk.step, r1 = walk.end;
-  constexpr int PL = (kWordCap + G - 1) / G;
+  constexpr int kCap = list_cap(G);
+  constexpr int PL = (kCap + G - 1) / G;
   // per-row output head: offsets, word count, sorted word list
   struct Head {
     int64_t off, end;
     int rw;
     uint2 ent[PL];
   };
-  auto load_head = [&](int i, Head& h) {
+  auto load_head = [&](int i, Head& h) {  // scalars; the list one hop later
     h.off = rowoff[i];
     h.end = rowoff[i + 1];
     h.rw = A.rowwords[i];
+  };
+  auto load_list = [&](int i, Head& h) {  // only the listed entries
+    const int nt = (h.rw & kNoWordList) ? 0 : (h.rw & kWordsMask);
     const uint2* wl = A.wlist + (size_t)i * kWordCap;
 #pragma unroll
-    for (int k = 0; k < PL; k++)   // read without checking the count; used only if valid
-      h.ent[k] = (lane + k * G) < kWordCap ? wl[lane + k * G] : make_uint2(0u, 0u);
+    for (int k = 0; k < PL; k++) {
+      const int u = lane + k * G;
+      h.ent[k] = (u < nt && u < kCap) ? wl[u] : make_uint2(0u, 0u);
+    }
   };
   int r = walk.first;
   // pipeline prologue: rows r .. r+3R filled to hop depths 4, 3, 2, 1
@@ -1635,9 +1648,12 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
   Head h0{}, h1{}, h2{}, h3{};
   q0.k = q1.k = q2.k = q3.k = -1;
   if (r < r1) {
-    q0.i = A.rows[r]; hop_scalars(q0, A); load_head(q0.i, h0); hop_entry(q0, lane, A); hop_bmeta<true>(q0, lane, A);
+    q0.i = A.rows[r]; hop_scalars(q0, A); load_head(q0.i, h0); load_list(q0.i, h0);
+    hop_entry(q0, lane, A); hop_bmeta<true>(q0, lane, A);
   }
-  if (r + S < r1) { q1.i = A.rows[r + S]; hop_scalars(q1, A); load_head(q1.i, h1); hop_entry(q1, lane, A); }
+  if (r + S < r1) {
+    q1.i = A.rows[r + S]; hop_scalars(q1, A); load_head(q1.i, h1); load_list(q1.i, h1); hop_entry(q1, lane, A);
+  }
   if (r + 2 * S < r1) { q2.i = A.rows[r + 2 * S]; hop_scalars(q2, A); load_head(q2.i, h2); }
   if (r + 3 * S < r1) q3.i = A.rows[r + 3 * S];
   gsync<G>();
@@ -1647,14 +1663,14 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
     q4.k = -1;
     if (r + 4 * S < r1) q4.i = A.rows[r + 4 * S];
     if (r + 3 * S < r1) { hop_scalars(q3, A); load_head(q3.i, h3); }
-    if (r + 2 * S < r1) hop_entry(q2, lane, A);
+    if (r + 2 * S < r1) { hop_entry(q2, lane, A); load_list(q2.i, h2); }
     if (r + S < r1) hop_bmeta<true>(q1, lane, A);
     const int i = q0.i;
     const int lo = q0.lo;
     const int64_t off = h0.off;
     const int c = (int)(h0.end - h0.off);
     const int nt = h0.rw & kWordsMask;
-    const bool have_list = (h0.rw & kNoWordList) == 0 && nt <= kWordCap;
+    const bool have_list = (h0.rw & kNoWordList) == 0 && nt <= kCap;
     uint32_t myw[PL];
     for (int t = lane; t < c; t += G) vals[t] = 0.0;
     mbe[lane] = q0.be;

@@ -101,10 +101,10 @@ struct RowArgs {
   int upper;
   int64_t bnnz;          // nnz of B (selects 32- or 64-bit gather offsets)
   const int4* bmeta;     // per B row: {start, end, min column, max column}
-  uint2* wlist;          // per output row: up to 32 sorted (word, mask) pairs from the symbolic pass
+  uint2* wlist;          // per output row: up to kWordListCap sorted (word, mask) pairs from the symbolic pass
   int32_t lds_budget;    // LDS bytes a block of the wave-level bins may take
 };
-constexpr int kWordListCap = 32;
+constexpr int kWordListCap = 64;  // word-list slots per output row (stride of RowArgs::wlist)
 
 // Heavy-row bin: column windows of wc_sym / wc_num columns, two row-claim
 // counters (work[0] symbolic, work[1] numeric), the B-sortedness flag of
