@@ -315,8 +315,8 @@ HeavyArgs Engine::heavy_args(Workspace& W, const Operands& O, const Config& cfg)
   HeavyArgs h{};
   // dynamic LDS: the block's LDS less the heavy kernel's static staging (~21 KB)
   const int64_t dyn = lds_max_ - 24 * 1024;
-  h.wc_sym = heavy_window_cols(false, dyn, cfg.heavy_cap);
-  h.wc_num = heavy_window_cols(true, dyn, cfg.heavy_cap);
+  h.wc_sym = heavy_window_cols(dyn, cfg.heavy_cap);
+  heavy_num_shape(dyn, cfg.heavy_cap, &h.wc_num, &h.vcap, &h.dcols);
   h.work = as<int32_t>(W.ctl.p);
   h.bflag = h.work + 2;
   h.cur = as<int32_t>(ensure(W.cur, sizeof(int32_t) * (size_t)std::max<int64_t>(O.annz, 1)));

@@ -110,13 +110,14 @@ constexpr int kWordListCap = 64;  // word-list slots per output row (stride of R
 // counters (work[0] symbolic, work[1] numeric), the B-sortedness flag of
 // launch_rowstats and a cursor per A entry (nnz(A) int32).
 struct HeavyArgs {
-  int32_t wc_sym, wc_num;
+  int32_t wc_sym, wc_num, vcap, dcols;  // window columns (symbolic / numeric sparse), slots, dense width
   int32_t* work;
   const int32_t* bflag;
   int32_t* cur;
 };
-int32_t heavy_window_cols(bool vals, int64_t lds_bytes, int32_t cap);
-int64_t heavy_lds_bytes(bool vals, int32_t wc);
+int32_t heavy_window_cols(int64_t lds_bytes, int32_t cap);
+void heavy_num_shape(int64_t lds_bytes, int32_t cap, int32_t* wcols, int32_t* vcap, int32_t* dcols);
+int64_t heavy_lds_bytes(bool vals, int32_t wc, int32_t vcap, int32_t dcols);
 
 // Symbolic: rowcnt[i] = distinct output columns of row i.
 // words_max: the bin's largest LDS statistic (span words or hash keys).

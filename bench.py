@@ -300,15 +300,19 @@ def bench_serve(args) -> dict:
     print(f"# serve pool: {pool_n} matrices, n " + ",".join(str(m.n) for m, _, _ in jobs), file=sys.stderr,
           flush=True)
 
+    # one job sequence and one set of unit inter-arrival gaps for every run,
+    # so the burst and the Poisson runs serve the same mix
+    picks = rng.integers(0, pool_n, args.serve_jobs)
+    gaps = rng.exponential(1.0, args.serve_jobs)
+
     def run(qps: float | None, count: int) -> dict:
         with ap.Server(0, slots=args.slots, hbm_budget_bytes=200 * 10 ** 9) as srv:
-            picks = rng.integers(0, pool_n, count)
             ids = []
             t0 = time.perf_counter()
             t_next = t0
-            for k in picks:
+            for n_k, k in enumerate(picks[:count]):
                 if qps:
-                    t_next += rng.exponential(1.0 / qps)
+                    t_next += gaps[n_k] / qps
                     dt = t_next - time.perf_counter()
                     if dt > 0:
                         time.sleep(dt)
