@@ -249,7 +249,7 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
     launch_fill_i32(cnt, n, 0, s);
     launch_colcount(nnz, A.col, cnt, s);
     launch_scan(cnt, n, off64, tp, tmp, s);
-    launch_transpose_gather(n, A.rowptr, A.col, A.val, tc, tv, asym, s);
+    launch_transpose_gather(n, A.rowptr, A.col, A.val, tp, tc, tv, asym, s);
     launch_i64_to_i32(off64, n, cnt, s);
     launch_transpose_scatter(n, A.rowptr, A.col, A.val, cnt, tc, tv, asym, s);
     launch_sort_rows(n, tp, tc, tv, slist, asym, s);

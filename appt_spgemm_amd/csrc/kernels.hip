@@ -131,6 +131,7 @@ __global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, co
     const int s = ap[k], e = ap[k + 1];
     bool miss = false;
     for (int p = s + lane; p < e; p += G) {
+      if ((p - s) % (16 * G) == lane && __hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
       const int i = ac[p];
       if (p > s && ac[p - 1] >= i) miss = true;   // descent or duplicate
       int lo = ap[i], hi = ap[i + 1];
@@ -150,9 +151,23 @@ __global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, co
     if (miss) *asym = 1;   // every writer stores 1
   }
 }
-void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, int32_t* bc,
-                             double* bv, int32_t* asym, hipStream_t s) {
+// A symmetric pattern has equal row and column counts: any row whose
+// transpose row pointer differs from A's marks the pattern asymmetric before
+// the gather runs.
+__global__ void k_pattern_counts(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ tp,
+                                 int32_t* __restrict__ asym) {
+  bool diff = false;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += (int64_t)gridDim.x * blockDim.x)
+    diff |= ap[k] != tp[k];
+  if (__ballot(diff) != 0ull && threadIdx.x % kWave == 0) *asym = 1;  // every writer stores 1
+}
+
+void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, const int32_t* tp,
+                             int32_t* bc, double* bv, int32_t* asym, hipStream_t s) {
   if (n <= 0) return;
+  int64_t b0 = ((int64_t)n + 256) / 256;
+  if (b0 > 4096) b0 = 4096;
+  hipLaunchKernelGGL(k_pattern_counts, dim3((unsigned)b0), dim3(256), 0, s, n, ap, tp, asym);
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_transpose_gather, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, av, bc, bv, asym);
@@ -899,28 +914,31 @@ __device__ __forceinline__ void lds_sort_pairs(int32_t* keys, V* vals, int len, 
 // ---------------------------------------------------------------------------
 // Sorted transpose: the atomic-cursor scatter leaves each B row in arbitrary
 // order; every row is then sorted by its column (the source row index), the
-// values moving with their keys.  Rows of at most 64 entries: one wave, in
-// registers; up to kSortLds entries: one block in LDS; longer rows: one
+// values moving with their keys.  Rows of at most 64 entries: a group of 16,
+// 32 or 64 lanes, in registers; up to kSortLds entries: one block in LDS; longer rows: one
 // block, in place in global memory.
 // ---------------------------------------------------------------------------
 constexpr int kSortLds = 4096;
 
+// Rows of (W/2, W] entries (2..16 for W = 16), one group of W lanes each.
+template <int W>
 __global__ void k_sort_rows_wave(int32_t n, const int32_t* __restrict__ bp, int32_t* __restrict__ bc,
                                  double* __restrict__ bv, const int32_t* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int lane = threadIdx.x % kWave;
-  const int64_t nw = (int64_t)gridDim.x * blockDim.x / kWave;
-  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave; r < n; r += nw) {
+  constexpr int kMin = W == 16 ? 2 : W / 2 + 1;
+  const int lane = threadIdx.x % W;
+  const int64_t ng = (int64_t)gridDim.x * blockDim.x / W;
+  for (int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / W; r < n; r += ng) {
     const int s = bp[r], len = bp[r + 1] - s;
-    if (len < 2 || len > kWave) continue;  // uniform across the wave
+    if (len < kMin || len > W) continue;  // uniform across the group
     int key = lane < len ? bc[s + lane] : INT32_MAX;
     double v = lane < len ? bv[s + lane] : 0.0;
 #pragma unroll
-    for (int k = 2; k <= kWave; k <<= 1) {
+    for (int k = 2; k <= W; k <<= 1) {
 #pragma unroll
       for (int j = k >> 1; j > 0; j >>= 1) {
-        const int pk = __shfl_xor(key, j, kWave);
-        const double pv = __shfl_xor(v, j, kWave);
+        const int pk = __shfl_xor(key, j, W);
+        const double pv = __shfl_xor(v, j, W);
         const bool up = (lane & k) == 0, lower = (lane & j) == 0;
         // lower lane of an ascending pair keeps the minimum, and so on
         if (lower == up ? pk < key : pk > key) {
@@ -996,9 +1014,11 @@ int64_t sort_rows_scratch_elems(int32_t n) { return (int64_t)n + 2; }
 void launch_sort_rows(int32_t n, const int32_t* bp, int32_t* bc, double* bv, int32_t* scratch, const int32_t* gate,
                       hipStream_t s) {
   if (n <= 0) return;
-  int64_t blocks = ((int64_t)n * kWave + 255) / 256;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_sort_rows_wave, dim3((unsigned)blocks), dim3(256), 0, s, n, bp, bc, bv, gate);
+  hipLaunchKernelGGL(k_sort_rows_wave<16>, dim3((unsigned)blocks), dim3(256), 0, s, n, bp, bc, bv, gate);
+  hipLaunchKernelGGL(k_sort_rows_wave<32>, dim3((unsigned)blocks), dim3(256), 0, s, n, bp, bc, bv, gate);
+  hipLaunchKernelGGL(k_sort_rows_wave<64>, dim3((unsigned)blocks), dim3(256), 0, s, n, bp, bc, bv, gate);
   int32_t* count = scratch + n;
   launch_fill_i32(count, 2, 0, s);
   int64_t b1 = ((int64_t)n + 255) / 256;

@@ -62,9 +62,10 @@ void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const u
                      const int32_t* col, int4* meta, int32_t* flag, hipStream_t s);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
 // Symmetric-pattern transpose into (bc, bv) laid out like A; sets *asym
-// (zero beforehand) when A is not strictly sorted with a symmetric pattern.
-void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, int32_t* bc,
-                             double* bv, int32_t* asym, hipStream_t s);
+// (zero beforehand) when A is not strictly sorted with a symmetric pattern
+// (tp: the transpose's row pointers from the column counts).
+void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, const int32_t* tp,
+                             int32_t* bc, double* bv, int32_t* asym, hipStream_t s);
 // Atomic-cursor scatter; gate (optional): runs only when *gate != 0.
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
                               int32_t* cursor, int32_t* bc, double* bv, const int32_t* gate, hipStream_t s);
