@@ -115,6 +115,11 @@ appt_status appt_spgemm_symbolic_rowptr(appt_engine* eng, const appt_csr_view* A
   return guard([&] { eng->impl->symbolic_rowptr(*A, op, rowptr); });
 }
 
+appt_status appt_csr_eliminate_zeros(appt_engine* eng, appt_csr_result* C) {
+  if (!eng || !C) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->eliminate_zeros(C); });
+}
+
 appt_status appt_csr_release(appt_engine* eng, appt_csr_result* C) {
   if (!eng || !C) return APPT_ERR_INVALID;
   return guard([&] { eng->impl->release(C); });

@@ -342,6 +342,7 @@ static RowArgs row_args(const Workspace& W, const Operands& O, const int32_t* ro
   ra.bmeta = as<const int4>(W.bmeta.p);
   ra.wlist = as<uint2>(W.wlist.p);
   ra.lds_budget = bp.lds_budget;
+  ra.ncols = O.n;
   return ra;
 }
 
@@ -545,6 +546,43 @@ void Engine::release(appt_csr_result* C) {
   pool_.put(R->val);
   delete R;
   std::memset(C, 0, sizeof(*C));
+}
+
+void Engine::eliminate_zeros(appt_csr_result* C) {
+  if (!C || !C->_priv) throw Error(APPT_ERR_INVALID, "not an engine-owned result");
+  APPT_HIP(hipSetDevice(device_));
+  hipStream_t s = stream_;
+  const int32_t n = C->n;
+  const size_t n1 = (size_t)n + 1;
+  Workspace& W = ws_;
+  int32_t* cnt = as<int32_t>(ensure(W.rowcnt, sizeof(int32_t) * n1));
+  int64_t* off = as<int64_t>(ensure(W.rowoff, sizeof(int64_t) * n1));
+  int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
+  launch_count_nonzero(n, C->rowptr, C->rowptr_is_64 != 0, C->val, cnt, s);
+  launch_scan(cnt, n, off, nullptr, tmp, s);
+  int64_t nnz = 0;
+  APPT_HIP(hipMemcpyAsync(&hs_->total, off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  APPT_HIP(hipStreamSynchronize(s));
+  nnz = hs_->total;
+  if (nnz == C->nnz) return;  // nothing to drop
+  const bool rp64 = nnz > INT32_MAX;
+  void* rp = pool_.get((rp64 ? sizeof(int64_t) : sizeof(int32_t)) * n1);
+  int32_t* col = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+  double* val = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
+  launch_compact_nonzero(n, C->rowptr, C->rowptr_is_64 != 0, C->col, C->val, off, col, val, s);
+  if (rp64) launch_copy_i64(off, (int64_t)n + 1, as<int64_t>(rp), s);
+  else launch_i64_to_i32(off, (int64_t)n + 1, as<int32_t>(rp), s);
+  APPT_HIP(hipGetLastError());
+  APPT_HIP(hipStreamSynchronize(s));
+  release(C);
+  Result* R = new Result{rp, col, val};
+  C->n = n;
+  C->nnz = nnz;
+  C->rowptr_is_64 = rp64 ? 1 : 0;
+  C->rowptr = rp;
+  C->col = col;
+  C->val = val;
+  C->_priv = R;
 }
 
 void Engine::run_timed(const char* task, const appt_csr_view& A, int reps, appt_eng_result* out) {

@@ -128,6 +128,9 @@ class Engine {
   // Exact row pointers of op(A) (symbolic phase only), copied to the host.
   void symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr_host);
   void release(appt_csr_result* C);
+  // Drop entries whose value is exactly zero (numeric semantics); C must be
+  // an engine-owned result.
+  void eliminate_zeros(appt_csr_result* C);
   void run_timed(const char* task, const appt_csr_view& A, int reps, appt_eng_result* out);
   void descriptors(const appt_csr_view& A, appt_op op, appt_descriptors* out);
 

@@ -327,20 +327,35 @@ __device__ __forceinline__ int group_index(const BinParams& P, int64_t f, int64_
        : f <= (int64_t)P.glen_max[2] * nA ? 2 : 3;
 }
 
+// Register-sort bin of a row with f products over nA A entries: slots
+// N = np2(max(f, nA, 16)) <= kSortMax, or -1.
+__device__ __forceinline__ int sort_bin_of(int64_t f, int64_t nA) {
+  const int64_t m = f > nA ? f : nA;
+  if (m > kSortMax) return -1;
+  int b = kBinSort0;
+  while ((int64_t)(16 << (b - kBinSort0)) < m) b++;
+  return b;
+}
+
 // Symbolic bin from products f, span words and mean B-row length L.  *stat
 // receives the LDS sizing statistic: span words (dense) or key bound (hash).
+// Rows whose span does not fit a dense window go to a register-sort bin
+// when they have at most kSortMax products, else to the word hash.
 __device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t words, int64_t nA, int64_t* stat) {
   *stat = 0;
   if (f <= 0 || words <= 0) return 0;
   const int gidx = group_index(P, f, nA);
   const int64_t keys = f < words ? f : words;
   const int64_t hash_bytes = np2_hd((keys * 4) / 3 + 1) * 8;
+  const int sb = sort_bin_of(f, nA);
   for (int c = 0; c < P.ncls; c++)
     if (f <= P.cls_max[c]) {
       if (sym_window_bytes(8 << gidx, words) <= P.cls_bytes[c]) { *stat = words; return 1 + (c * 2 + 0) * 4 + gidx; }
+      if (sb >= 0) return sb;
       if (hash_bytes <= P.cls_bytes[c]) { *stat = keys; return 1 + (c * 2 + 1) * 4 + gidx; }
       break;
     }
+  if (sb >= 0) return sb;
   if (words * 4 <= P.wg_bytes) { *stat = words; return kBinWgBitmap; }
   if (hash_bytes <= P.wg_bytes) { *stat = keys; return kBinWgHash; }
   *stat = words;
@@ -361,7 +376,7 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
   const int gidx = group_index(P, f, nA);
   const int64_t lim = (int64_t)P.dense_ratio * w;
   const bool dense_ok = words <= (lim > 64 ? lim : 64);
-  const int64_t dense_bytes = ((words * 8 + c * 12) + 15) & ~(int64_t)15;
+  const int sb = sort_bin_of(f, nA);
   for (int cl = 0; cl < P.ncls; cl++)
     if (c <= P.cls_max[cl]) {
       // listed: the row's word-list length, or -1 without a list
@@ -370,9 +385,11 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
         return kBinList0 + cl * 4 + gidx;
       }
       if (num_window_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) { *stat = words; return 1 + (cl * 2 + 0) * 4 + gidx; }
+      if (sb >= 0) return sb;
       if (whash_bytes_hd(w, c) <= P.cls_bytes[cl]) { *stat = w; return 1 + (cl * 2 + 1) * 4 + gidx; }
       break;
     }
+  if (sb >= 0) return sb;
   // workgroup bins: bound each term separately so bin maxima still fit
   if (dense_ok && words * 8 <= P.wg_bytes / 2 && c * 12 <= P.wg_bytes / 2) { *stat = words; return kBinWgBitmap; }
   if (whash_bytes_hd(c, c) <= P.wg_bytes) { *stat = w; return kBinWgHash; }
@@ -1215,6 +1232,260 @@ __global__ void __launch_bounds__(256) k_numeric_lds(RowArgs A, int32_t words_ma
 }
 
 // ---------------------------------------------------------------------------
+// Register-sort bins: rows of at most N = G*E products whose output columns
+// are too scattered for a window (random sparsity).  One group of G lanes
+// per row:
+//   1. the row's A entries go to LDS with the exclusive scan of their B-row
+//      lengths (epos), so product t (in A-row storage order, then B-row
+//      order) belongs to the last entry r with epos[r] <= t;
+//   2. lane l forms products t = e*G + l (coalesced B gathers), keyed
+//      (column << 9 | t) -- or the column alone in the symbolic pass -- and
+//      the numeric pass parks each product value in LDS at index t;
+//   3. a bitonic network sorts the G*E keys held E per lane in registers
+//      (lane-local compare-exchanges, then xor shuffles across lanes);
+//   4. distinct columns are the run heads of the sorted keys; a numeric head
+//      sums its run in t order starting from 0.0, which is exactly the
+//      A-row-order accumulation of the other bins, and writes the output
+//      entry at its rank among the heads.
+// ---------------------------------------------------------------------------
+template <typename K>
+__device__ __forceinline__ K shfl_xor_key(K v, int m, int width) {
+  if constexpr (sizeof(K) == 8) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m, width), hi = __shfl_xor((uint32_t)(v >> 32), m, width);
+    return ((K)hi << 32) | lo;
+  } else {
+    return (K)__shfl_xor((uint32_t)v, m, width);
+  }
+}
+
+// Ascending bitonic sort of the G*E keys of a group; lane l holds positions
+// l*E .. l*E + E - 1 on return.
+template <int G, int E, typename K>
+__device__ __forceinline__ void reg_bitonic(K (&k)[E], int lane) {
+#pragma unroll
+  for (int size = 2; size <= G * E; size <<= 1) {
+#pragma unroll
+    for (int j = size >> 1; j > 0; j >>= 1) {
+      if (j >= E) {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          const int p = lane * E + e;
+          const K o = shfl_xor_key<K>(k[e], j / E, G);
+          // the lower position of an ascending pair keeps the minimum
+          const bool keep_min = ((p & size) == 0) == ((p & j) == 0);
+          k[e] = keep_min ? (o < k[e] ? o : k[e]) : (o > k[e] ? o : k[e]);
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < E; e++) {
+          if ((e & j) == 0) {
+            const int p = lane * E + e;
+            const K a = k[e], b = k[e | j];
+            const bool asc = (p & size) == 0;
+            const K mn = a < b ? a : b, mx = a < b ? b : a;
+            k[e] = asc ? mn : mx;
+            k[e | j] = asc ? mx : mn;
+          }
+        }
+      }
+    }
+  }
+}
+
+__host__ __device__ __forceinline__ constexpr int64_t sort_row_bytes(int N, bool vals) {
+  // epos (N + 1) i32 | eb N i32 [| ea N f64 | val N f64]; the sorted keys
+  // reuse the epos / eb area (8 N bytes suffice for 64-bit keys)
+  return align16((int64_t)(N + 1) * 4) + align16((int64_t)N * 4) + (vals ? (int64_t)N * 16 : 0);
+}
+
+// Step 1: A entries of row i into LDS; returns the row's product count.
+template <int G, bool VALS>
+__device__ __forceinline__ int sort_stage_entries(int i, int lane, const RowArgs& A, int32_t* epos, int32_t* eb,
+                                                  double* ea) {
+  const int s = A.ap[i], nA = A.ap[i + 1] - s;
+  int run = 0;
+  for (int base = 0; base < nA; base += G) {
+    const int r = base + lane;
+    int len = 0, bs = 0;
+    double a = 0.0;
+    if (r < nA) {
+      const int4 bm = A.bmeta[A.ac[s + r]];
+      bs = bm.x;
+      len = bm.y - bm.x;
+      if (VALS) a = A.av[s + r];
+    }
+    int tot;
+    const int ex = gscan_excl<G, int>(len, lane, &tot);
+    if (r < nA) {
+      epos[r] = run + ex;
+      eb[r] = bs - (run + ex);
+      if (VALS) ea[r] = a;
+    }
+    run += tot;
+  }
+  if (lane == 0) epos[nA] = run;
+  gsync<G>();
+  return run;
+}
+
+// Entry of product t: the last r in [0, nA) with epos[r] <= t.
+__device__ __forceinline__ int sort_entry_of(const int32_t* epos, int nA, int t) {
+  int lo = 0, hi = nA;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (epos[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int G, int E>
+__global__ void __launch_bounds__(256) k_sym_sort(RowArgs A, int32_t* __restrict__ rowcnt) {
+  constexpr int N = G * E;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int R = blockDim.x / G;
+  const int gi = threadIdx.x / G, lane = threadIdx.x % G;
+  const int r = xcd_block(blockIdx.x, gridDim.x) * R + gi;
+  if (r >= A.nrows) return;
+  const int i = A.rows[r];
+  unsigned char* base = lds_raw + (size_t)gi * sort_row_bytes(N, false);
+  int32_t* epos = (int32_t*)base;
+  int32_t* eb = (int32_t*)(base + align16((int64_t)(N + 1) * 4));
+  const int nA = A.ap[i + 1] - A.ap[i];
+  const int f = sort_stage_entries<G, false>(i, lane, A, epos, eb, nullptr);
+  uint32_t key[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int t = e * G + lane;
+    key[e] = 0xFFFFFFFFu;
+    if (t < f) key[e] = (uint32_t)A.bc[eb[sort_entry_of(epos, nA, t)] + t];
+  }
+  reg_bitonic<G, E, uint32_t>(key, lane);
+  // run heads: previous position is the own previous slot, or the last slot
+  // of the previous lane
+  const uint32_t before = __shfl_up(key[E - 1], 1, G);
+  int cnt = 0, nw = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const uint32_t prev = e > 0 ? key[e - 1] : (lane > 0 ? before : 0xFFFFFFFFu);
+    const bool real = key[e] != 0xFFFFFFFFu;
+    const bool first = e == 0 && lane == 0;
+    cnt += (real && (first || key[e] != prev)) ? 1 : 0;
+    nw += (real && (first || (key[e] >> 5) != (prev >> 5))) ? 1 : 0;
+  }
+  cnt = gsum<G, int32_t>(cnt, nullptr);
+  nw = gsum<G, int32_t>(nw, nullptr);
+  if (lane == 0) {
+    rowcnt[i] = cnt;
+    A.rowwords[i] = nw | kNoWordList;
+  }
+}
+
+template <int G, int E, bool UPPER, typename K>
+__global__ void __launch_bounds__(256) k_num_sort(RowArgs A, const int64_t* __restrict__ rowoff,
+                                                  int32_t* __restrict__ ccol, double* __restrict__ cval) {
+  constexpr int N = G * E;
+  constexpr K kNone = ~(K)0;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int R = blockDim.x / G;
+  const int gi = threadIdx.x / G, lane = threadIdx.x % G;
+  const int r = xcd_block(blockIdx.x, gridDim.x) * R + gi;
+  if (r >= A.nrows) return;
+  const int i = A.rows[r];
+  unsigned char* base = lds_raw + (size_t)gi * sort_row_bytes(N, true);
+  int32_t* epos = (int32_t*)base;
+  int32_t* eb = (int32_t*)(base + align16((int64_t)(N + 1) * 4));
+  double* ea = (double*)(base + align16((int64_t)(N + 1) * 4) + align16((int64_t)N * 4));
+  double* val = ea + N;
+  K* skey = (K*)base;
+  const int nA = A.ap[i + 1] - A.ap[i];
+  const int f = sort_stage_entries<G, true>(i, lane, A, epos, eb, ea);
+  K key[E];
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int t = e * G + lane;
+    key[e] = kNone;
+    if (t < f) {
+      const int en = sort_entry_of(epos, nA, t);
+      const int q = eb[en] + t;
+      const int j = A.bc[q];
+      // AAtS: values only for j >= i (the mirror fills the rest)
+      val[t] = (!UPPER || j >= i) ? ea[en] * A.bv[q] : 0.0;
+      key[e] = ((K)j << 9) | (K)t;
+    }
+  }
+  reg_bitonic<G, E, K>(key, lane);
+  gsync<G>();  // every lane is done with epos / eb before the keys overwrite them
+#pragma unroll
+  for (int e = 0; e < E; e++) skey[lane * E + e] = key[e];
+  gsync<G>();
+  bool head[E];
+  int cnt = 0;
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    const int p = lane * E + e;
+    head[e] = key[e] != kNone && (p == 0 || (skey[p - 1] >> 9) != (key[e] >> 9));
+    cnt += head[e] ? 1 : 0;
+  }
+  int tot;
+  int64_t out = rowoff[i] + gscan_excl<G, int>(cnt, lane, &tot);
+#pragma unroll
+  for (int e = 0; e < E; e++) {
+    if (!head[e]) continue;
+    const int col = (int)(key[e] >> 9);
+    double acc = 0.0;
+    if (!UPPER || col >= i) {
+      acc = acc + val[(int)(key[e] & 511)];
+      for (int p = lane * E + e + 1; p < f; p++) {
+        const K kq = skey[p];
+        if ((kq >> 9) != key[e] >> 9) break;
+        acc = acc + val[(int)(kq & 511)];
+      }
+    }
+    ccol[out] = col;
+    cval[out] = acc;
+    out++;
+  }
+}
+
+template <int G, int E>
+static void sort_launch_shape(const RowArgs& a, int64_t row_bytes, int* R, unsigned* blocks) {
+  int r = 256 / G;
+  while (r > 1 && (int64_t)r * row_bytes > a.lds_budget) r >>= 1;
+  *R = r;
+  *blocks = (unsigned)((a.nrows + r - 1) / r);
+}
+template <int G, int E>
+static void sym_sort_launch(const RowArgs& a, int32_t* rowcnt, hipStream_t s) {
+  int R; unsigned blocks;
+  sort_launch_shape<G, E>(a, sort_row_bytes(G * E, false), &R, &blocks);
+  const size_t lds = (size_t)R * sort_row_bytes(G * E, false);
+  (void)hipFuncSetAttribute((const void*)k_sym_sort<G, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL((k_sym_sort<G, E>), dim3(blocks), dim3(R * G), lds, s, a, rowcnt);
+}
+template <int G, int E, bool UPPER, typename K>
+static void num_sort_launch_k(const RowArgs& a, const int64_t* rowoff, int32_t* ccol, double* cval, hipStream_t s) {
+  int R; unsigned blocks;
+  sort_launch_shape<G, E>(a, sort_row_bytes(G * E, true), &R, &blocks);
+  const size_t lds = (size_t)R * sort_row_bytes(G * E, true);
+  (void)hipFuncSetAttribute((const void*)k_num_sort<G, E, UPPER, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds);
+  hipLaunchKernelGGL((k_num_sort<G, E, UPPER, K>), dim3(blocks), dim3(R * G), lds, s, a, rowoff, ccol, cval);
+}
+template <int G, int E>
+static void num_sort_launch(const RowArgs& a, const int64_t* rowoff, int32_t* ccol, double* cval, hipStream_t s) {
+  // 32-bit keys hold column << 9 | t while columns stay below 2^23 - 1
+  const bool k64 = a.ncols >= (1 << 23) - 1;
+  if (a.upper) {
+    if (k64) num_sort_launch_k<G, E, true, uint64_t>(a, rowoff, ccol, cval, s);
+    else num_sort_launch_k<G, E, true, uint32_t>(a, rowoff, ccol, cval, s);
+  } else {
+    if (k64) num_sort_launch_k<G, E, false, uint64_t>(a, rowoff, ccol, cval, s);
+    else num_sort_launch_k<G, E, false, uint32_t>(a, rowoff, ccol, cval, s);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Heavy rows: outputs too wide for one workgroup's LDS.  One workgroup per
 // row, rows claimed from a counter (heavy rows differ widely in cost).  The
 // row's output span [lo, hi] is processed in column windows of `wc` columns
@@ -1995,6 +2266,7 @@ static bool bin_dense(int bin) {
 
 int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
   if (bin == kBinHeavy || bin == 0) return 0;
+  if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), false);
   (void)metric_max;
   if (bin == kBinWgBitmap) return (int64_t)words_max * 4;
   if (bin_dense(bin)) return sym_window_bytes(bin_group(bin), words_max);
@@ -2002,6 +2274,7 @@ int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
 }
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max) {
   if (bin == kBinHeavy || bin == 0) return 0;
+  if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), true);
   if (bin == kBinWgBitmap) return dense_row_bytes(words_max, c_max);
   if (bin_is_list(bin)) return num_list_bytes(bin_group(bin), words_max, c_max);
   if (bin_dense(bin)) return num_window_bytes(bin_group(bin), words_max, c_max);
@@ -2169,6 +2442,17 @@ void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* 
     heavy_launch(false, a, h, h.work, rowcnt, nullptr, nullptr, nullptr, s);
     return;
   }
+  if (bin_is_sort(bin)) {
+    switch (sort_bin_slots(bin)) {
+      case 16: sym_sort_launch<16, 1>(a, rowcnt, s); break;
+      case 32: sym_sort_launch<32, 1>(a, rowcnt, s); break;
+      case 64: sym_sort_launch<64, 1>(a, rowcnt, s); break;
+      case 128: sym_sort_launch<64, 2>(a, rowcnt, s); break;
+      case 256: sym_sort_launch<64, 4>(a, rowcnt, s); break;
+      default: sym_sort_launch<64, 8>(a, rowcnt, s); break;
+    }
+    return;
+  }
   if (bin == kBinWgBitmap) { sym_lds<256, true>(a, words_max, rowcnt, s); return; }
   if (bin == kBinWgHash) { sym_lds<256, false>(a, words_max, rowcnt, s); return; }
   const int G = bin_group(bin);
@@ -2191,6 +2475,17 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
   if (a.nrows <= 0 || bin <= 0) return;
   if (bin == kBinHeavy) {
     heavy_launch(true, a, h, h.work + 1, nullptr, rowoff, ccol, cval, s);
+    return;
+  }
+  if (bin_is_sort(bin)) {
+    switch (sort_bin_slots(bin)) {
+      case 16: num_sort_launch<16, 1>(a, rowoff, ccol, cval, s); break;
+      case 32: num_sort_launch<32, 1>(a, rowoff, ccol, cval, s); break;
+      case 64: num_sort_launch<64, 1>(a, rowoff, ccol, cval, s); break;
+      case 128: num_sort_launch<64, 2>(a, rowoff, ccol, cval, s); break;
+      case 256: num_sort_launch<64, 4>(a, rowoff, ccol, cval, s); break;
+      default: num_sort_launch<64, 8>(a, rowoff, ccol, cval, s); break;
+    }
     return;
   }
   if (bin_is_list(bin)) {
@@ -2217,6 +2512,69 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
     case 128: num_lds<64, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     default: break;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Numeric semantics (scipy's eliminate_zeros): entries whose value is exactly
+// zero (structural products that cancelled, and explicit zeros) are dropped.
+// Count per row, scan, order-preserving compaction with 16-lane ballots.
+// ---------------------------------------------------------------------------
+template <typename RP>
+__global__ void k_count_nonzero(int32_t n, const RP* __restrict__ rp, const double* __restrict__ val,
+                                int32_t* __restrict__ cnt) {
+  constexpr int G = 16;
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb) {
+    int c = 0;
+    for (int64_t p = (int64_t)rp[i] + lane; p < (int64_t)rp[i + 1]; p += G) c += val[p] != 0.0 ? 1 : 0;
+    c = gsum<G, int32_t>(c, nullptr);
+    if (lane == 0) cnt[i] = c;
+  }
+}
+template <typename RP>
+__global__ void k_compact_nonzero(int32_t n, const RP* __restrict__ rp, const int32_t* __restrict__ col,
+                                  const double* __restrict__ val, const int64_t* __restrict__ off,
+                                  int32_t* __restrict__ ocol, double* __restrict__ oval) {
+  constexpr int G = 16;
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb) {
+    const int64_t s = rp[i], e = rp[i + 1];
+    int64_t o = off[i];
+    for (int64_t b = s; b < e; b += G) {
+      const int64_t p = b + lane;
+      const bool keep = p < e && val[p] != 0.0;
+      const uint64_t m = group_bits<G>(__ballot(keep));
+      if (keep) {
+        const int64_t q = o + __popcll(m & ((1ull << lane) - 1ull));
+        ocol[q] = col[p];
+        oval[q] = val[p];
+      }
+      o += __popcll(m);
+    }
+  }
+}
+void launch_count_nonzero(int32_t n, const void* rp, bool rp64, const double* val, int32_t* cnt, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  if (rp64)
+    hipLaunchKernelGGL(k_count_nonzero<int64_t>, dim3((unsigned)blocks), dim3(256), 0, s, n, (const int64_t*)rp, val, cnt);
+  else
+    hipLaunchKernelGGL(k_count_nonzero<int32_t>, dim3((unsigned)blocks), dim3(256), 0, s, n, (const int32_t*)rp, val, cnt);
+}
+void launch_compact_nonzero(int32_t n, const void* rp, bool rp64, const int32_t* col, const double* val,
+                            const int64_t* off, int32_t* ocol, double* oval, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  if (rp64)
+    hipLaunchKernelGGL(k_compact_nonzero<int64_t>, dim3((unsigned)blocks), dim3(256), 0, s, n, (const int64_t*)rp, col,
+                       val, off, ocol, oval);
+  else
+    hipLaunchKernelGGL(k_compact_nonzero<int32_t>, dim3((unsigned)blocks), dim3(256), 0, s, n, (const int32_t*)rp, col,
+                       val, off, ocol, oval);
 }
 
 // ---------------------------------------------------------------------------

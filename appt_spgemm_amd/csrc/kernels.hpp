@@ -14,6 +14,10 @@
 //                     pass left a sorted word list, 28 + cls*4 + gidx; the
 //                     LDS holds a byte index per window word and the listed
 //                     words instead of a bitmap and prefix counts
+//   40 .. 45          register sort: rows of at most N = 16 << (bin - 40)
+//                     products (and A entries) whose output is too scattered
+//                     for a window; the products are sorted by column in
+//                     registers (bitonic network over G lanes x E slots)
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -21,8 +25,9 @@
 namespace appt {
 
 constexpr int kNumCls = 3;
-constexpr int kNumBins = 40;
-constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27, kBinList0 = 28;
+constexpr int kNumBins = 46;
+constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27, kBinList0 = 28, kBinSort0 = 40;
+constexpr int kSortMax = 512;   // products (and A entries) of the largest sort bin
 constexpr int kHeavyT = 1024;   // threads of a heavy-row workgroup
 constexpr int kStatFields = 3;  // count, max metric, max words
 constexpr int kStatsMaxBlocks = 4096;
@@ -39,10 +44,12 @@ struct BinParams {
   int32_t lds_budget;             // LDS bytes a block of the wave-level bins may take
 };
 
-inline bool bin_is_list(int bin) { return bin >= kBinList0; }
+inline bool bin_is_sort(int bin) { return bin >= kBinSort0; }
+inline int sort_bin_slots(int bin) { return 16 << (bin - kBinSort0); }
+inline bool bin_is_list(int bin) { return bin >= kBinList0 && bin < kBinSort0; }
 inline int bin_group(int bin) { return bin_is_list(bin) ? 8 << ((bin - kBinList0) % 4) : 8 << ((bin - 1) % 4); }
 inline int bin_cls(int bin) { return bin_is_list(bin) ? (bin - kBinList0) / 4 : (bin - 1) / 8; }
-inline bool bin_is_bitmap(int bin) { return !bin_is_list(bin) && ((bin - 1) / 4) % 2 == 0; }
+inline bool bin_is_bitmap(int bin) { return bin < kBinWgBitmap && ((bin - 1) / 4) % 2 == 0; }
 
 // Symbolic hash table slots for a row with f products (np2l((4f)/3+1) of the
 // reference sizing rule); numeric hash slots for a row with c outputs.
@@ -104,6 +111,7 @@ struct RowArgs {
   const int4* bmeta;     // per B row: {start, end, min column, max column}
   uint2* wlist;          // per output row: up to kWordListCap sorted (word, mask) pairs from the symbolic pass
   int32_t lds_budget;    // LDS bytes a block of the wave-level bins may take
+  int32_t ncols;         // column count of B (bounds every output column index)
 };
 constexpr int kWordListCap = 64;  // word-list slots per output row (stride of RowArgs::wlist)
 
@@ -158,6 +166,12 @@ void launch_batch_local_rowptr(int32_t N, int32_t K, const int64_t* rowoff, cons
 void launch_batch_block_sum(int32_t K, const int32_t* blk_row, const int64_t* rowf, int64_t* out, hipStream_t s);
 void launch_batch_gather(int32_t K, const BatchSrc* src, const int32_t* blk_row, const int64_t* blk_nnz, int32_t* rp,
                          int32_t* col, double* val, hipStream_t s);
+
+// Numeric semantics: per-row counts of non-zero values, then their
+// order-preserving compaction to offsets `off` (int64 exclusive scan).
+void launch_count_nonzero(int32_t n, const void* rp, bool rp64, const double* val, int32_t* cnt, hipStream_t s);
+void launch_compact_nonzero(int32_t n, const void* rp, bool rp64, const int32_t* col, const double* val,
+                            const int64_t* off, int32_t* ocol, double* oval, hipStream_t s);
 
 void set_num_cus(int n);
 void set_lds_per_cu(int64_t b);

@@ -169,6 +169,11 @@ class DeviceResult:
                                        val.ctypes.data_as(C.POINTER(C.c_double))), "csr_download_rows")
         return HostMatrix(r1 - r0, rp, col[:cnt], val[:cnt])
 
+    def eliminate_zeros(self) -> "DeviceResult":
+        """Numeric semantics: drop entries whose value is exactly zero (in place)."""
+        check(lib().appt_csr_eliminate_zeros(self._engine.handle, C.byref(self.res)), "csr_eliminate_zeros")
+        return self
+
     def release(self) -> None:
         if self.res._priv:
             check(lib().appt_csr_release(self._engine.handle, C.byref(self.res)), "csr_release")
@@ -250,12 +255,15 @@ class Engine:
                                                 rp.ctypes.data_as(C.POINTER(C.c_int64))), "symbolic_rowptr")
         return rp
 
-    def multiply(self, m: HostMatrix, op: str = "AA") -> HostMatrix:
-        """Upload, compute, download, release: C = op(m) as a host matrix."""
+    def multiply(self, m: HostMatrix, op: str = "AA", eliminate_zeros: bool = False) -> HostMatrix:
+        """Upload, compute, download, release: C = op(m) as a host matrix
+        (structural semantics; numeric semantics with eliminate_zeros)."""
         A = self.upload(m)
         try:
             R = self.compute(A, op)
             try:
+                if eliminate_zeros:
+                    R.eliminate_zeros()
                 return R.download()
             finally:
                 R.release()

@@ -147,6 +147,12 @@ appt_status appt_spgemm_compute(appt_engine* eng, const appt_csr_view* A, appt_o
                                 appt_csr_result* C);
 appt_status appt_csr_release(appt_engine* eng, appt_csr_result* C);
 
+/* Numeric semantics on request: drop the entries of an engine-owned result
+ * whose value is exactly zero (scipy's eliminate_zeros; the default is the
+ * reference's structural semantics, report/CORRECTNESS_NOTES.md:3-16).  C is
+ * replaced by the compacted result. */
+appt_status appt_csr_eliminate_zeros(appt_engine* eng, appt_csr_result* C);
+
 /* Row panel: rows [row_begin, row_end) of op(A) (op AA or AAT) as a CSR with
  * row_end - row_begin rows and global column indices.  For products whose C
  * does not fit the device at once; no counterpart in the reference, whose

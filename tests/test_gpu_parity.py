@@ -142,6 +142,44 @@ def test_heavy_windows_unsorted(eng_heavy, op):
     assert_product(got, ref(m, op), exact=False)
 
 
+def random_rows(n, lens, col_hi, seed):
+    """Rows of the given lengths with distinct random columns below col_hi."""
+    rng = np.random.default_rng(seed)
+    rp = np.zeros(n + 1, np.int64)
+    rp[1:] = np.cumsum(lens)
+    col = np.concatenate([np.sort(rng.choice(col_hi, size=int(k), replace=False)) for k in lens]).astype(np.int32)
+    return host(n, rp.astype(np.int32), col, rng.uniform(-1, 1, size=len(col)))
+
+
+def scattered_cases():
+    # uniform random rows: outputs too scattered for a window (register-sort bins)
+    yield "er_200k_16", ap.erdos_renyi(200_000, 16.0, 7)
+    yield "er_60k_22", ap.erdos_renyi(60_000, 22.0, 8)   # rows on both sides of the 512-product limit
+    # A entries whose B rows are empty: rows >= n/2 are empty, so about half
+    # of every row's entries contribute no products (nA > products possible)
+    n = 6000
+    lens = np.where(np.arange(n) < n // 2, 12, 0)
+    lens[7] = 300
+    yield "empty_b_rows", random_rows(n, lens, n, 9)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_scattered_rows(eng, op):
+    for name, m in scattered_cases():
+        say("case scattered", name, op)
+        got = eng.multiply(m, op)
+        assert_product(got, ref(m, op), exact=True)
+
+
+def test_scattered_rows_wide_columns(eng):
+    # more than 2^23 columns: 64-bit sort keys
+    m = ap.erdos_renyi(9_000_000, 1.5, 10)
+    for op in ("AA", "AAt"):
+        say("case scattered wide", op)
+        got = eng.multiply(m, op)
+        assert_product(got, ref(m, op), exact=True)
+
+
 def edge_matrices():
     yield "n0", host(0, [0], [], [])
     yield "n1", host(1, [0, 1], [0], [3.0])
@@ -168,6 +206,46 @@ def test_cancellation_structural(eng):
     got = eng.multiply(m, "AAt")
     assert got.rowptr.tolist() == [0, 2, 4] and got.col.tolist() == [0, 1, 0, 1]
     assert got.val.tolist() == [2.0, 0.0, 0.0, 2.0]
+
+
+def drop_zeros(rp, col, val):
+    """Numeric semantics from a structural product: scipy's eliminate_zeros."""
+    import scipy.sparse as sp
+    rp = np.asarray(rp, np.int64)
+    M = sp.csr_matrix((np.asarray(val, np.float64), np.asarray(col), rp), shape=(len(rp) - 1, len(rp) - 1))
+    M.eliminate_zeros()
+    return M.indptr.astype(np.int64), M.indices, M.data
+
+
+def small_int_values(m, seed, lo=-2, hi=3):
+    """Same pattern, values drawn from small integers (products and sums are
+    exact in FP64, so every cancellation is an exact zero in any order)."""
+    v = np.random.default_rng(seed).integers(lo, hi, size=len(m.col)).astype(np.float64)
+    return ap.HostMatrix(m.n, m.rowptr, m.col, v)
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_eliminate_zeros(eng, op):
+    cases = [("cancel", dict(edge_matrices())["cancel"]), ("explicit_zeros", dict(edge_matrices())["explicit_zeros"]),
+             ("all_empty", dict(edge_matrices())["all_empty"]),
+             ("stencil_int", small_int_values(ap.stencil27(12, 0.1, 3), 1, -1, 2)),
+             ("er_int", small_int_values(ap.erdos_renyi(3000, 8.0, 2), 2)),
+             ("rmat_int", small_int_values(ap.rmat(11, 16, seed=4), 3))]
+    for name, m in cases:
+        say("case eliminate_zeros", name, op)
+        rp, col, val, _ = ref(m, op)
+        erp, ecol, evals = drop_zeros(rp, col, val)
+        got = eng.multiply(m, op, eliminate_zeros=True)
+        assert_structure(got.rowptr, got.col, erp, ecol)
+        np.testing.assert_array_equal(got.val, evals)
+        assert not np.any(got.val == 0.0)
+        if name == "stencil_int":
+            assert len(ecol) < len(col)  # the case does cancel
+
+
+def test_eliminate_zeros_cancel_2x2(eng):
+    got = eng.multiply(dict(edge_matrices())["cancel"], "AAt", eliminate_zeros=True)
+    assert got.rowptr.tolist() == [0, 1, 2] and got.col.tolist() == [0, 1] and got.val.tolist() == [2.0, 2.0]
 
 
 def test_deterministic_repeat(eng):
