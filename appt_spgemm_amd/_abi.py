@@ -44,7 +44,8 @@ class Tuning(C.Structure):
     _fields_ = [("f_class_max", C.c_int32 * MAX_CLASSES), ("wg_lds_bytes", C.c_int32),
                 ("lds_budget_bytes", C.c_int32), ("bitmap_words_factor", C.c_int32),
                 ("group_for_len", C.c_int32 * 4), ("heavy_window_cols", C.c_int32),
-                ("reserved", C.c_int32 * 7)]
+                ("sort_span_factor", C.c_int32), ("sort_span_factor_symbolic", C.c_int32),
+                ("reserved", C.c_int32 * 5)]
 
 
 class Options(C.Structure):
@@ -58,6 +59,14 @@ class PhaseTimes(C.Structure):
                 ("numeric_ms", C.c_float), ("mirror_ms", C.c_float),
                 ("total_ms", C.c_float), ("products", C.c_int64), ("nnz_c", C.c_int64),
                 ("n_bins_used", C.c_int32), ("sync_points", C.c_int32)]
+
+
+MAX_BINS = 64
+
+
+class BinReport(C.Structure):
+    _fields_ = [("n_bins", C.c_int32), ("sym_rows", C.c_int32 * MAX_BINS), ("num_rows", C.c_int32 * MAX_BINS),
+                ("sym_metric_max", C.c_int32 * MAX_BINS), ("num_metric_max", C.c_int32 * MAX_BINS)]
 
 
 class Descriptors(C.Structure):
@@ -114,6 +123,8 @@ _SIGNATURES = {
     "appt_engine_destroy": (C.c_int, [C.c_void_p]),
     "appt_engine_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "appt_engine_phase_times": (C.c_int, [C.c_void_p, P(PhaseTimes)]),
+    "appt_engine_bin_report": (C.c_int, [C.c_void_p, P(BinReport)]),
+    "appt_bin_name": (C.c_char_p, [C.c_int32]),
     "appt_engine_trim": (C.c_int, [C.c_void_p]),
     "appt_engine_device_bytes": (C.c_int64, [C.c_void_p]),
     "appt_spgemm_compute": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, P(CsrResult)]),

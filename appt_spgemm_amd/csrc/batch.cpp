@@ -21,6 +21,8 @@ appt_tuning tuning_of(const Config& c) {
   t.bitmap_words_factor = c.num.dense_ratio;
   for (int g = 0; g < 3; g++) t.group_for_len[g] = c.sym.glen_max[g];
   t.heavy_window_cols = c.heavy_cap;
+  t.sort_span_factor = c.num.sort_ratio > 0 ? c.num.sort_ratio : -1;
+  t.sort_span_factor_symbolic = c.sym.sort_ratio > 0 ? c.sym.sort_ratio : -1;
   return t;
 }
 

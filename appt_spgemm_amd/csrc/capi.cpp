@@ -86,6 +86,31 @@ appt_status appt_engine_set_timing(appt_engine* eng, int enable) {
   return APPT_OK;
 }
 
+appt_status appt_engine_bin_report(const appt_engine* eng, appt_bin_report* out) {
+  if (!eng || !out) return APPT_ERR_INVALID;
+  *out = eng->impl->bin_report();
+  return APPT_OK;
+}
+
+const char* appt_bin_name(int32_t bin) {
+  static const char* const names[appt::kNumBins] = {
+      "empty",
+      "window/c0/G8", "window/c0/G16", "window/c0/G32", "window/c0/G64",
+      "hash/c0/G8", "hash/c0/G16", "hash/c0/G32", "hash/c0/G64",
+      "window/c1/G8", "window/c1/G16", "window/c1/G32", "window/c1/G64",
+      "hash/c1/G8", "hash/c1/G16", "hash/c1/G32", "hash/c1/G64",
+      "window/c2/G8", "window/c2/G16", "window/c2/G32", "window/c2/G64",
+      "hash/c2/G8", "hash/c2/G16", "hash/c2/G32", "hash/c2/G64",
+      "wg-window", "wg-hash", "heavy",
+      "list/c0/G8", "list/c0/G16", "list/c0/G32", "list/c0/G64",
+      "list/c1/G8", "list/c1/G16", "list/c1/G32", "list/c1/G64",
+      "list/c2/G8", "list/c2/G16", "list/c2/G32", "list/c2/G64",
+      "sort/N16", "sort/N32", "sort/N64", "sort/N128", "sort/N256", "sort/N512", "sort/N1024", "sort/N2048"};
+  static_assert(appt::kNumBins == 48, "bin names cover every bin id");
+  if (bin < 0 || bin >= appt::kNumBins) return nullptr;
+  return names[bin];
+}
+
 appt_status appt_engine_phase_times(const appt_engine* eng, appt_phase_times* out) {
   if (!eng || !out) return APPT_ERR_INVALID;
   *out = eng->impl->times();

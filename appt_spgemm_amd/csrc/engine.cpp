@@ -90,6 +90,11 @@ void Pool::trim_locked() {
 // ---------------------------------------------------------------------------
 // Engine
 // ---------------------------------------------------------------------------
+// Scattered rows: the register sort beats a window once the span has this
+// many times more words than the row has products (symbolic: by span alone,
+// so only far sparser spans qualify).
+constexpr int32_t kDefaultSortRatio = 2, kDefaultSymSortRatio = 16;
+
 static BinParams default_params() {
   BinParams P{};
   P.ncls = 3;
@@ -98,6 +103,7 @@ static BinParams default_params() {
   P.glen_max[0] = 8; P.glen_max[1] = 16; P.glen_max[2] = 32;
   P.dense_ratio = 4;
   P.lds_budget = 64 * 1024;
+  P.sort_ratio = kDefaultSortRatio;
   return P;
 }
 
@@ -105,6 +111,7 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
   Config c;
   c.sym = default_params();
   c.num = default_params();
+  c.sym.sort_ratio = kDefaultSymSortRatio;
   int64_t group_budget = 64 * 1024;
   // workgroup-per-row bins may use the device's LDS per block minus static LDS
   int32_t wg = (int32_t)(lds_max - 8192);
@@ -121,6 +128,9 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
       if (t->group_for_len[g] > 0) c.sym.glen_max[g] = c.num.glen_max[g] = t->group_for_len[g];
     if (t->wg_lds_bytes > 0) wg = std::min<int32_t>(wg, t->wg_lds_bytes);
     if (t->heavy_window_cols > 0) c.heavy_cap = t->heavy_window_cols;
+    if (t->sort_span_factor != 0) c.num.sort_ratio = t->sort_span_factor > 0 ? t->sort_span_factor : 0;
+    if (t->sort_span_factor_symbolic != 0)
+      c.sym.sort_ratio = t->sort_span_factor_symbolic > 0 ? t->sort_span_factor_symbolic : 0;
   }
   if (group_budget > lds_max) group_budget = lds_max;
   c.sym.lds_budget = c.num.lds_budget = (int32_t)group_budget;
@@ -521,6 +531,14 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   C->val = cval;
   C->_priv = R;
 
+  bins_ = appt_bin_report{};
+  bins_.n_bins = kNumBins;
+  for (int b = 0; b < kNumBins; b++) {
+    bins_.sym_rows[b] = P.sym[b * kStatFields];
+    bins_.num_rows[b] = P.num[b * kStatFields];
+    bins_.sym_metric_max[b] = P.sym[b * kStatFields + 1];
+    bins_.num_metric_max[b] = P.num[b * kStatFields + 1];
+  }
   times_.products = P.products;
   times_.nnz_c = P.nnz;
   times_.n_bins_used = P.bins_used;

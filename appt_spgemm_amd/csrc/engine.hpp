@@ -139,6 +139,7 @@ class Engine {
   Pool& pool() { return pool_; }
   void set_timing(bool t) { timing_ = t; }
   const appt_phase_times& times() const { return times_; }
+  const appt_bin_report& bin_report() const { return bins_; }
   const Config& config() const { return cfg_; }
   void set_config(const Config& c) { cfg_ = c; }
   int64_t lds_max() const { return lds_max_; }
@@ -188,6 +189,7 @@ class Engine {
   bool timing_ = false;
   hipEvent_t ev_[8] = {};
   appt_phase_times times_ = {};
+  appt_bin_report bins_ = {};
   int64_t lds_max_ = 65536;
   Admission admit_;
 };

@@ -348,14 +348,16 @@ __device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t
   const int64_t keys = f < words ? f : words;
   const int64_t hash_bytes = np2_hd((keys * 4) / 3 + 1) * 8;
   const int sb = sort_bin_of(f, nA);
+  // sort bins: the statistic is the A-row length (sizes the entry arrays)
+  if (sb >= 0 && P.sort_ratio > 0 && words > (int64_t)P.sort_ratio * f) { *stat = nA; return sb; }
   for (int c = 0; c < P.ncls; c++)
     if (f <= P.cls_max[c]) {
       if (sym_window_bytes(8 << gidx, words) <= P.cls_bytes[c]) { *stat = words; return 1 + (c * 2 + 0) * 4 + gidx; }
-      if (sb >= 0) return sb;
+      if (sb >= 0) { *stat = nA; return sb; }
       if (hash_bytes <= P.cls_bytes[c]) { *stat = keys; return 1 + (c * 2 + 1) * 4 + gidx; }
       break;
     }
-  if (sb >= 0) return sb;
+  if (sb >= 0) { *stat = nA; return sb; }
   if (words * 4 <= P.wg_bytes) { *stat = words; return kBinWgBitmap; }
   if (hash_bytes <= P.wg_bytes) { *stat = keys; return kBinWgHash; }
   *stat = words;
@@ -377,6 +379,9 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
   const int64_t lim = (int64_t)P.dense_ratio * w;
   const bool dense_ok = words <= (lim > 64 ? lim : 64);
   const int sb = sort_bin_of(f, nA);
+  // scattered: under 4 columns per 3 distinct words (clustered rows, such as
+  // stencils, keep their windows however wide the span)
+  if (sb >= 0 && P.sort_ratio > 0 && words > (int64_t)P.sort_ratio * f && 4 * w > 3 * c) { *stat = nA; return sb; }
   for (int cl = 0; cl < P.ncls; cl++)
     if (c <= P.cls_max[cl]) {
       // listed: the row's word-list length, or -1 without a list
@@ -385,11 +390,11 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
         return kBinList0 + cl * 4 + gidx;
       }
       if (num_window_bytes(8 << gidx, words, c) <= P.cls_bytes[cl]) { *stat = words; return 1 + (cl * 2 + 0) * 4 + gidx; }
-      if (sb >= 0) return sb;
+      if (sb >= 0) { *stat = nA; return sb; }
       if (whash_bytes_hd(w, c) <= P.cls_bytes[cl]) { *stat = w; return 1 + (cl * 2 + 1) * 4 + gidx; }
       break;
     }
-  if (sb >= 0) return sb;
+  if (sb >= 0) { *stat = nA; return sb; }
   // workgroup bins: bound each term separately so bin maxima still fit
   if (dense_ok && words * 8 <= P.wg_bytes / 2 && c * 12 <= P.wg_bytes / 2) { *stat = words; return kBinWgBitmap; }
   if (whash_bytes_hd(c, c) <= P.wg_bytes) { *stat = w; return kBinWgHash; }
@@ -1239,8 +1244,9 @@ __global__ void __launch_bounds__(256) k_numeric_lds(RowArgs A, int32_t words_ma
 //      lengths (epos), so product t (in A-row storage order, then B-row
 //      order) belongs to the last entry r with epos[r] <= t;
 //   2. lane l forms products t = e*G + l (coalesced B gathers), keyed
-//      (column << 9 | t) -- or the column alone in the symbolic pass -- and
-//      the numeric pass parks each product value in LDS at index t;
+//      (column << S | t) with S = max(9, log2 N) -- or the column alone in
+//      the symbolic pass -- and the numeric pass parks each product value in
+//      LDS at index t;
 //   3. a bitonic network sorts the G*E keys held E per lane in registers
 //      (lane-local compare-exchanges, then xor shuffles across lanes);
 //   4. distinct columns are the run heads of the sorted keys; a numeric head
@@ -1292,34 +1298,36 @@ __device__ __forceinline__ void reg_bitonic(K (&k)[E], int lane) {
   }
 }
 
-__host__ __device__ __forceinline__ constexpr int64_t sort_row_bytes(int N, bool vals) {
-  // epos (N + 1) i32 | eb N i32 [| ea N f64 | val N f64]; the sorted keys
-  // reuse the epos / eb area (8 N bytes suffice for 64-bit keys)
-  return align16((int64_t)(N + 1) * 4) + align16((int64_t)N * 4) + (vals ? (int64_t)N * 16 : 0);
+// LDS of one row of a sort bin with N slots, A rows of at most `na`
+// entries and keys of `kb` bytes: the entry area epos (na + 1) i32 | eb na
+// i32, which the sorted keys (N x kb) overwrite, then (numeric) the product
+// values val N f64.
+__host__ __device__ __forceinline__ int64_t sort_entry_bytes(int64_t na) {
+  return align16((na + 1) * 4) + align16(na * 4);
+}
+__host__ __device__ __forceinline__ int64_t sort_row_bytes(int N, int64_t na, int kb, bool vals) {
+  const int64_t e = sort_entry_bytes(na), k = align16((int64_t)N * kb);
+  return (e > k ? e : k) + (vals ? (int64_t)N * 8 : 0);
 }
 
 // Step 1: A entries of row i into LDS; returns the row's product count.
-template <int G, bool VALS>
-__device__ __forceinline__ int sort_stage_entries(int i, int lane, const RowArgs& A, int32_t* epos, int32_t* eb,
-                                                  double* ea) {
+template <int G>
+__device__ __forceinline__ int sort_stage_entries(int i, int lane, const RowArgs& A, int32_t* epos, int32_t* eb) {
   const int s = A.ap[i], nA = A.ap[i + 1] - s;
   int run = 0;
   for (int base = 0; base < nA; base += G) {
     const int r = base + lane;
     int len = 0, bs = 0;
-    double a = 0.0;
     if (r < nA) {
       const int4 bm = A.bmeta[A.ac[s + r]];
       bs = bm.x;
       len = bm.y - bm.x;
-      if (VALS) a = A.av[s + r];
     }
     int tot;
     const int ex = gscan_excl<G, int>(len, lane, &tot);
     if (r < nA) {
       epos[r] = run + ex;
       eb[r] = bs - (run + ex);
-      if (VALS) ea[r] = a;
     }
     run += tot;
   }
@@ -1340,7 +1348,7 @@ __device__ __forceinline__ int sort_entry_of(const int32_t* epos, int nA, int t)
 }
 
 template <int G, int E>
-__global__ void __launch_bounds__(256) k_sym_sort(RowArgs A, int32_t* __restrict__ rowcnt) {
+__global__ void __launch_bounds__(256) k_sym_sort(RowArgs A, int32_t na_max, int32_t* __restrict__ rowcnt) {
   constexpr int N = G * E;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int R = blockDim.x / G;
@@ -1348,11 +1356,11 @@ __global__ void __launch_bounds__(256) k_sym_sort(RowArgs A, int32_t* __restrict
   const int r = xcd_block(blockIdx.x, gridDim.x) * R + gi;
   if (r >= A.nrows) return;
   const int i = A.rows[r];
-  unsigned char* base = lds_raw + (size_t)gi * sort_row_bytes(N, false);
+  unsigned char* base = lds_raw + (size_t)gi * sort_row_bytes(N, na_max, 4, false);
   int32_t* epos = (int32_t*)base;
-  int32_t* eb = (int32_t*)(base + align16((int64_t)(N + 1) * 4));
+  int32_t* eb = (int32_t*)(base + align16((int64_t)(na_max + 1) * 4));
   const int nA = A.ap[i + 1] - A.ap[i];
-  const int f = sort_stage_entries<G, false>(i, lane, A, epos, eb, nullptr);
+  const int f = sort_stage_entries<G>(i, lane, A, epos, eb);
   uint32_t key[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -1361,45 +1369,71 @@ __global__ void __launch_bounds__(256) k_sym_sort(RowArgs A, int32_t* __restrict
     if (t < f) key[e] = (uint32_t)A.bc[eb[sort_entry_of(epos, nA, t)] + t];
   }
   reg_bitonic<G, E, uint32_t>(key, lane);
-  // run heads: previous position is the own previous slot, or the last slot
-  // of the previous lane
-  const uint32_t before = __shfl_up(key[E - 1], 1, G);
+  gsync<G>();  // every lane is done with epos / eb before the keys overwrite them
+  uint32_t* skey = (uint32_t*)epos;
+#pragma unroll
+  for (int e = 0; e < E; e++) skey[lane * E + e] = key[e];
+  gsync<G>();
+  // run heads of columns and of 32-column words
   int cnt = 0, nw = 0;
+  bool whead[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
-    const uint32_t prev = e > 0 ? key[e - 1] : (lane > 0 ? before : 0xFFFFFFFFu);
+    const int p = lane * E + e;
+    const uint32_t prev = p > 0 ? skey[p - 1] : 0xFFFFFFFFu;
     const bool real = key[e] != 0xFFFFFFFFu;
-    const bool first = e == 0 && lane == 0;
-    cnt += (real && (first || key[e] != prev)) ? 1 : 0;
-    nw += (real && (first || (key[e] >> 5) != (prev >> 5))) ? 1 : 0;
+    cnt += (real && (p == 0 || key[e] != prev)) ? 1 : 0;
+    whead[e] = real && (p == 0 || (key[e] >> 5) != (prev >> 5));
+    nw += whead[e] ? 1 : 0;
   }
   cnt = gsum<G, int32_t>(cnt, nullptr);
-  nw = gsum<G, int32_t>(nw, nullptr);
+  int words;
+  int rank = gscan_excl<G, int>(nw, lane, &words);
+  // the row's sorted (word, mask) list for the numeric list bins, as the
+  // window kernels leave it: words relative to the 32-aligned span start
+  if (words <= kWordListCap) {
+    uint2* wl = A.wlist + (size_t)i * kWordListCap;
+    const uint32_t w0 = (uint32_t)A.rowlo[i] >> 5;
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+      if (!whead[e]) continue;
+      const uint32_t wk = key[e] >> 5;
+      uint32_t mask = 0u;
+      for (int p = lane * E + e; p < N && (skey[p] >> 5) == wk; p++) mask |= 1u << (skey[p] & 31u);
+      wl[rank++] = make_uint2(wk - w0, mask);
+    }
+  }
   if (lane == 0) {
     rowcnt[i] = cnt;
-    A.rowwords[i] = nw | kNoWordList;
+    A.rowwords[i] = words <= kWordListCap ? words : (words | kNoWordList);
   }
 }
 
+// Bits of the product index in a numeric sort key.
+__host__ __device__ __forceinline__ constexpr int sort_key_shift(int N) {
+  return N <= 512 ? 9 : N == 1024 ? 10 : 11;
+}
+
 template <int G, int E, bool UPPER, typename K>
-__global__ void __launch_bounds__(256) k_num_sort(RowArgs A, const int64_t* __restrict__ rowoff,
+__global__ void __launch_bounds__(256) k_num_sort(RowArgs A, int32_t na_max, const int64_t* __restrict__ rowoff,
                                                   int32_t* __restrict__ ccol, double* __restrict__ cval) {
   constexpr int N = G * E;
-  constexpr K kNone = ~(K)0;
+  constexpr int SH = sort_key_shift(N);
+  constexpr K kNone = ~(K)0, kIdx = ((K)1 << SH) - 1;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   const int R = blockDim.x / G;
   const int gi = threadIdx.x / G, lane = threadIdx.x % G;
   const int r = xcd_block(blockIdx.x, gridDim.x) * R + gi;
   if (r >= A.nrows) return;
   const int i = A.rows[r];
-  unsigned char* base = lds_raw + (size_t)gi * sort_row_bytes(N, true);
+  const int64_t ebytes = sort_entry_bytes(na_max), kbytes = align16((int64_t)N * sizeof(K));
+  unsigned char* base = lds_raw + (size_t)gi * sort_row_bytes(N, na_max, (int)sizeof(K), true);
   int32_t* epos = (int32_t*)base;
-  int32_t* eb = (int32_t*)(base + align16((int64_t)(N + 1) * 4));
-  double* ea = (double*)(base + align16((int64_t)(N + 1) * 4) + align16((int64_t)N * 4));
-  double* val = ea + N;
+  int32_t* eb = (int32_t*)(base + align16((int64_t)(na_max + 1) * 4));
+  double* val = (double*)(base + (ebytes > kbytes ? ebytes : kbytes));
   K* skey = (K*)base;
-  const int nA = A.ap[i + 1] - A.ap[i];
-  const int f = sort_stage_entries<G, true>(i, lane, A, epos, eb, ea);
+  const int s = A.ap[i], nA = A.ap[i + 1] - s;
+  const int f = sort_stage_entries<G>(i, lane, A, epos, eb);
   K key[E];
 #pragma unroll
   for (int e = 0; e < E; e++) {
@@ -1410,8 +1444,8 @@ __global__ void __launch_bounds__(256) k_num_sort(RowArgs A, const int64_t* __re
       const int q = eb[en] + t;
       const int j = A.bc[q];
       // AAtS: values only for j >= i (the mirror fills the rest)
-      val[t] = (!UPPER || j >= i) ? ea[en] * A.bv[q] : 0.0;
-      key[e] = ((K)j << 9) | (K)t;
+      val[t] = (!UPPER || j >= i) ? A.av[s + en] * A.bv[q] : 0.0;
+      key[e] = ((K)j << SH) | (K)t;
     }
   }
   reg_bitonic<G, E, K>(key, lane);
@@ -1424,7 +1458,7 @@ __global__ void __launch_bounds__(256) k_num_sort(RowArgs A, const int64_t* __re
 #pragma unroll
   for (int e = 0; e < E; e++) {
     const int p = lane * E + e;
-    head[e] = key[e] != kNone && (p == 0 || (skey[p - 1] >> 9) != (key[e] >> 9));
+    head[e] = key[e] != kNone && (p == 0 || (skey[p - 1] >> SH) != (key[e] >> SH));
     cnt += head[e] ? 1 : 0;
   }
   int tot;
@@ -1432,14 +1466,14 @@ __global__ void __launch_bounds__(256) k_num_sort(RowArgs A, const int64_t* __re
 #pragma unroll
   for (int e = 0; e < E; e++) {
     if (!head[e]) continue;
-    const int col = (int)(key[e] >> 9);
+    const int col = (int)(key[e] >> SH);
     double acc = 0.0;
     if (!UPPER || col >= i) {
-      acc = acc + val[(int)(key[e] & 511)];
+      acc = acc + val[(int)(key[e] & kIdx)];
       for (int p = lane * E + e + 1; p < f; p++) {
         const K kq = skey[p];
-        if ((kq >> 9) != key[e] >> 9) break;
-        acc = acc + val[(int)(kq & 511)];
+        if ((kq >> SH) != key[e] >> SH) break;
+        acc = acc + val[(int)(kq & kIdx)];
       }
     }
     ccol[out] = col;
@@ -1448,7 +1482,13 @@ __global__ void __launch_bounds__(256) k_num_sort(RowArgs A, const int64_t* __re
   }
 }
 
-template <int G, int E>
+// 32-bit keys hold column << S | t while columns stay below 2^(32 - S) - 1
+// (the all-ones key marks an empty slot)
+__host__ __device__ __forceinline__ bool sort_keys64(int32_t ncols, int N) {
+  return (int64_t)ncols >= ((int64_t)1 << (32 - sort_key_shift(N))) - 1;
+}
+
+template <int G>
 static void sort_launch_shape(const RowArgs& a, int64_t row_bytes, int* R, unsigned* blocks) {
   int r = 256 / G;
   while (r > 1 && (int64_t)r * row_bytes > a.lds_budget) r >>= 1;
@@ -1456,32 +1496,36 @@ static void sort_launch_shape(const RowArgs& a, int64_t row_bytes, int* R, unsig
   *blocks = (unsigned)((a.nrows + r - 1) / r);
 }
 template <int G, int E>
-static void sym_sort_launch(const RowArgs& a, int32_t* rowcnt, hipStream_t s) {
+static void sym_sort_launch(const RowArgs& a, int32_t na_max, int32_t* rowcnt, hipStream_t s) {
+  const int64_t rb = sort_row_bytes(G * E, na_max, 4, false);
   int R; unsigned blocks;
-  sort_launch_shape<G, E>(a, sort_row_bytes(G * E, false), &R, &blocks);
-  const size_t lds = (size_t)R * sort_row_bytes(G * E, false);
+  sort_launch_shape<G>(a, rb, &R, &blocks);
+  const size_t lds = (size_t)R * rb;
   (void)hipFuncSetAttribute((const void*)k_sym_sort<G, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  hipLaunchKernelGGL((k_sym_sort<G, E>), dim3(blocks), dim3(R * G), lds, s, a, rowcnt);
+  hipLaunchKernelGGL((k_sym_sort<G, E>), dim3(blocks), dim3(R * G), lds, s, a, na_max, rowcnt);
 }
 template <int G, int E, bool UPPER, typename K>
-static void num_sort_launch_k(const RowArgs& a, const int64_t* rowoff, int32_t* ccol, double* cval, hipStream_t s) {
+static void num_sort_launch_k(const RowArgs& a, int32_t na_max, const int64_t* rowoff, int32_t* ccol, double* cval,
+                              hipStream_t s) {
+  const int64_t rb = sort_row_bytes(G * E, na_max, (int)sizeof(K), true);
   int R; unsigned blocks;
-  sort_launch_shape<G, E>(a, sort_row_bytes(G * E, true), &R, &blocks);
-  const size_t lds = (size_t)R * sort_row_bytes(G * E, true);
+  sort_launch_shape<G>(a, rb, &R, &blocks);
+  const size_t lds = (size_t)R * rb;
   (void)hipFuncSetAttribute((const void*)k_num_sort<G, E, UPPER, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)lds);
-  hipLaunchKernelGGL((k_num_sort<G, E, UPPER, K>), dim3(blocks), dim3(R * G), lds, s, a, rowoff, ccol, cval);
+  hipLaunchKernelGGL((k_num_sort<G, E, UPPER, K>), dim3(blocks), dim3(R * G), lds, s, a, na_max, rowoff, ccol,
+                     cval);
 }
 template <int G, int E>
-static void num_sort_launch(const RowArgs& a, const int64_t* rowoff, int32_t* ccol, double* cval, hipStream_t s) {
-  // 32-bit keys hold column << 9 | t while columns stay below 2^23 - 1
-  const bool k64 = a.ncols >= (1 << 23) - 1;
+static void num_sort_launch(const RowArgs& a, int32_t na_max, const int64_t* rowoff, int32_t* ccol, double* cval,
+                            hipStream_t s) {
+  const bool k64 = sort_keys64(a.ncols, G * E);
   if (a.upper) {
-    if (k64) num_sort_launch_k<G, E, true, uint64_t>(a, rowoff, ccol, cval, s);
-    else num_sort_launch_k<G, E, true, uint32_t>(a, rowoff, ccol, cval, s);
+    if (k64) num_sort_launch_k<G, E, true, uint64_t>(a, na_max, rowoff, ccol, cval, s);
+    else num_sort_launch_k<G, E, true, uint32_t>(a, na_max, rowoff, ccol, cval, s);
   } else {
-    if (k64) num_sort_launch_k<G, E, false, uint64_t>(a, rowoff, ccol, cval, s);
-    else num_sort_launch_k<G, E, false, uint32_t>(a, rowoff, ccol, cval, s);
+    if (k64) num_sort_launch_k<G, E, false, uint64_t>(a, na_max, rowoff, ccol, cval, s);
+    else num_sort_launch_k<G, E, false, uint32_t>(a, na_max, rowoff, ccol, cval, s);
   }
 }
 
@@ -2266,7 +2310,7 @@ static bool bin_dense(int bin) {
 
 int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
   if (bin == kBinHeavy || bin == 0) return 0;
-  if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), false);
+  if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), words_max, 4, false);
   (void)metric_max;
   if (bin == kBinWgBitmap) return (int64_t)words_max * 4;
   if (bin_dense(bin)) return sym_window_bytes(bin_group(bin), words_max);
@@ -2274,7 +2318,7 @@ int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
 }
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max) {
   if (bin == kBinHeavy || bin == 0) return 0;
-  if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), true);
+  if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), words_max, 8, true);
   if (bin == kBinWgBitmap) return dense_row_bytes(words_max, c_max);
   if (bin_is_list(bin)) return num_list_bytes(bin_group(bin), words_max, c_max);
   if (bin_dense(bin)) return num_window_bytes(bin_group(bin), words_max, c_max);
@@ -2444,12 +2488,14 @@ void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* 
   }
   if (bin_is_sort(bin)) {
     switch (sort_bin_slots(bin)) {
-      case 16: sym_sort_launch<16, 1>(a, rowcnt, s); break;
-      case 32: sym_sort_launch<32, 1>(a, rowcnt, s); break;
-      case 64: sym_sort_launch<64, 1>(a, rowcnt, s); break;
-      case 128: sym_sort_launch<64, 2>(a, rowcnt, s); break;
-      case 256: sym_sort_launch<64, 4>(a, rowcnt, s); break;
-      default: sym_sort_launch<64, 8>(a, rowcnt, s); break;
+      case 16: sym_sort_launch<16, 1>(a, words_max, rowcnt, s); break;
+      case 32: sym_sort_launch<32, 1>(a, words_max, rowcnt, s); break;
+      case 64: sym_sort_launch<64, 1>(a, words_max, rowcnt, s); break;
+      case 128: sym_sort_launch<64, 2>(a, words_max, rowcnt, s); break;
+      case 256: sym_sort_launch<64, 4>(a, words_max, rowcnt, s); break;
+      case 512: sym_sort_launch<64, 8>(a, words_max, rowcnt, s); break;
+      case 1024: sym_sort_launch<64, 16>(a, words_max, rowcnt, s); break;
+      default: sym_sort_launch<64, 32>(a, words_max, rowcnt, s); break;
     }
     return;
   }
@@ -2479,12 +2525,14 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
   }
   if (bin_is_sort(bin)) {
     switch (sort_bin_slots(bin)) {
-      case 16: num_sort_launch<16, 1>(a, rowoff, ccol, cval, s); break;
-      case 32: num_sort_launch<32, 1>(a, rowoff, ccol, cval, s); break;
-      case 64: num_sort_launch<64, 1>(a, rowoff, ccol, cval, s); break;
-      case 128: num_sort_launch<64, 2>(a, rowoff, ccol, cval, s); break;
-      case 256: num_sort_launch<64, 4>(a, rowoff, ccol, cval, s); break;
-      default: num_sort_launch<64, 8>(a, rowoff, ccol, cval, s); break;
+      case 16: num_sort_launch<16, 1>(a, words_max, rowoff, ccol, cval, s); break;
+      case 32: num_sort_launch<32, 1>(a, words_max, rowoff, ccol, cval, s); break;
+      case 64: num_sort_launch<64, 1>(a, words_max, rowoff, ccol, cval, s); break;
+      case 128: num_sort_launch<64, 2>(a, words_max, rowoff, ccol, cval, s); break;
+      case 256: num_sort_launch<64, 4>(a, words_max, rowoff, ccol, cval, s); break;
+      case 512: num_sort_launch<64, 8>(a, words_max, rowoff, ccol, cval, s); break;
+      case 1024: num_sort_launch<64, 16>(a, words_max, rowoff, ccol, cval, s); break;
+      default: num_sort_launch<64, 32>(a, words_max, rowoff, ccol, cval, s); break;
     }
     return;
   }

@@ -14,7 +14,7 @@
 //                     pass left a sorted word list, 28 + cls*4 + gidx; the
 //                     LDS holds a byte index per window word and the listed
 //                     words instead of a bitmap and prefix counts
-//   40 .. 45          register sort: rows of at most N = 16 << (bin - 40)
+//   40 .. 47          register sort: rows of at most N = 16 << (bin - 40)
 //                     products (and A entries) whose output is too scattered
 //                     for a window; the products are sorted by column in
 //                     registers (bitonic network over G lanes x E slots)
@@ -25,9 +25,9 @@
 namespace appt {
 
 constexpr int kNumCls = 3;
-constexpr int kNumBins = 46;
+constexpr int kNumBins = 48;
 constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27, kBinList0 = 28, kBinSort0 = 40;
-constexpr int kSortMax = 512;   // products (and A entries) of the largest sort bin
+constexpr int kSortMax = 2048;  // products (and A entries) of the largest sort bin
 constexpr int kHeavyT = 1024;   // threads of a heavy-row workgroup
 constexpr int kStatFields = 3;  // count, max metric, max words
 constexpr int kStatsMaxBlocks = 4096;
@@ -42,6 +42,8 @@ struct BinParams {
   int32_t glen_max[3];            // mean B-row length bounds for G = 8, 16, 32
   int32_t dense_ratio;            // numeric: dense if words(span) <= max(ratio * distinct words, 64)
   int32_t lds_budget;             // LDS bytes a block of the wave-level bins may take
+  int32_t sort_ratio;             // register sort first when span words > sort_ratio * products (0: never);
+                                  // numeric pass: scattered rows only
 };
 
 inline bool bin_is_sort(int bin) { return bin >= kBinSort0; }

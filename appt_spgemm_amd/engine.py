@@ -16,7 +16,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._abi import (JOB_STATES, OPS, STATUS_NAMES, BatchBuffers, BatchInfo, CsrResult, CsrView, Descriptors,
+from ._abi import (JOB_STATES, OPS, STATUS_NAMES, BatchBuffers, BatchInfo, BinReport, CsrResult, CsrView, Descriptors,
                    EngResult, HostCsr, JobInfo, Options, PhaseTimes, ServerOptions, ServerStats, Tuning, check,
                    lib)
 
@@ -223,6 +223,20 @@ class Engine:
         check(lib().appt_engine_phase_times(self.handle, C.byref(t)), "phase_times")
         return {f: getattr(t, f) for f, _ in PhaseTimes._fields_}
 
+    def bin_report(self) -> dict:
+        """Rows per symbolic / numeric bin of the last compute call, by bin name
+        (bins that received no rows are left out)."""
+        r = BinReport()
+        check(lib().appt_engine_bin_report(self.handle, C.byref(r)), "bin_report")
+        out = {"symbolic": {}, "numeric": {}}
+        for b in range(1, r.n_bins):
+            name = lib().appt_bin_name(b).decode()
+            if r.sym_rows[b]:
+                out["symbolic"][name] = (int(r.sym_rows[b]), int(r.sym_metric_max[b]))
+            if r.num_rows[b]:
+                out["numeric"][name] = (int(r.num_rows[b]), int(r.num_metric_max[b]))
+        return out
+
     def device_bytes(self) -> int:
         return int(lib().appt_engine_device_bytes(self.handle))
 
@@ -286,7 +300,9 @@ class Engine:
 def _tuning_dict(t: Tuning) -> dict:
     return {"f_class_max": [int(x) for x in t.f_class_max if x > 0], "wg_lds_bytes": int(t.wg_lds_bytes),
             "lds_budget_bytes": int(t.lds_budget_bytes), "bitmap_words_factor": int(t.bitmap_words_factor),
-            "group_for_len": [int(x) for x in t.group_for_len[:3]], "heavy_window_cols": int(t.heavy_window_cols)}
+            "group_for_len": [int(x) for x in t.group_for_len[:3]], "heavy_window_cols": int(t.heavy_window_cols),
+            "sort_span_factor": int(t.sort_span_factor),
+            "sort_span_factor_symbolic": int(t.sort_span_factor_symbolic)}
 
 
 def panels(rowptr: np.ndarray, max_bytes: int) -> list:

@@ -87,7 +87,13 @@ typedef struct appt_tuning {
   int32_t bitmap_words_factor;/* numeric dense window when words(span) <= factor * distinct words */
   int32_t group_for_len[4];   /* lanes per row for mean B-row length <= 8,16,32,inf (0 = default) */
   int32_t heavy_window_cols;  /* column-window cap of the heavy-row bin (0 = as wide as LDS allows) */
-  int32_t reserved[7];
+  int32_t sort_span_factor;   /* numeric pass: rows of at most 512 products whose columns are
+                                 scattered (fewer than 4 columns per 3 distinct 32-column words) take
+                                 the register-sort bin when their output span exceeds factor * products
+                                 words, even where a window fits (0 = default, < 0 = only rows no
+                                 window fits) */
+  int32_t sort_span_factor_symbolic; /* symbolic pass: the same by span alone (0 = default, < 0 = off) */
+  int32_t reserved[5];
 } appt_tuning;
 
 typedef struct appt_options {
@@ -132,6 +138,23 @@ appt_status appt_engine_destroy(appt_engine* eng);
 /* Enable per-phase HIP-event timing (adds event records, no extra syncs). */
 appt_status appt_engine_set_timing(appt_engine* eng, int enable);
 appt_status appt_engine_phase_times(const appt_engine* eng, appt_phase_times* out);
+
+/* Row bins of the last compute call: how many output rows each symbolic and
+ * numeric bin received (the engine's internal dispatch, the role of the
+ * reference dispatcher's ours/cuSPARSE verdict, bench/dispatch.cu:39-47, at
+ * row granularity).  Bin names come from appt_bin_name. */
+#define APPT_MAX_BINS 64
+typedef struct appt_bin_report {
+  int32_t n_bins;                       /* bins defined: ids 0 .. n_bins - 1 */
+  int32_t sym_rows[APPT_MAX_BINS];      /* rows per symbolic bin */
+  int32_t num_rows[APPT_MAX_BINS];      /* rows per numeric bin */
+  int32_t sym_metric_max[APPT_MAX_BINS];/* largest products count in the bin */
+  int32_t num_metric_max[APPT_MAX_BINS];/* largest output row in the bin */
+} appt_bin_report;
+appt_status appt_engine_bin_report(const appt_engine* eng, appt_bin_report* out);
+/* Static name of a bin id, e.g. "window/c1/G16", "hash/c0/G8", "list/c0/G32",
+ * "sort/N256", "wg-window", "wg-hash", "heavy"; "empty" for 0, NULL past the end. */
+const char* appt_bin_name(int32_t bin);
 /* Release cached device memory held by the engine's pool. */
 appt_status appt_engine_trim(appt_engine* eng);
 /* Bytes currently allocated from HIP by the engine (pool + live results). */

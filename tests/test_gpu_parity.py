@@ -145,9 +145,11 @@ def test_heavy_windows_unsorted(eng_heavy, op):
 def random_rows(n, lens, col_hi, seed):
     """Rows of the given lengths with distinct random columns below col_hi."""
     rng = np.random.default_rng(seed)
+    lens = np.asarray(lens, np.int64)
     rp = np.zeros(n + 1, np.int64)
     rp[1:] = np.cumsum(lens)
-    col = np.concatenate([np.sort(rng.choice(col_hi, size=int(k), replace=False)) for k in lens]).astype(np.int32)
+    parts = [np.sort(rng.choice(col_hi, size=int(lens[r]), replace=False)) for r in np.nonzero(lens)[0]]
+    col = (np.concatenate(parts) if parts else np.zeros(0)).astype(np.int32)
     return host(n, rp.astype(np.int32), col, rng.uniform(-1, 1, size=len(col)))
 
 
@@ -161,6 +163,17 @@ def scattered_cases():
     lens = np.where(np.arange(n) < n // 2, 12, 0)
     lens[7] = 300
     yield "empty_b_rows", random_rows(n, lens, n, 9)
+    # A*A^T rows of ~700 and ~1400 products spread over 5M columns (sort
+    # bins of 1024 and 2048 slots, 64-bit keys): 60000 rows scattered over
+    # [0, n) with entries in the first 1000 columns
+    n = 5_000_000
+    rng = np.random.default_rng(11)
+    lens = np.zeros(n, np.int64)
+    rows = rng.choice(n, size=60_000, replace=False)
+    lens[rows] = 2
+    lens[rows[:300]] = 12
+    lens[rows[300:400]] = 6
+    yield "aat_wide_5m", random_rows(n, lens, 1000, 12)
 
 
 @pytest.mark.parametrize("op", OPS)
