@@ -267,12 +267,15 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
     O.bv = tv;
     bptr = tp;
   }
+  if (op == APPT_OP_AATS) launch_rows_strict(n, A.rowptr, A.col, ctl + 4, s);
   stage_bmeta(W, O, bptr, nullptr, nullptr, 0);
 }
 
+// ctl[0..1]: heavy-row claim counters; ctl[2]: B rows unsorted; ctl[3]: the
+// symmetric-pattern gather failed; ctl[4]: A rows not strictly increasing
 int32_t* Engine::reset_ctl(Workspace& W) {
-  int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * 4));
-  launch_fill_i32(ctl, 4, 0, stream_);
+  int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * 8));
+  launch_fill_i32(ctl, 8, 0, stream_);
   return ctl;
 }
 
@@ -314,9 +317,11 @@ void Engine::read_sym(Workspace& W, Plan& P) {
   APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, sizeof(int32_t) * stat_words(), hipMemcpyDeviceToHost, s));
   APPT_HIP(hipMemcpyAsync(&hs_->products, dstats + stat_words() * 2, sizeof(unsigned long long),
                           hipMemcpyDeviceToHost, s));
+  APPT_HIP(hipMemcpyAsync(hs_->ctl, W.ctl.p, sizeof(hs_->ctl), hipMemcpyDeviceToHost, s));
   APPT_HIP(hipStreamSynchronize(s));
   std::memcpy(P.sym, hs_->sym, sizeof(P.sym));
   P.products = (int64_t)hs_->products;
+  P.a_strict = hs_->ctl[4] == 0;
   P.bins_used = 0;
   for (int b = 1; b < kNumBins; b++) P.bins_used += P.sym[b * kStatFields] > 0 ? 1 : 0;
 }
@@ -421,8 +426,9 @@ void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, c
     const int32_t wmax = P.num[b * kStatFields + 2];
     const int64_t rb = numeric_row_lds_bytes(b, cmax, wmax);
     if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "numeric row exceeds LDS");
-    // AAtS: values are formed for j >= i only; the structure is the full row
-    RowArgs ra = row_args(W, O, numrows + off, cnt, O.upper, cfg.num);
+    // AAtS: values are formed for j >= i only (unless computed directly);
+    // the structure is the full row
+    RowArgs ra = row_args(W, O, numrows + off, cnt, P.direct_sym ? 0 : O.upper, cfg.num);
     launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, heavy, s);
     off += cnt;
   }
@@ -430,7 +436,7 @@ void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, c
 
 void Engine::stage_mirror(Workspace& W, const Operands& O, const Plan& P, const int32_t* ccol, double* cval) {
   // AAtS: lower entries take the value of their upper mirror image
-  if (O.upper) launch_mirror_pull(O.n, P.nnz, as<int64_t>(W.rowoff.p), ccol, cval, stream_);
+  if (O.upper && !P.direct_sym) launch_mirror_pull(O.n, P.nnz, as<int64_t>(W.rowoff.p), ccol, cval, stream_);
 }
 
 // ---------------------------------------------------------------------------
@@ -497,6 +503,12 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   try {
     stage_offsets(W, O, cfg_, rp32);
     read_num(W, O, P);
+    // AAtS on strictly increasing rows with every numeric row in an
+    // order-exact bin: c_ij and c_ji sum the same products a_ik * a_jk in the
+    // same ascending k order, so computing the lower entries directly gives
+    // the mirror's values bit for bit, without the mirror pass
+    P.direct_sym = O.upper && P.a_strict && P.num[kBinWgBitmap * kStatFields] == 0 &&
+                   P.num[kBinWgHash * kStatFields] == 0 && P.num[kBinHeavy * kStatFields] == 0;
     record(4);
     if (admit_) admit_(result_bytes(n, P.nnz));
     // output buffers (the symbolic pass sized C exactly)

@@ -102,6 +102,8 @@ struct Plan {
   int64_t products = 0;
   int64_t nnz = 0;
   int bins_used = 0;
+  bool a_strict = false;    // A's rows strictly increasing (read for AAtS)
+  bool direct_sym = false;  // AAtS: lower entries computed directly, no mirror
 };
 
 struct Result {  // what appt_csr_result::_priv points to
@@ -115,6 +117,7 @@ struct HostStats {  // pinned staging for the small device->host reads
   int32_t num[kNumBins * kStatFields];
   unsigned long long products;
   int64_t total;
+  int32_t ctl[8];
 };
 
 class Engine {

@@ -104,6 +104,23 @@ void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const u
 // inside a B row is not fixed; it does not affect any output value because a
 // B row holds each column once (see the value-order note at the top).
 // ---------------------------------------------------------------------------
+__global__ void k_rows_strict(int32_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                              int32_t* __restrict__ flag) {
+  constexpr int G = 16;
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb)
+    for (int p = ptr[i] + lane; p + 1 < ptr[i + 1]; p += G) bad |= col[p] >= col[p + 1];
+  if (__ballot(bad) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
+}
+void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_rows_strict, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, flag);
+}
+
 __global__ void k_colcount(int64_t nnz, const int32_t* __restrict__ col, int32_t* __restrict__ cnt) {
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz;
        p += (int64_t)gridDim.x * blockDim.x)

@@ -70,6 +70,9 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
                      const int32_t* col, int4* meta, int32_t* flag, hipStream_t s);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
+// flag[0] = 1 (zero beforehand) when some row of (ptr, col) is not strictly
+// increasing
+void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);
 // Symmetric-pattern transpose into (bc, bv) laid out like A; sets *asym
 // (zero beforehand) when A is not strictly sorted with a symmetric pattern
 // (tp: the transpose's row pointers from the column counts).
