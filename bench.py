@@ -40,7 +40,16 @@ def make_workload(name: str):
     if name.startswith("rmat"):
         scale = int(name[4:] or 16)
         return ap.rmat(scale, 16, seed=1), f"R-MAT scale {scale}, edge factor 16, (.57,.19,.19,.05), seed 1"
+    if name.startswith("er"):
+        n, deg = er_params(name)
+        return ap.erdos_renyi(n, float(deg), 3), f"Erdos-Renyi n={n}, mean row nnz {deg}, seed 3"
     raise SystemExit(f"unknown workload {name}")
+
+
+def er_params(name: str):
+    """er<n>_<mean row nnz>, e.g. er200000_16."""
+    n, deg = name[2:].split("_")
+    return int(n), int(deg)
 
 
 GEN_SPEC = {"stencil27_96": "gen:stencil27:96:0:1", "stencil27_96_drop": "gen:stencil27:96:0.1:1",
@@ -55,6 +64,9 @@ def rocsparse_context(workload: str, op: str, reps: int, timeout_s: float = 240.
     spec = GEN_SPEC.get(workload)
     if spec is None and workload.startswith("rmat"):
         spec = f"gen:rmat:{int(workload[4:] or 16)}:16:1"
+    if spec is None and workload.startswith("er"):
+        n, deg = er_params(workload)
+        spec = f"gen:er:{n}:{deg}:3"
     exe = os.path.join(ROOT, "appt_spgemm_amd", "bin", "appt_rocsparse")
     if spec is None or not os.path.exists(exe):
         return None
