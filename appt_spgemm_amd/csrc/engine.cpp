@@ -245,9 +245,9 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
   int32_t* ctl = reset_ctl(W);
   if (op != APPT_OP_AA) {
     // B = A^T with sorted rows (heavy rows then carry per-entry cursors
-    // across windows): column histogram and scan; a symmetric pattern is
-    // gathered in place of A's layout, any other takes the atomic-cursor
-    // scatter and a per-row sort (both skipped when the gather succeeded)
+    // across windows): a symmetric pattern is gathered in A's layout; any
+    // other takes the column histogram, scan, atomic-cursor scatter and
+    // per-row sort (all skipped on the device when the gather succeeded)
     int32_t* tp = as<int32_t>(ensure(W.bptr, sizeof(int32_t) * ((size_t)n + 1)));
     int32_t* tc = as<int32_t>(ensure(W.bcol, sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
     double* tv = as<double>(ensure(W.bval, sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
@@ -256,13 +256,7 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
     int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
     int32_t* slist = as<int32_t>(ensure(W.sortlist, sizeof(int32_t) * (size_t)sort_rows_scratch_elems(n)));
     int32_t* asym = ctl + 3;
-    launch_fill_i32(cnt, n, 0, s);
-    launch_colcount(nnz, A.col, cnt, s);
-    launch_scan(cnt, n, off64, tp, tmp, s);
-    launch_transpose_gather(n, A.rowptr, A.col, A.val, tp, tc, tv, asym, s);
-    launch_i64_to_i32(off64, n, cnt, s);
-    launch_transpose_scatter(n, A.rowptr, A.col, A.val, cnt, tc, tv, asym, s);
-    launch_sort_rows(n, tp, tc, tv, slist, asym, s);
+    launch_transpose(n, nnz, A.rowptr, A.col, A.val, tp, tc, tv, cnt, off64, tmp, slist, asym, s);
     O.bc = tc;
     O.bv = tv;
     bptr = tp;

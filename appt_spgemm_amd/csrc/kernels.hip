@@ -21,28 +21,34 @@ namespace appt {
 // ---------------------------------------------------------------------------
 // small utilities
 // ---------------------------------------------------------------------------
-__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+// gate (optional) on a kernel: it runs only when *gate != 0 -- device-side
+// control flow, no host synchronisation
+__device__ __forceinline__ bool gated_off(const int32_t* gate) { return gate && *gate == 0; }
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v, const int32_t* gate) {
+  if (gated_off(gate)) return;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x)
     p[t] = v;
 }
-void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s, const int32_t* gate) {
   if (n <= 0) return;
   int64_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v, gate);
 }
 
-__global__ void k_i64_to_i32(const int64_t* in, int64_t n, int32_t* out) {
+__global__ void k_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, const int32_t* gate) {
+  if (gated_off(gate)) return;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x)
     out[t] = (int32_t)in[t];
 }
-void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s) {
+void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s, const int32_t* gate) {
   if (n <= 0) return;
   int64_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)blocks), dim3(256), 0, s, in, n, out);
+  hipLaunchKernelGGL(k_i64_to_i32, dim3((unsigned)blocks), dim3(256), 0, s, in, n, out, gate);
 }
 __global__ void k_copy_i64(const int64_t* in, int64_t n, int64_t* out) {
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -121,22 +127,27 @@ void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32
   hipLaunchKernelGGL(k_rows_strict, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, flag);
 }
 
-__global__ void k_colcount(int64_t nnz, const int32_t* __restrict__ col, int32_t* __restrict__ cnt) {
+__global__ void k_colcount(int64_t nnz, const int32_t* __restrict__ col, int32_t* __restrict__ cnt,
+                           const int32_t* gate) {
+  if (gated_off(gate)) return;
   for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nnz;
        p += (int64_t)gridDim.x * blockDim.x)
     atomicAdd(&cnt[col[p]], 1);
 }
-void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s) {
+void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s, const int32_t* gate) {
   if (nnz <= 0) return;
   int64_t blocks = (nnz + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_colcount, dim3((unsigned)blocks), dim3(256), 0, s, nnz, col, cnt);
+  hipLaunchKernelGGL(k_colcount, dim3((unsigned)blocks), dim3(256), 0, s, nnz, col, cnt, gate);
 }
 
 // Symmetric-pattern transpose: when every row is strictly sorted and each
 // stored (k, i) has a stored (i, k), B = A^T has A's pattern, so B row k is
-// A row k with values a_ik found by binary search in row i.  Any miss,
-// duplicate or descent sets *asym and the scatter path below runs instead.
+// A row k with values a_ik looked up in row i: first at the mirrored index
+// (entry d of row k is expected at index len_i - 1 - d of row i, exact for
+// the interior rows of a symmetric stencil), else by binary search.  Any
+// miss, duplicate or descent sets *asym and the scatter path below runs
+// instead.
 __global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
                                    const double* __restrict__ av, int32_t* __restrict__ bc, double* __restrict__ bv,
                                    int32_t* __restrict__ asym) {
@@ -153,10 +164,15 @@ __global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, co
       if (p > s && ac[p - 1] >= i) miss = true;   // descent or duplicate
       int lo = ap[i], hi = ap[i + 1];
       const int end = hi;
-      while (lo < hi) {
-        const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
-        if (ac[mid] < (int)k) lo = mid + 1;
-        else hi = mid;
+      const int g = hi - 1 - (p - s);
+      if (g >= lo && g < hi && ac[g] == (int)k) {
+        lo = g;
+      } else {
+        while (lo < hi) {
+          const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
+          if (ac[mid] < (int)k) lo = mid + 1;
+          else hi = mid;
+        }
       }
       if (lo < end && ac[lo] == (int)k) {
         bc[p] = i;
@@ -168,26 +184,12 @@ __global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, co
     if (miss) *asym = 1;   // every writer stores 1
   }
 }
-// A symmetric pattern has equal row and column counts: any row whose
-// transpose row pointer differs from A's marks the pattern asymmetric before
-// the gather runs.
-__global__ void k_pattern_counts(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ tp,
-                                 int32_t* __restrict__ asym) {
-  bool diff = false;
+// B's row pointers are A's when the gather succeeded (*asym == 0)
+__global__ void k_rowptr_if_symmetric(int32_t n, const int32_t* __restrict__ ap, int32_t* __restrict__ tp,
+                                      const int32_t* __restrict__ asym) {
+  if (*asym != 0) return;
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k <= n; k += (int64_t)gridDim.x * blockDim.x)
-    diff |= ap[k] != tp[k];
-  if (__ballot(diff) != 0ull && threadIdx.x % kWave == 0) *asym = 1;  // every writer stores 1
-}
-
-void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, const int32_t* tp,
-                             int32_t* bc, double* bv, int32_t* asym, hipStream_t s) {
-  if (n <= 0) return;
-  int64_t b0 = ((int64_t)n + 256) / 256;
-  if (b0 > 4096) b0 = 4096;
-  hipLaunchKernelGGL(k_pattern_counts, dim3((unsigned)b0), dim3(256), 0, s, n, ap, tp, asym);
-  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_transpose_gather, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, av, bc, bv, asym);
+    tp[k] = ap[k];
 }
 
 // gate (optional): the scatter and the row sort run only when *gate != 0
@@ -225,7 +227,9 @@ constexpr int kScanThreads = 256, kScanItems = 8, kScanTile = kScanThreads * kSc
 
 int64_t scan_tmp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 2; }
 
-__global__ void k_scan_reduce(const int32_t* __restrict__ in, int64_t n, int64_t* __restrict__ part) {
+__global__ void k_scan_reduce(const int32_t* __restrict__ in, int64_t n, int64_t* __restrict__ part,
+                              const int32_t* gate) {
+  if (gated_off(gate)) return;
   __shared__ int64_t scratch[17];
   const int64_t base = (int64_t)blockIdx.x * kScanTile;
   int64_t sum = 0;
@@ -239,7 +243,8 @@ __global__ void k_scan_reduce(const int32_t* __restrict__ in, int64_t n, int64_t
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-__global__ void k_scan_partials(int64_t* part, int64_t nb) {
+__global__ void k_scan_partials(int64_t* part, int64_t nb, const int32_t* gate) {
+  if (gated_off(gate)) return;
   __shared__ int64_t scratch[17];
   int64_t carry = 0;
   for (int64_t base = 0; base < nb; base += blockDim.x) {
@@ -254,7 +259,9 @@ __global__ void k_scan_partials(int64_t* part, int64_t nb) {
 }
 
 __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const int64_t* __restrict__ part,
-                             int64_t nb, int64_t* __restrict__ out64, int32_t* __restrict__ out32) {
+                             int64_t nb, int64_t* __restrict__ out64, int32_t* __restrict__ out32,
+                             const int32_t* gate) {
+  if (gated_off(gate)) return;
   __shared__ int64_t scratch[17];
   const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
   int32_t v[kScanItems];
@@ -283,16 +290,36 @@ __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const in
 }
 
 void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, int64_t* tmp,
-                 hipStream_t s) {
+                 hipStream_t s, const int32_t* gate) {
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 0) {
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)out64, (int64_t)2, 0);
-    if (out32) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out32, (int64_t)1, 0);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)out64, (int64_t)2, 0, gate);
+    if (out32) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out32, (int64_t)1, 0, gate);
     return;
   }
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, tmp);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, tmp, nb);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, tmp, nb, out64, out32);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, tmp, gate);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, tmp, nb, gate);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, tmp, nb, out64, out32,
+                     gate);
+}
+
+void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
+                      int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
+                      int32_t* slist, int32_t* asym, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_transpose_gather, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, av, bc, bv, asym);
+  int64_t b0 = ((int64_t)n + 256) / 256;
+  if (b0 > 4096) b0 = 4096;
+  hipLaunchKernelGGL(k_rowptr_if_symmetric, dim3((unsigned)b0), dim3(256), 0, s, n, ap, tp, asym);
+  // asymmetric: column histogram, scan, atomic-cursor scatter, row sort
+  launch_fill_i32(cnt, n, 0, s, asym);
+  launch_colcount(nnz, ac, cnt, s, asym);
+  launch_scan(cnt, n, off64, tp, tmp, s, asym);
+  launch_i64_to_i32(off64, n, cnt, s, asym);
+  launch_transpose_scatter(n, ap, ac, av, cnt, bc, bv, asym, s);
+  launch_sort_rows(n, tp, bc, bv, slist, asym, s);
 }
 
 // ---------------------------------------------------------------------------

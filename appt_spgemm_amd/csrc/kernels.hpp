@@ -61,7 +61,8 @@ inline int64_t sym_slots(int64_t f) { return np2_host((f * 4) / 3 + 1); }
 inline int64_t num_slots(int64_t c) { return np2_host((c * 3) / 2 + 1); }
 
 // ---- launchers (all asynchronous on `s`) ----
-void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+// gate (optional) on a launcher: its kernels run only when *gate != 0
+void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s, const int32_t* gate = nullptr);
 // B-row metadata {start, end, min column, max column} (min = INT32_MAX,
 // max = -1 for an empty row).
 // Row k is [ptr[k], ptr[k+1]) of col, or [off2 + ptr2[k], off2 + ptr2[k+1])
@@ -69,15 +70,18 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 // columns are not in nondecreasing order (flag must be zero beforehand).
 void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
                      const int32_t* col, int4* meta, int32_t* flag, hipStream_t s);
-void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s);
+void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s, const int32_t* gate = nullptr);
 // flag[0] = 1 (zero beforehand) when some row of (ptr, col) is not strictly
 // increasing
 void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);
-// Symmetric-pattern transpose into (bc, bv) laid out like A; sets *asym
-// (zero beforehand) when A is not strictly sorted with a symmetric pattern
-// (tp: the transpose's row pointers from the column counts).
-void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, const int32_t* tp,
-                             int32_t* bc, double* bv, int32_t* asym, hipStream_t s);
+// B = A^T with sorted rows into (tp, bc, bv), decided on the device: a
+// symmetric-pattern gather in A's layout (tp = ap); when A is not strictly
+// sorted with a symmetric pattern it sets *asym (zero beforehand) and the
+// column histogram (cnt, n + 1 int32), scan (off64, tmp), atomic-cursor
+// scatter and row sort (slist) run instead.
+void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
+                      int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
+                      int32_t* slist, int32_t* asym, hipStream_t s);
 // Atomic-cursor scatter; gate (optional): runs only when *gate != 0.
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
                               int32_t* cursor, int32_t* bc, double* bv, const int32_t* gate, hipStream_t s);
@@ -91,7 +95,7 @@ void launch_sort_rows(int32_t n, const int32_t* bp, int32_t* bc, double* bv, int
 // scan_tmp_elems(n) int64.
 int64_t scan_tmp_elems(int64_t n);
 void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, int64_t* tmp,
-                 hipStream_t s);
+                 hipStream_t s, const int32_t* gate = nullptr);
 void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4* bmeta, int upper,
                     const BinParams& P,
                     int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin, int32_t* stats,
@@ -180,7 +184,7 @@ void launch_compact_nonzero(int32_t n, const void* rp, bool rp64, const int32_t*
 
 void set_num_cus(int n);
 void set_lds_per_cu(int64_t b);
-void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s);
+void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s, const int32_t* gate = nullptr);
 void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s);
 
 }  // namespace appt
