@@ -342,9 +342,16 @@ __device__ __forceinline__ T ld_idx(const T* base, uint32_t idx) {
 
 __host__ __device__ __forceinline__ int64_t align16(int64_t x) { return (x + 15) & ~(int64_t)15; }
 
-// symbolic window LDS per group: meta (G int2) | bits (W + G u32) | touched (TL u32) | ctl (4 int)
+// The symbolic window's bitmap is stored skewed by one LDS bank per 32
+// words (word w at w + w/32): words a multiple of 32 apart -- the plane
+// offsets of a 3-D stencil, 288 words on a 96^3 grid -- then fall in
+// different banks instead of serialising the group's atomic ORs.
+__host__ __device__ __forceinline__ uint32_t bank_skew(uint32_t w) { return w + (w >> 5); }
+__host__ __device__ __forceinline__ int64_t skewed_words(int64_t W) { return W + (W >> 5) + 1; }
+// symbolic window LDS per group: meta (G int2) | bits (skewed W, then G dummy
+// u32) | touched (TL u32) | ctl (4 int)
 __host__ __device__ __forceinline__ int64_t sym_window_bytes(int G, int64_t W) {
-  return align16(G * 8) + align16((W + G) * 4) + align16(kTouchCap * 4) + 16;
+  return align16(G * 8) + align16((skewed_words(W) + G) * 4) + align16(kTouchCap * 4) + 16;
 }
 // numeric window LDS per group: meta (G int2 + G f64) | bits (W + G u32) | pre (W u16) |
 // vals (Cm + G f64) | cols (Cm i32)
@@ -1977,7 +1984,11 @@ __device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool ln
     const bool use = ok;
     const uint32_t b = (uint32_t)(j - lo);
     const uint32_t wd = (b >> 5) < (uint32_t)W ? (b >> 5) : (uint32_t)W;  // in range by construction
-    const uint32_t old = atomicOr(&bits[use ? wd : (uint32_t)(W + lane)], 1u << (b & 31u));
+    // the symbolic window (TRACK) stores its bitmap bank-skewed; the numeric
+    // dense window does not
+    const uint32_t slot = TRACK ? (use ? bank_skew(wd) : (uint32_t)(skewed_words(W) + lane))
+                                : (use ? wd : (uint32_t)(W + lane));
+    const uint32_t old = atomicOr(&bits[slot], 1u << (b & 31u));
     if (TRACK && __builtin_expect(use && old == 0u, 0)) {
       const int idx = atomicAdd(&ctl[0], 1);
       if (idx < kTouchCap) touched[idx] = b >> 5;
@@ -2078,9 +2089,9 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
   unsigned char* base = lds_raw + (size_t)gi * sym_window_bytes(G, W);
   int2* mbe = (int2*)base;
   uint32_t* bits = (uint32_t*)(base + align16(G * 8));
-  uint32_t* touched = (uint32_t*)((unsigned char*)bits + align16((W + G) * 4));
+  uint32_t* touched = (uint32_t*)((unsigned char*)bits + align16((skewed_words(W) + G) * 4));
   int* ctl = (int*)((unsigned char*)touched + align16(kTouchCap * 4));
-  for (int t = lane; t < W + G; t += G) bits[t] = 0u;
+  for (int t = lane; t < skewed_words(W) + G; t += G) bits[t] = 0u;
   const RowWalk walk = row_walk(A.nrows, R, gi);
   const int S = walk.step, r1 = walk.end;
   int r = walk.first;
@@ -2125,7 +2136,7 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
       for (int k = 0; k < PL; k++) {
         const int u = lane + k * G;
         myw[k] = u < nt ? touched[u] : 0u;
-        mym[k] = u < nt ? bits[myw[k]] : 0u;
+        mym[k] = u < nt ? bits[bank_skew(myw[k])] : 0u;
         c += __popc(mym[k]);
       }
       if (nt <= kWordCap) {
@@ -2143,10 +2154,14 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
       gsync<G>();
 #pragma unroll
       for (int k = 0; k < PL; k++)
-        if (lane + k * G < nt) bits[myw[k]] = 0u;
+        if (lane + k * G < nt) bits[bank_skew(myw[k])] = 0u;
     } else {
       const int words = (int)span_words(lo, A.rowhi[i]);
-      for (int w = lane; w < words; w += G) { c += __popc(bits[w]); bits[w] = 0u; }
+      for (int w = lane; w < words; w += G) {
+        const uint32_t sw = bank_skew((uint32_t)w);
+        c += __popc(bits[sw]);
+        bits[sw] = 0u;
+      }
     }
     c = gsum<G, int32_t>(c, nullptr);
     if (lane == 0) { rowcnt[i] = c; A.rowwords[i] = nt; }
