@@ -165,6 +165,7 @@ _SIGNATURES = {
     "appt_server_release": (C.c_int, [C.c_void_p, C.c_uint64]),
     "appt_server_stats_get": (C.c_int, [C.c_void_p, P(ServerStats)]),
     "appt_device_upload": (C.c_int, [C.c_void_p, P(HostCsr), P(CsrView)]),
+    "appt_csr_check": (C.c_int, [C.c_void_p, P(CsrView)]),
     "appt_device_free_view": (C.c_int, [C.c_void_p, P(CsrView)]),
     "appt_csr_download_rows": (C.c_int, [C.c_void_p, P(CsrResult), C.c_int32, C.c_int32, P(C.c_int64),
                                           P(C.c_int32), P(C.c_double)]),

@@ -241,8 +241,17 @@ appt_status appt_batch_info_get(const appt_batch* b, appt_batch_info* out) {
   return guard([&] { b->impl->info(out); });
 }
 
+appt_status appt_csr_check(appt_engine* eng, const appt_csr_view* A) {
+  if (!eng || !A) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->check(*A); });
+}
+
 appt_status appt_device_upload(appt_engine* eng, const appt_host_csr* h, appt_csr_view* v) {
   if (!eng || !h || !v) return APPT_ERR_INVALID;
+  if (!appt::host_csr_valid(*h)) {
+    appt::set_last_error("host CSR structure invalid (row pointers or column indices)");
+    return APPT_ERR_INVALID;
+  }
   return guard([&] {
     APPT_HIP(hipSetDevice(eng->impl->device()));
     int32_t *rp = nullptr, *c = nullptr;

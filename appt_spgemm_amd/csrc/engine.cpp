@@ -266,7 +266,8 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
 }
 
 // ctl[0..1]: heavy-row claim counters; ctl[2]: B rows unsorted; ctl[3]: the
-// symmetric-pattern gather failed; ctl[4]: A rows not strictly increasing
+// symmetric-pattern gather failed; ctl[4]: A rows not strictly increasing;
+// ctl[5..7]: structure check failures (ends, row pointer order, columns)
 int32_t* Engine::reset_ctl(Workspace& W) {
   int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * 8));
   launch_fill_i32(ctl, 8, 0, stream_);
@@ -570,6 +571,18 @@ void Engine::release(appt_csr_result* C) {
   pool_.put(R->val);
   delete R;
   std::memset(C, 0, sizeof(*C));
+}
+
+void Engine::check(const appt_csr_view& A) {
+  validate(A);
+  APPT_HIP(hipSetDevice(device_));
+  int32_t* flags = reset_ctl(ws_) + 5;
+  launch_csr_check(A.n, A.nnz, A.rowptr, A.col, flags, stream_);
+  APPT_HIP(hipMemcpyAsync(hs_->ctl, ws_.ctl.p, sizeof(hs_->ctl), hipMemcpyDeviceToHost, stream_));
+  APPT_HIP(hipStreamSynchronize(stream_));
+  if (hs_->ctl[5]) throw Error(APPT_ERR_INVALID, "rowptr[0] != 0 or rowptr[n] != nnz");
+  if (hs_->ctl[6]) throw Error(APPT_ERR_INVALID, "row pointers decrease");
+  if (hs_->ctl[7]) throw Error(APPT_ERR_INVALID, "column index outside [0, n)");
 }
 
 void Engine::eliminate_zeros(appt_csr_result* C) {

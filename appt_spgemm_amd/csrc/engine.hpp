@@ -134,6 +134,8 @@ class Engine {
   // Drop entries whose value is exactly zero (numeric semantics); C must be
   // an engine-owned result.
   void eliminate_zeros(appt_csr_result* C);
+  // Device-side structure check of A (throws APPT_ERR_INVALID).
+  void check(const appt_csr_view& A);
   void run_timed(const char* task, const appt_csr_view& A, int reps, appt_eng_result* out);
   void descriptors(const appt_csr_view& A, appt_op op, appt_descriptors* out);
 
@@ -200,6 +202,8 @@ class Engine {
 Config make_config(const appt_tuning* t, int64_t lds_max);
 // Device bytes of a CSR result with n rows and nnz entries.
 int64_t result_bytes(int32_t n, int64_t nnz);
+// Square CSR structure check of a host matrix (gen.cpp).
+bool host_csr_valid(const appt_host_csr& m);
 appt_tuning tuning_of(const Config& c);
 
 // Many independent matrices as one block-diagonal product (A*A or A*A^T per

@@ -85,6 +85,22 @@ appt_status from_pairs(int32_t n, std::vector<uint64_t>& keys, ValFn&& val, appt
 
 }  // namespace
 
+namespace appt {
+// Square CSR structure: rowptr[0] = 0, rowptr[n] = nnz, non-decreasing row
+// pointers, columns in [0, n).
+bool host_csr_valid(const appt_host_csr& m) {
+  if (m.n < 0 || m.nnz < 0 || m.nnz > INT32_MAX || !m.rowptr) return false;
+  if (m.nnz > 0 && !m.col) return false;
+  if (m.rowptr[0] != 0 || m.rowptr[m.n] != m.nnz) return false;
+  for (int32_t i = 0; i < m.n; i++)
+    if (m.rowptr[i + 1] < m.rowptr[i]) return false;
+  for (int64_t p = 0; p < m.nnz; p++)
+    if (m.col[p] < 0 || m.col[p] >= m.n) return false;
+  return true;
+}
+
+}  // namespace appt
+
 extern "C" {
 
 void appt_host_csr_free(appt_host_csr* m) {
@@ -123,11 +139,11 @@ appt_status appt_csr1_read(const char* path, appt_host_csr* out) {
             (int64_t)std::fread(out->val, 8, (size_t)nnz, f) == nnz;
   std::fclose(f);
   if (!ok) { appt_host_csr_free(out); appt::set_last_error("CSR1 truncated"); return APPT_ERR_IO; }
-  // structural validation
-  if (out->rowptr[0] != 0 || out->rowptr[n] != nnz) ok = false;
-  for (int32_t i = 0; ok && i < n; i++) if (out->rowptr[i + 1] < out->rowptr[i]) ok = false;
-  for (int64_t p = 0; ok && p < nnz; p++) if (out->col[p] < 0 || out->col[p] >= n) ok = false;
-  if (!ok) { appt_host_csr_free(out); appt::set_last_error("CSR1 structure invalid"); return APPT_ERR_INVALID; }
+  if (!appt::host_csr_valid(*out)) {
+    appt_host_csr_free(out);
+    appt::set_last_error("CSR1 structure invalid");
+    return APPT_ERR_INVALID;
+  }
   return APPT_OK;
 }
 

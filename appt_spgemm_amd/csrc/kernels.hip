@@ -127,6 +127,25 @@ void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32
   hipLaunchKernelGGL(k_rows_strict, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, flag);
 }
 
+__global__ void k_csr_check(int32_t n, int64_t nnz, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                            int32_t* __restrict__ flag) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, st = (int64_t)gridDim.x * blockDim.x;
+  if (t0 == 0 && (ptr[0] != 0 || (int64_t)ptr[n] != nnz)) flag[0] = 1;
+  bool bad = false;
+  for (int64_t i = t0; i < n; i += st) bad |= ptr[i + 1] < ptr[i];
+  if (__ballot(bad) != 0ull && threadIdx.x % kWave == 0) flag[1] = 1;
+  bad = false;
+  for (int64_t p = t0; p < nnz; p += st) bad |= (uint32_t)col[p] >= (uint32_t)n;
+  if (__ballot(bad) != 0ull && threadIdx.x % kWave == 0) flag[2] = 1;
+}
+void launch_csr_check(int32_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s) {
+  const int64_t work = std::max<int64_t>((int64_t)n, nnz);
+  int64_t blocks = (work + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_csr_check, dim3((unsigned)blocks), dim3(256), 0, s, n, nnz, ptr, col, flag);
+}
+
 __global__ void k_colcount(int64_t nnz, const int32_t* __restrict__ col, int32_t* __restrict__ cnt,
                            const int32_t* gate) {
   if (gated_off(gate)) return;
@@ -2033,19 +2052,18 @@ __device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool
   auto add = [&](int j, double prod, bool ok) {
     const bool use = ok && (!UPPER || j >= i);
     const uint32_t bb = use ? (uint32_t)(j - lo) : 0u;
-    const uint32_t w = (bb >> 5) < (uint32_t)W ? (bb >> 5) : 0u;  // in range by construction
+    // valid operands keep every index below in range: the word inside the
+    // window, the listed entry below list_cap(G), the position below the
+    // row's count (appt_csr_check is the check for untrusted input)
+    const uint32_t w = bb >> 5;
     uint32_t pos;
     if constexpr (LIST) {
-      // word -> listed entry (mask, first position); indices < list_cap(G) by construction
-      const uint32_t x = widx[w];
-      const uint2 te = wtab[x < (uint32_t)list_cap(G) ? x : 0u];
+      const uint2 te = wtab[widx[w]];  // word -> listed entry (mask, first position)
       pos = te.y + __popc(te.x & ((1u << (bb & 31u)) - 1u));
     } else {
       pos = (uint32_t)pre[w] + __popc(bits[w] & ((1u << (bb & 31u)) - 1u));
     }
-    // positions are < c <= Cm by construction; the clamp keeps a damaged
-    // input from writing outside this group's LDS region
-    const uint32_t slot = use ? (pos < (uint32_t)Cm ? pos : (uint32_t)Cm) : (uint32_t)(Cm + lane);
+    const uint32_t slot = use ? pos : (uint32_t)(Cm + lane);
     atomicAdd(&vals[slot], prod);
   };
   if (!lng) {

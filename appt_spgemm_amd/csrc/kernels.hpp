@@ -71,6 +71,11 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s, const int3
 void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
                      const int32_t* col, int4* meta, int32_t* flag, hipStream_t s);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s, const int32_t* gate = nullptr);
+// Structure check of a square CSR (flags zero beforehand): flag[0] = 1 when
+// ptr[0] != 0 or ptr[n] != nnz, flag[1] = 1 when row pointers decrease,
+// flag[2] = 1 when a column lies outside [0, n).  Columns are read only
+// within [0, nnz).
+void launch_csr_check(int32_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);
 // flag[0] = 1 (zero beforehand) when some row of (ptr, col) is not strictly
 // increasing
 void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);

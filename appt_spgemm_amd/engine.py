@@ -249,6 +249,10 @@ class Engine:
         check(lib().appt_device_upload(self.handle, C.byref(h), C.byref(v)), "device_upload")
         return DeviceMatrix(self, v)
 
+    def check(self, A: DeviceMatrix) -> None:
+        """Structure check of a device CSR on the GPU; raises on invalid input."""
+        check(lib().appt_csr_check(self.handle, C.byref(A.view)), "csr_check")
+
     def compute(self, A: DeviceMatrix, op: str = "AA") -> DeviceResult:
         r = CsrResult()
         check(lib().appt_spgemm_compute(self.handle, C.byref(A.view), OPS[op], C.byref(r)),

@@ -353,7 +353,17 @@ appt_status appt_gen_banded(int32_t n, int32_t bw, double fill, uint64_t seed, a
 appt_status appt_gen_erdos_renyi(int32_t n, double mean_deg, uint64_t seed, appt_host_csr* out);
 
 /* ---- host <-> device helpers (for callers without their own allocator) ------- */
+/* Copies a host CSR to new device buffers; APPT_ERR_INVALID (nothing
+ * allocated) unless rowptr[0] = 0, rowptr[n] = nnz, row pointers are
+ * non-decreasing and every column is in [0, n). */
 appt_status appt_device_upload(appt_engine* eng, const appt_host_csr* h, appt_csr_view* out_dev);
+/* The same structure check for a caller's device CSR, on the GPU (one pass
+ * over rowptr and col, one synchronisation): APPT_OK, or APPT_ERR_INVALID
+ * with the first failed condition in appt_last_error().  The compute calls
+ * take their operands as valid, as the reference does (an out-of-range
+ * column is undefined behaviour there); this is the check for untrusted
+ * input.  No reference counterpart. */
+appt_status appt_csr_check(appt_engine* eng, const appt_csr_view* A);
 appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v);
 /* Rows [row_begin, row_end) of a device result to host arrays: rowptr gets
  * row_end - row_begin + 1 int64 offsets relative to the first row, col / val

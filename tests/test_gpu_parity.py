@@ -261,6 +261,41 @@ def test_eliminate_zeros_cancel_2x2(eng):
     assert got.rowptr.tolist() == [0, 1, 2] and got.col.tolist() == [0, 1] and got.val.tolist() == [2.0, 2.0]
 
 
+def _poke_i32(addr, index, value):
+    """Overwrite one int32 of device memory (hipMemcpy from the host)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    x = ctypes.c_int32(value)
+    assert hip.hipMemcpy(ctypes.c_void_p(addr + 4 * index), ctypes.byref(x), ctypes.c_size_t(4), 1) == 0
+
+
+def test_csr_check_device(eng):
+    m = ap.stencil27(6, 0.1, 3)
+    say("case csr_check")
+    for poke, msg in ((None, None), (("col", 5, m.n), "column index"), (("col", 7, -1), "column index"),
+                      (("rowptr", m.n, m.nnz + 1), "!= nnz"), (("rowptr", 3, 0), "decrease")):
+        A = eng.upload(m)
+        try:
+            if poke is None:
+                eng.check(A)
+                continue
+            which, idx, val = poke
+            _poke_i32(A.view.col if which == "col" else A.view.rowptr, idx, val)
+            with pytest.raises(ap.ApptError, match=msg):
+                eng.check(A)
+        finally:
+            A.free()
+
+
+def test_upload_rejects_invalid_host_csr(eng):
+    bad = host(3, [0, 1, 2, 3], [0, 3, 1], [1.0, 2.0, 3.0])   # column 3 in a 3x3 matrix
+    with pytest.raises(ap.ApptError, match="invalid"):
+        eng.upload(bad)
+    bad = host(3, [0, 2, 1, 3], [0, 1, 2], [1.0, 2.0, 3.0])   # decreasing row pointers
+    with pytest.raises(ap.ApptError):
+        eng.upload(bad)
+
+
 def test_deterministic_repeat(eng):
     m = ap.stencil27(12, 0.1, 9)
     a = eng.multiply(m, "AAt")
