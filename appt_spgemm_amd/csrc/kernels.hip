@@ -2024,8 +2024,27 @@ __device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool ln
       okm |= (ok ? 1u : 0u) << u;
       jj[u] = ld_idx<int32_t, W64>(A.bc, ok ? (uint32_t)q : 0u);
     }
+    // the chunk's atomic ORs are issued back to back; first touches are
+    // recorded after all of them returned
+    uint32_t old[B];
 #pragma unroll
-    for (int u = 0; u < B; u++) apply(jj[u], (okm >> u) & 1u);
+    for (int u = 0; u < B; u++) {
+      const bool use = (okm >> u) & 1u;
+      const uint32_t b = (uint32_t)(jj[u] - lo);
+      const uint32_t wd = (b >> 5) < (uint32_t)W ? (b >> 5) : (uint32_t)W;  // in range by construction
+      const uint32_t slot = TRACK ? (use ? bank_skew(wd) : (uint32_t)(skewed_words(W) + lane))
+                                  : (use ? wd : (uint32_t)(W + lane));
+      old[u] = atomicOr(&bits[slot], 1u << (b & 31u));
+    }
+    if (TRACK) {
+#pragma unroll
+      for (int u = 0; u < B; u++) {
+        if (__builtin_expect(((okm >> u) & 1u) && old[u] == 0u, 0)) {
+          const int idx = atomicAdd(&ctl[0], 1);
+          if (idx < kTouchCap) touched[idx] = (uint32_t)(jj[u] - lo) >> 5;
+        }
+      }
+    }
   }
   if (lng) {
     for (int t = 0; t < m; t++) {
