@@ -568,20 +568,43 @@ int64_t stats_part_elems() { return (int64_t)kNumBins * kStatFields * kStatsMaxB
 
 // Structural pre-pass: products f_i (k_flops of the reference), output span
 // [lo, hi] from the B rows' min/max columns, symbolic bin, bin statistics.
+constexpr int kPrepassLanes = 4, kPrepassLong = 64;
 __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
                           const int4* __restrict__ bmeta, int upper, BinParams P,
                           int64_t* __restrict__ rowf, int32_t* __restrict__ rowlo, int32_t* __restrict__ rowhi,
                           uint8_t* __restrict__ bin, int32_t* __restrict__ part,
                           unsigned long long* __restrict__ prodpart) {
-  constexpr int G = 16;
+  // kPrepassLanes lanes per row (more rows in flight, short reductions);
+  // rows of more than kPrepassLong entries are left to the whole wave
+  constexpr int G = kPrepassLanes;
   __shared__ BinStatLds st;
   __shared__ unsigned long long red[16];
   stats_init(st);
-  const int lane = threadIdx.x % G;
+  const int lane = threadIdx.x % G, wl = threadIdx.x % kWave;
   const int64_t gpb = blockDim.x / G;
   unsigned long long myprod = 0;
+  (void)upper;
+  // one row's bin from its reduced products and span; `mine`: the lane that
+  // writes the row
+  auto finish = [&](int64_t i, int64_t f, int lo, int hi, int nA, bool mine) {
+    // bitmap windows start on a 32-column boundary, so a window word and a
+    // word-hash key (j >> 5) cover the same columns and distinct-word counts
+    // agree between accumulator modes
+    if (lo <= hi) lo &= ~31;
+    const int64_t words = span_words(lo, hi);
+    int64_t stat;
+    const int b = sym_bin_of(P, f, words, (int64_t)nA, &stat);
+    if (mine) {
+      rowf[i] = f;
+      rowlo[i] = lo;
+      rowhi[i] = hi;
+      bin[i] = (uint8_t)b;
+      myprod += (unsigned long long)f;
+    }
+    return std::pair<int, int64_t>(b, stat);
+  };
   // every wave runs the same number of iterations so the wave-wide statistics
-  // update below is executed by converged waves
+  // updates below are executed by converged waves
   const int64_t stride = (int64_t)gridDim.x * gpb;
   const int64_t i0 = (int64_t)blockIdx.x * gpb + threadIdx.x / G;
   const int64_t wave_first = (int64_t)blockIdx.x * gpb + (threadIdx.x / kWave) * (kWave / G);
@@ -593,6 +616,9 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     if (live) {
       s = ap[i];
       e = ap[i + 1];
+    }
+    const bool lng = live && e - s > kPrepassLong;
+    if (live && !lng) {
       for (int p = s + lane; p < e; p += G) {
         const int4 bm = bmeta[ac[p]];
         const int len = bm.y - bm.x;
@@ -606,23 +632,33 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     f = gsum<G, int64_t>(f, nullptr);
     lo = gmin<G>(lo);
     hi = gmax<G>(hi);
-    (void)upper;
-    // bitmap windows start on a 32-column boundary, so a window word and a
-    // word-hash key (j >> 5) cover the same columns and distinct-word counts
-    // agree between accumulator modes
-    if (lo <= hi) lo &= ~31;
-    const int64_t words = span_words(lo, hi);
-    int64_t stat;
-    const int b = sym_bin_of(P, f, words, (int64_t)(e - s), &stat);
-    const bool leader = live && lane == 0;
-    if (leader) {
-      rowf[i] = f;
-      rowlo[i] = lo;
-      rowhi[i] = hi;
-      bin[i] = (uint8_t)b;
-      myprod += (unsigned long long)f;
+    const bool leader = live && !lng && lane == 0;
+    const auto bs = finish(i, f, lo, hi, e - s, leader);
+    stats_add_wave(st, leader ? bs.first : -1, f, bs.second);
+    // this wave's long rows, one at a time with all 64 lanes
+    uint64_t longs = __ballot(lng && lane == 0);
+    while (longs) {
+      const int src = __builtin_ctzll(longs);
+      longs &= longs - 1ull;
+      const int64_t li = __shfl((long long)i, src, kWave);
+      const int ls = __shfl(s, src, kWave), le = __shfl(e, src, kWave);
+      int64_t lf = 0;
+      int llo = INT32_MAX, lhi = -1;
+      for (int p = ls + wl; p < le; p += kWave) {
+        const int4 bm = bmeta[ac[p]];
+        const int len = bm.y - bm.x;
+        lf += len;
+        if (len > 0) {
+          llo = bm.z < llo ? bm.z : llo;
+          lhi = bm.w > lhi ? bm.w : lhi;
+        }
+      }
+      lf = gsum<kWave, int64_t>(lf, nullptr);
+      llo = gmin<kWave>(llo);
+      lhi = gmax<kWave>(lhi);
+      const auto lb = finish(li, lf, llo, lhi, le - ls, wl == 0);
+      stats_add_wave(st, wl == 0 ? lb.first : -1, lf, lb.second);
     }
-    stats_add_wave(st, leader ? b : -1, f, stat);
   }
   // block reduction of the product count
 #pragma unroll
@@ -637,7 +673,7 @@ void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4*
                     const BinParams& P, int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin,
                     int32_t* stats, unsigned long long* total_products, int32_t* part, hipStream_t s) {
   if (n <= 0) return;
-  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  int64_t blocks = ((int64_t)n * kPrepassLanes + 255) / 256;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   unsigned long long* prodpart = reinterpret_cast<unsigned long long*>(part + stats_part_elems());
   hipLaunchKernelGGL(k_prepass, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, bmeta, upper, P, rowf, rowlo,
