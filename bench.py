@@ -117,10 +117,59 @@ def cpu_baseline(m, op: str, target_s: float) -> dict:
     for _ in range(reps):
         oracle.rows_range(m.n, m.rowptr, m.col, m.val, 0, m.n, op, threads, b)
     dt = time.perf_counter() - t0
+    # one thread on a leading row sample of about a tenth of the work
+    r1 = int(np.searchsorted(np.cumsum(f_all), max(prods // 10, 1))) + 1
+    r1 = min(max(r1, 1), m.n)
+    p1 = int(f_all[:r1].sum())
+    t1 = time.perf_counter()
+    oracle.rows_range(m.n, m.rowptr, m.col, m.val, 0, r1, op, 1, b)
+    d1 = max(time.perf_counter() - t1, 1e-6)
     return {"value": round(2.0 * prods * reps / dt / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
             "kind": "port", "ms_per_call": round(dt * 1e3 / reps, 3),
+            "value_1thread": round(2.0 * p1 / d1 / 1e9, 4), "cpu_model": cpu_model(),
             "sample": f"the whole {op} product ({m.n} rows, {prods} products) x {reps} calls, {dt:.2f} s, "
-                      f"oracle/oracle.c Gustavson (symbolic + numeric passes), OpenMP dynamic rows"}
+                      f"oracle/oracle.c Gustavson (symbolic + numeric passes), OpenMP dynamic rows; "
+                      f"1 thread: rows [0, {r1}) ({p1} products), {d1:.2f} s"}
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def copy_bandwidth(mib: int = 1024, reps: int = 5) -> float:
+    """Achievable HBM bandwidth: a device-to-device hipMemcpy of `mib` MiB
+    (read + write counted), timed with HIP events on the null stream; GB/s.
+    Calls the HIP runtime the engine library already loaded."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = mib * (1 << 20)
+    x, y, e0, e1 = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+    ok = hip.hipMalloc(C.byref(x), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(y), C.c_size_t(nbytes)) == 0
+    ms = C.c_float(0.0)
+    if ok:
+        hip.hipMemset(x, 0, C.c_size_t(nbytes))
+        hip.hipEventCreate(C.byref(e0))
+        hip.hipEventCreate(C.byref(e1))
+        hip.hipMemcpy(y, x, C.c_size_t(nbytes), 3)  # device to device
+        hip.hipEventRecord(e0, None)
+        for _ in range(reps):
+            hip.hipMemcpyAsync(y, x, C.c_size_t(nbytes), 3, None)
+        hip.hipEventRecord(e1, None)
+        hip.hipEventSynchronize(e1)
+        hip.hipEventElapsedTime(C.byref(ms), e0, e1)
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    if x:
+        hip.hipFree(x)
+    if y:
+        hip.hipFree(y)
+    return 2.0 * nbytes / (ms.value / reps * 1e-3) / 1e9 if ms.value > 0 else float("nan")
 
 
 def gen_parallel(fns, workers: int = 16):
@@ -416,10 +465,13 @@ def main() -> int:
         R.release()
     eng.set_timing(True)
     phase_sum: dict[str, float] = {}
+    step_ms = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         R = eng.compute(A, args.op)
         R.release()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
         pt = eng.phase_times()
         for k in ("transpose_ms", "prepass_ms", "symbolic_ms", "scan_ms", "numeric_ms", "mirror_ms",
                   "total_ms"):
@@ -443,6 +495,7 @@ def main() -> int:
         roc["GFLOPs"] = round(2.0 * products / (roc["ms"] * 1e-3) / 1e9, 3)
         roc["nnz_c_matches"] = roc["nnz_c"] == nnz_c
     cpu = None if args.no_cpu else cpu_baseline(m, args.op, args.cpu_seconds)
+    copy_gbs = copy_bandwidth()
     out = {
         "metric": "SpGEMM GFLOP/s (FP64, structural C=A*A) with HBM GB/s vs roofline",
         "value": round(gflops, 3),
@@ -457,12 +510,14 @@ def main() -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": desc, "op": args.op, "n": m.n, "nnz_a": m.nnz, "nnz_c": nnz_c,
+                   "median_ms_per_step": round(float(np.median(step_ms)), 4),
                    "products": products, "flops": 2 * products, "bytes_alg": b_alg,
                    "call_GBps": round(call_gbs, 2), "call_roofline_frac": round(call_gbs / HBM_PEAK_GBS, 4),
                    "phase_ms": phases, "rocsparse": roc},
         "roofline": {"bound": "hbm", "kernel": dom.replace("_ms", ""), "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": pmc_traffic(args.workload, args.op) if dom == "numeric_ms" else None},
+                     "traffic": pmc_traffic(args.workload, args.op) if dom == "numeric_ms" else None,
+                     "copy_peak": round(copy_gbs, 1), "frac_of_copy": round(achieved / copy_gbs, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
