@@ -128,6 +128,8 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
       if (t->group_for_len[g] > 0) c.sym.glen_max[g] = c.num.glen_max[g] = t->group_for_len[g];
     if (t->wg_lds_bytes > 0) wg = std::min<int32_t>(wg, t->wg_lds_bytes);
     if (t->heavy_window_cols > 0) c.heavy_cap = t->heavy_window_cols;
+    if (t->transpose_radix_min_nnz > 0) c.radix_min_nnz = t->transpose_radix_min_nnz;
+    if (t->transpose_radix_min_nnz < 0) c.radix_min_nnz = INT64_MAX;
     if (t->sort_span_factor != 0) c.num.sort_ratio = t->sort_span_factor > 0 ? t->sort_span_factor : 0;
     if (t->sort_span_factor_symbolic != 0)
       c.sym.sort_ratio = t->sort_span_factor_symbolic > 0 ? t->sort_span_factor_symbolic : 0;
@@ -256,7 +258,21 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
     int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
     int32_t* slist = as<int32_t>(ensure(W.sortlist, sizeof(int32_t) * (size_t)sort_rows_scratch_elems(n)));
     int32_t* asym = ctl + 3;
-    launch_transpose(n, nnz, A.rowptr, A.col, A.val, tp, tc, tv, cnt, off64, tmp, slist, asym, s);
+    RadixScratch rx{};
+    const bool radix = nnz >= cfg_.radix_min_nnz;
+    if (radix) {
+      const size_t nz = (size_t)std::max<int64_t>(nnz, 1);
+      const int64_t hn = ((int64_t)1 << radix_bits(n)) * radix_tiles(nnz);
+      rx.keys = as<int32_t>(ensure(W.rkeys, sizeof(int32_t) * nz));
+      rx.rows = as<int32_t>(ensure(W.rrows, sizeof(int32_t) * nz));
+      rx.vals = as<double>(ensure(W.rvals, sizeof(double) * nz));
+      rx.keys2 = as<int32_t>(ensure(W.rkeys2, sizeof(int32_t) * nz));
+      rx.hist = as<int32_t>(ensure(W.rhist, sizeof(int32_t) * (size_t)std::max<int64_t>(hn, 1)));
+      rx.hoff = as<int64_t>(ensure(W.rhoff, sizeof(int64_t) * (size_t)(hn + 1)));
+      rx.htmp = as<int64_t>(ensure(W.rtmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max<int64_t>(hn, 1))));
+    }
+    launch_transpose(n, nnz, A.rowptr, A.col, A.val, tp, tc, tv, cnt, off64, tmp, slist, radix ? &rx : nullptr,
+                     asym, s);
     O.bc = tc;
     O.bv = tv;
     bptr = tp;

@@ -62,18 +62,20 @@ struct Scratch {
 struct Config {
   BinParams sym{}, num{};
   int32_t heavy_cap = 0;  // column-window cap of the heavy-row bin (0: as wide as LDS allows)
+  int64_t radix_min_nnz = 1 << 18;  // asymmetric transposes of at least this many entries use the radix sort
 };
 
 // Device scratch of one pipeline.  Buffers only grow, so a HIP graph that
 // captured their addresses stays valid while the owner lives.
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
-      numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist;
+      numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
+      rhist, rhoff, rtmp;
   template <typename F>
   void each(F&& f) {
     Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
-                      &cur,    &sortlist};
+                      &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp};
     for (Scratch* s : all) f(*s);
   }
 };

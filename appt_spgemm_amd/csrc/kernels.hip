@@ -322,9 +322,126 @@ void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, i
                      gate);
 }
 
+// ---------------------------------------------------------------------------
+// Radix transpose: a stable LSD counting sort of A's entries by column, 8
+// bits per pass.  Entries come in row-major order, so a stable sort leaves
+// each column's entries (B's row) ordered by row id.  One wave per tile of
+// kRadixTile entries: a per-tile digit histogram (digit-major over tiles),
+// one exclusive scan of all histograms, then each wave walks its tile in
+// order, ranking equal digits inside a 64-entry chunk with ballots (lanes of
+// equal digit found bit by bit) and a per-digit running count in LDS.  No
+// global atomics; B's row pointers are the boundaries of the sorted keys.
+// ---------------------------------------------------------------------------
+int64_t radix_tiles(int64_t nnz) { return (nnz + kRadixTile - 1) / kRadixTile; }
+static int key_bits(int32_t n) { return n > 1 ? 32 - __builtin_clz((unsigned)(n - 1)) : 1; }
+int radix_passes(int32_t n) { return (key_bits(n) + kRadixMaxBits - 1) / kRadixMaxBits; }
+int radix_bits(int32_t n) {
+  const int p = radix_passes(n);
+  return (key_bits(n) + p - 1) / p;
+}
+
+__global__ void k_row_ids(int32_t n, const int32_t* __restrict__ ap, int32_t* __restrict__ rid,
+                          const int32_t* gate) {
+  if (gated_off(gate)) return;
+  constexpr int G = 16;
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb)
+    for (int p = ap[i] + lane; p < ap[i + 1]; p += G) rid[p] = (int32_t)i;
+}
+
+constexpr int kRadixBuckets = 1 << kRadixMaxBits;
+
+__global__ void __launch_bounds__(256) k_radix_hist(int64_t nnz, const int32_t* __restrict__ keys, int shift, int bits,
+                                                    int64_t ntiles, int32_t* __restrict__ hist, const int32_t* gate) {
+  if (gated_off(gate)) return;
+  __shared__ int32_t cnt[4][kRadixBuckets];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + w;
+  if (tile >= ntiles) return;
+  const int nb = 1 << bits;
+  const uint32_t dm = (uint32_t)nb - 1u;
+  for (int d = lane; d < nb; d += kWave) cnt[w][d] = 0;
+  gsync<kWave>();
+  const int64_t base = tile * kRadixTile;
+  for (int c = 0; c < kRadixChunks; c++) {
+    const int64_t e = base + (int64_t)c * kWave + lane;
+    if (e < nnz) atomicAdd(&cnt[w][((uint32_t)keys[e] >> shift) & dm], 1);
+  }
+  gsync<kWave>();
+  for (int d = lane; d < nb; d += kWave) hist[(int64_t)d * ntiles + tile] = cnt[w][d];
+}
+
+__global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, int bits, int64_t ntiles,
+                                                       const int64_t* __restrict__ hoff,
+                                                       const int32_t* __restrict__ kin, const int32_t* __restrict__ rin,
+                                                       const double* __restrict__ vin, int32_t* __restrict__ kout,
+                                                       int32_t* __restrict__ rout, double* __restrict__ vout,
+                                                       const int32_t* gate) {
+  if (gated_off(gate)) return;
+  // running output position per digit (positions < nnz < 2^31)
+  __shared__ int32_t run[4][kRadixBuckets];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t tile = (int64_t)blockIdx.x * 4 + w;
+  if (tile >= ntiles) return;
+  const int nb = 1 << bits;
+  const uint32_t dm = (uint32_t)nb - 1u;
+  for (int d = lane; d < nb; d += kWave) run[w][d] = (int32_t)hoff[(int64_t)d * ntiles + tile];
+  gsync<kWave>();
+  const int64_t base = tile * kRadixTile;
+  const uint64_t below = (1ull << lane) - 1ull;
+  constexpr int U = 4;  // chunks whose entries are loaded together
+  for (int c0 = 0; c0 < kRadixChunks; c0 += U) {
+    int32_t kk[U], rr[U];
+    double vv[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int64_t e = base + (int64_t)(c0 + u) * kWave + lane;
+      const bool valid = e < nnz;
+      kk[u] = valid ? kin[e] : -1;
+      rr[u] = valid ? rin[e] : 0;
+      vv[u] = valid ? vin[e] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const bool valid = kk[u] >= 0;
+      const uint32_t digit = ((uint32_t)kk[u] >> shift) & dm;
+      // lanes holding the same digit: intersect the ballots of every digit bit
+      uint64_t peers = __ballot(valid);
+      for (int b = 0; b < bits; b++) {
+        const bool bit = (digit >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+      }
+      const int rank = __popcll(peers & below);
+      const int32_t pos = run[w][digit] + rank;
+      gsync<kWave>();  // every lane has read its digit's count before it moves
+      if (valid && rank == 0) run[w][digit] += __popcll(peers);
+      gsync<kWave>();
+      if (valid) {
+        kout[pos] = kk[u];
+        rout[pos] = rr[u];
+        vout[pos] = vv[u];
+      }
+    }
+  }
+}
+
+// B's row pointers from the sorted keys: tp[k] = first position whose key is
+// >= k (positions p in [0, nnz] fill the columns since the previous key)
+__global__ void k_radix_rowptr(int32_t n, int64_t nnz, const int32_t* __restrict__ keys, int32_t* __restrict__ tp,
+                               const int32_t* gate) {
+  if (gated_off(gate)) return;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p <= nnz; p += (int64_t)gridDim.x * blockDim.x) {
+    const int kp = p < nnz ? keys[p] : n;
+    const int kq = p > 0 ? keys[p - 1] : -1;
+    for (int k = kq + 1; k <= kp; k++) tp[k] = (int32_t)p;
+  }
+}
+
 void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
                       int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
-                      int32_t* slist, int32_t* asym, hipStream_t s) {
+                      int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
@@ -332,7 +449,42 @@ void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* 
   int64_t b0 = ((int64_t)n + 256) / 256;
   if (b0 > 4096) b0 = 4096;
   hipLaunchKernelGGL(k_rowptr_if_symmetric, dim3((unsigned)b0), dim3(256), 0, s, n, ap, tp, asym);
-  // asymmetric: column histogram, scan, atomic-cursor scatter, row sort
+  if (rx) {
+    // asymmetric, large: radix passes; the last one writes (keys, bc, bv)
+    const int P = radix_passes(n), bits = radix_bits(n);
+    const int64_t nt = radix_tiles(nnz);
+    const unsigned tb = (unsigned)((nt + 3) / 4);
+    int32_t* xk = rx->keys;
+    int32_t* xr = rx->rows;
+    double* xv = rx->vals;
+    int32_t* yk = rx->keys2;
+    // the row ids go to the rows buffer the first pass does not write
+    int32_t* rid = (P % 2 == 1) ? xr : bc;
+    hipLaunchKernelGGL(k_row_ids, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, rid, asym);
+    const int32_t* sk = ac;
+    const int32_t* sr = rid;
+    const double* sv = av;
+    for (int p = 0; p < P; p++) {
+      const bool to_y = (P - 1 - p) % 2 == 0;
+      int32_t* dk = to_y ? yk : xk;
+      int32_t* dr = to_y ? bc : xr;
+      double* dv = to_y ? bv : xv;
+      if (nt > 0) {
+        hipLaunchKernelGGL(k_radix_hist, dim3(tb), dim3(256), 0, s, nnz, sk, p * bits, bits, nt, rx->hist, asym);
+        launch_scan(rx->hist, ((int64_t)1 << bits) * nt, rx->hoff, nullptr, rx->htmp, s, asym);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(tb), dim3(256), 0, s, nnz, p * bits, bits, nt, rx->hoff, sk, sr, sv,
+                           dk, dr, dv, asym);
+      }
+      sk = dk;
+      sr = dr;
+      sv = dv;
+    }
+    int64_t b1 = (nnz + 256) / 256;
+    if (b1 > 8192) b1 = 8192;
+    hipLaunchKernelGGL(k_radix_rowptr, dim3((unsigned)b1), dim3(256), 0, s, n, nnz, (const int32_t*)yk, tp, asym);
+    return;
+  }
+  // asymmetric, small: column histogram, scan, atomic-cursor scatter, row sort
   launch_fill_i32(cnt, n, 0, s, asym);
   launch_colcount(nnz, ac, cnt, s, asym);
   launch_scan(cnt, n, off64, tp, tmp, s, asym);

@@ -79,14 +79,35 @@ void launch_csr_check(int32_t n, int64_t nnz, const int32_t* ptr, const int32_t*
 // flag[0] = 1 (zero beforehand) when some row of (ptr, col) is not strictly
 // increasing
 void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);
+// Radix transpose: digits of at most kRadixMaxBits bits (the fewest passes
+// that cover the column range, bits split evenly), one wave per tile of
+// kRadixTile entries.
+constexpr int kRadixMaxBits = 8, kRadixChunks = 8;
+constexpr int64_t kRadixTile = 64 * kRadixChunks;
+int64_t radix_tiles(int64_t nnz);
+int radix_passes(int32_t n);
+int radix_bits(int32_t n);  // digit bits per pass
+// Scratch of the radix transpose: keys / rows / vals (nnz each) for the
+// ping-pong passes, keys2 (nnz) for the final keys, hist (2^bits x tiles
+// int32), hoff (2^bits x tiles + 1 int64) and htmp (scan_tmp_elems of that).
+struct RadixScratch {
+  int32_t* keys;
+  int32_t* rows;
+  double* vals;
+  int32_t* keys2;
+  int32_t* hist;
+  int64_t* hoff;
+  int64_t* htmp;
+};
 // B = A^T with sorted rows into (tp, bc, bv), decided on the device: a
 // symmetric-pattern gather in A's layout (tp = ap); when A is not strictly
-// sorted with a symmetric pattern it sets *asym (zero beforehand) and the
+// sorted with a symmetric pattern it sets *asym (zero beforehand) and B is
+// built by a stable radix sort of A's entries by column (rx given), or by the
 // column histogram (cnt, n + 1 int32), scan (off64, tmp), atomic-cursor
-// scatter and row sort (slist) run instead.
+// scatter and row sort (slist) otherwise.
 void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
                       int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
-                      int32_t* slist, int32_t* asym, hipStream_t s);
+                      int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s);
 // Atomic-cursor scatter; gate (optional): runs only when *gate != 0.
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
                               int32_t* cursor, int32_t* bc, double* bv, const int32_t* gate, hipStream_t s);

@@ -93,7 +93,9 @@ typedef struct appt_tuning {
                                  words, even where a window fits (0 = default, < 0 = only rows no
                                  window fits) */
   int32_t sort_span_factor_symbolic; /* symbolic pass: the same by span alone (0 = default, < 0 = off) */
-  int32_t reserved[5];
+  int32_t transpose_radix_min_nnz; /* asymmetric A^T of at least this many entries by radix sort (0 = default
+                                      262,144; < 0 = never) */
+  int32_t reserved[4];
 } appt_tuning;
 
 typedef struct appt_options {

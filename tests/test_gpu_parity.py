@@ -193,6 +193,27 @@ def test_scattered_rows_wide_columns(eng):
         assert_product(got, ref(m, op), exact=True)
 
 
+@pytest.fixture(scope="module")
+def eng_radix():
+    # every asymmetric transpose through the radix sort, however small
+    e = ap.Engine(0, tuning={"transpose_radix_min_nnz": 1})
+    yield e
+    e.close()
+
+
+@pytest.mark.parametrize("op", ["AAt", "AAtS"])
+def test_radix_transpose(eng_radix, op):
+    cases = [(name, m) for name, m in edge_matrices()]
+    cases += [("stencil_drop", ap.stencil27(14, 0.1, 5)), ("er_300", ap.erdos_renyi(300, 9.0, 4)),
+              ("er_70000", ap.erdos_renyi(70_000, 3.0, 5)), ("banded", ap.banded(3000, 8, 0.7, 6)),
+              ("rmat12", ap.rmat(12, 16, seed=3)), ("heavy_unsorted", shuffled_rows(ap.rmat(11, 16, seed=2), 3))]
+    for name, m in cases:
+        say("case radix transpose", name, op)
+        got = eng_radix.multiply(m, op)
+        exact = name not in ("duplicates", "rmat12", "heavy_unsorted")
+        assert_product(got, ref(m, op), exact=exact)
+
+
 def edge_matrices():
     yield "n0", host(0, [0], [], [])
     yield "n1", host(1, [0, 1], [0], [3.0])
