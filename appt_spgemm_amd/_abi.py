@@ -45,7 +45,8 @@ class Tuning(C.Structure):
                 ("lds_budget_bytes", C.c_int32), ("bitmap_words_factor", C.c_int32),
                 ("group_for_len", C.c_int32 * 4), ("heavy_window_cols", C.c_int32),
                 ("sort_span_factor", C.c_int32), ("sort_span_factor_symbolic", C.c_int32),
-                ("transpose_radix_min_nnz", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("transpose_radix_min_nnz", C.c_int32), ("mirror_radix_min_nnz", C.c_int32),
+                ("reserved", C.c_int32 * 3)]
 
 
 class Options(C.Structure):

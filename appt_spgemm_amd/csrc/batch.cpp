@@ -24,6 +24,7 @@ appt_tuning tuning_of(const Config& c) {
   t.sort_span_factor = c.num.sort_ratio > 0 ? c.num.sort_ratio : -1;
   t.sort_span_factor_symbolic = c.sym.sort_ratio > 0 ? c.sym.sort_ratio : -1;
   t.transpose_radix_min_nnz = c.radix_min_nnz > INT32_MAX ? -1 : (int32_t)c.radix_min_nnz;
+  t.mirror_radix_min_nnz = c.radix_mirror_min_nnz > INT32_MAX ? -1 : (int32_t)c.radix_mirror_min_nnz;
   return t;
 }
 

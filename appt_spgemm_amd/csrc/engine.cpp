@@ -130,6 +130,8 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
     if (t->heavy_window_cols > 0) c.heavy_cap = t->heavy_window_cols;
     if (t->transpose_radix_min_nnz > 0) c.radix_min_nnz = t->transpose_radix_min_nnz;
     if (t->transpose_radix_min_nnz < 0) c.radix_min_nnz = INT64_MAX;
+    if (t->mirror_radix_min_nnz > 0) c.radix_mirror_min_nnz = t->mirror_radix_min_nnz;
+    if (t->mirror_radix_min_nnz < 0) c.radix_mirror_min_nnz = INT64_MAX;
     if (t->sort_span_factor != 0) c.num.sort_ratio = t->sort_span_factor > 0 ? t->sort_span_factor : 0;
     if (t->sort_span_factor_symbolic != 0)
       c.sym.sort_ratio = t->sort_span_factor_symbolic > 0 ? t->sort_span_factor_symbolic : 0;
@@ -447,7 +449,39 @@ void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, c
 
 void Engine::stage_mirror(Workspace& W, const Operands& O, const Plan& P, const int32_t* ccol, double* cval) {
   // AAtS: lower entries take the value of their upper mirror image
-  if (O.upper && !P.direct_sym) launch_mirror_pull(O.n, P.nnz, as<int64_t>(W.rowoff.p), ccol, cval, stream_);
+  if (!O.upper || P.direct_sym) return;
+  const int32_t n = O.n;
+  const int64_t* off = as<int64_t>(W.rowoff.p);
+  if (P.nnz < cfg_.radix_mirror_min_nnz) {
+    launch_mirror_pull(n, P.nnz, off, ccol, cval, stream_);
+    return;
+  }
+  // large results: the strictly-upper entries sorted by column into the lower halves
+  hipStream_t s = stream_;
+  const size_t nn = (size_t)std::max(n, 1);
+  int32_t* lower = as<int32_t>(ensure(W.mlow, sizeof(int32_t) * nn));
+  int32_t* upper = as<int32_t>(ensure(W.mup, sizeof(int32_t) * nn));
+  int64_t* lo_off = as<int64_t>(ensure(W.mlo_off, sizeof(int64_t) * (nn + 1)));
+  int64_t* up_off = as<int64_t>(ensure(W.mup_off, sizeof(int64_t) * (nn + 1)));
+  int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max(n, 1))));
+  launch_mirror_counts(n, off, ccol, lower, upper, s);
+  launch_scan(lower, n, lo_off, nullptr, tmp, s);
+  launch_scan(upper, n, up_off, nullptr, tmp, s);
+  APPT_HIP(hipMemcpyAsync(&hs_->total, up_off + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  APPT_HIP(hipStreamSynchronize(s));
+  const int64_t count = hs_->total;
+  if (count <= 0) return;
+  const size_t nz = (size_t)count;
+  const int64_t hn = ((int64_t)1 << radix_bits(n)) * radix_tiles(count);
+  RadixScratch rx{};
+  rx.keys = as<int32_t>(ensure(W.rkeys, sizeof(int32_t) * nz));
+  rx.vals = as<double>(ensure(W.rvals, sizeof(double) * nz));
+  rx.keys2 = as<int32_t>(ensure(W.rkeys2, sizeof(int32_t) * nz));
+  rx.hist = as<int32_t>(ensure(W.rhist, sizeof(int32_t) * (size_t)std::max<int64_t>(hn, 1)));
+  rx.hoff = as<int64_t>(ensure(W.rhoff, sizeof(int64_t) * (size_t)(hn + 1)));
+  rx.htmp = as<int64_t>(ensure(W.rtmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max<int64_t>(hn, 1))));
+  double* vals2 = as<double>(ensure(W.rvals2, sizeof(double) * nz));
+  launch_mirror_radix(n, count, off, ccol, cval, upper, up_off, lo_off, rx, vals2, s);
 }
 
 // ---------------------------------------------------------------------------

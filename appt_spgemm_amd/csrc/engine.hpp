@@ -63,6 +63,7 @@ struct Config {
   BinParams sym{}, num{};
   int32_t heavy_cap = 0;  // column-window cap of the heavy-row bin (0: as wide as LDS allows)
   int64_t radix_min_nnz = 1 << 18;  // asymmetric transposes of at least this many entries use the radix sort
+  int64_t radix_mirror_min_nnz = 1 << 20;  // AAtS mirrors of results this large use the radix sort
 };
 
 // Device scratch of one pipeline.  Buffers only grow, so a HIP graph that
@@ -70,12 +71,13 @@ struct Config {
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
       numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
-      rhist, rhoff, rtmp;
+      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off;
   template <typename F>
   void each(F&& f) {
     Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
-                      &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp};
+                      &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp,
+                      &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off};
     for (Scratch* s : all) f(*s);
   }
 };

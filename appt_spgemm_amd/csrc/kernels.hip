@@ -399,7 +399,7 @@ __global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, i
       const int64_t e = base + (int64_t)(c0 + u) * kWave + lane;
       const bool valid = e < nnz;
       kk[u] = valid ? kin[e] : -1;
-      rr[u] = valid ? rin[e] : 0;
+      rr[u] = (valid && rin) ? rin[e] : 0;
       vv[u] = valid ? vin[e] : 0.0;
     }
 #pragma unroll
@@ -420,7 +420,7 @@ __global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, i
       gsync<kWave>();
       if (valid) {
         kout[pos] = kk[u];
-        rout[pos] = rr[u];
+        if (rout) rout[pos] = rr[u];
         vout[pos] = vv[u];
       }
     }
@@ -439,6 +439,36 @@ __global__ void k_radix_rowptr(int32_t n, int64_t nnz, const int32_t* __restrict
   }
 }
 
+// The radix passes over `count` entries with keys in [0, n): input (k0, r0,
+// v0), ping-pong between the x and y arrays, ending in the y arrays (r*
+// null: no row payload).  The input may share storage only with the arrays
+// the first pass does not write: the x arrays when the pass count is odd,
+// the y arrays when it is even.
+static void radix_passes_run(int32_t n, int64_t count, const int32_t* k0, const int32_t* r0, const double* v0,
+                             int32_t* xk, int32_t* xr, double* xv, int32_t* yk, int32_t* yr, double* yv,
+                             int32_t* hist, int64_t* hoff, int64_t* htmp, const int32_t* gate, hipStream_t s) {
+  const int P = radix_passes(n), bits = radix_bits(n);
+  const int64_t nt = radix_tiles(count);
+  if (nt <= 0) return;
+  const unsigned tb = (unsigned)((nt + 3) / 4);
+  const int32_t* sk = k0;
+  const int32_t* sr = r0;
+  const double* sv = v0;
+  for (int p = 0; p < P; p++) {
+    const bool to_y = (P - 1 - p) % 2 == 0;
+    int32_t* dk = to_y ? yk : xk;
+    int32_t* dr = to_y ? yr : xr;
+    double* dv = to_y ? yv : xv;
+    hipLaunchKernelGGL(k_radix_hist, dim3(tb), dim3(256), 0, s, count, sk, p * bits, bits, nt, hist, gate);
+    launch_scan(hist, ((int64_t)1 << bits) * nt, hoff, nullptr, htmp, s, gate);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(tb), dim3(256), 0, s, count, p * bits, bits, nt, (const int64_t*)hoff,
+                       sk, sr, sv, dk, dr, dv, gate);
+    sk = dk;
+    sr = dr;
+    sv = dv;
+  }
+}
+
 void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
                       int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
                       int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s) {
@@ -450,38 +480,16 @@ void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* 
   if (b0 > 4096) b0 = 4096;
   hipLaunchKernelGGL(k_rowptr_if_symmetric, dim3((unsigned)b0), dim3(256), 0, s, n, ap, tp, asym);
   if (rx) {
-    // asymmetric, large: radix passes; the last one writes (keys, bc, bv)
-    const int P = radix_passes(n), bits = radix_bits(n);
-    const int64_t nt = radix_tiles(nnz);
-    const unsigned tb = (unsigned)((nt + 3) / 4);
-    int32_t* xk = rx->keys;
-    int32_t* xr = rx->rows;
-    double* xv = rx->vals;
-    int32_t* yk = rx->keys2;
-    // the row ids go to the rows buffer the first pass does not write
-    int32_t* rid = (P % 2 == 1) ? xr : bc;
+    // asymmetric, large: radix passes; the last one writes (keys2, bc, bv).
+    // The row ids go to the rows buffer the first pass does not write.
+    int32_t* rid = (radix_passes(n) % 2 == 1) ? rx->rows : bc;
     hipLaunchKernelGGL(k_row_ids, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, rid, asym);
-    const int32_t* sk = ac;
-    const int32_t* sr = rid;
-    const double* sv = av;
-    for (int p = 0; p < P; p++) {
-      const bool to_y = (P - 1 - p) % 2 == 0;
-      int32_t* dk = to_y ? yk : xk;
-      int32_t* dr = to_y ? bc : xr;
-      double* dv = to_y ? bv : xv;
-      if (nt > 0) {
-        hipLaunchKernelGGL(k_radix_hist, dim3(tb), dim3(256), 0, s, nnz, sk, p * bits, bits, nt, rx->hist, asym);
-        launch_scan(rx->hist, ((int64_t)1 << bits) * nt, rx->hoff, nullptr, rx->htmp, s, asym);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(tb), dim3(256), 0, s, nnz, p * bits, bits, nt, rx->hoff, sk, sr, sv,
-                           dk, dr, dv, asym);
-      }
-      sk = dk;
-      sr = dr;
-      sv = dv;
-    }
+    radix_passes_run(n, nnz, ac, rid, av, rx->keys, rx->rows, rx->vals, rx->keys2, bc, bv, rx->hist, rx->hoff,
+                     rx->htmp, asym, s);
     int64_t b1 = (nnz + 256) / 256;
     if (b1 > 8192) b1 = 8192;
-    hipLaunchKernelGGL(k_radix_rowptr, dim3((unsigned)b1), dim3(256), 0, s, n, nnz, (const int32_t*)yk, tp, asym);
+    hipLaunchKernelGGL(k_radix_rowptr, dim3((unsigned)b1), dim3(256), 0, s, n, nnz, (const int32_t*)rx->keys2, tp,
+                       asym);
     return;
   }
   // asymmetric, small: column histogram, scan, atomic-cursor scatter, row sort
@@ -2988,6 +2996,76 @@ __global__ void __launch_bounds__(256) k_mirror_long(int32_t n, int64_t nnz, con
     if (r >= c) continue;         // upper part: formed by the numeric pass
     cval[p] = mirror_value(off, ccol, cval, r, c);
   }
+}
+
+// Radix mirror.  Strictly-upper entries (j, i), i > j, taken in row-major
+// order and stably sorted by column i, come out grouped by i with j
+// ascending: exactly row i's lower part, which starts at off[i].
+// lower[i] / upper[i]: row i's entries left / right of the diagonal.
+__global__ void k_mirror_counts(int32_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ ccol,
+                                int32_t* __restrict__ lower, int32_t* __restrict__ upper) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = off[i], hi = off[i + 1];
+    const int64_t s = lo, e = hi;
+    while (lo < hi) {  // first column >= i
+      const int64_t mid = (lo + hi) >> 1;
+      if (ccol[mid] < (int32_t)i) lo = mid + 1;
+      else hi = mid;
+    }
+    const int diag = (lo < e && ccol[lo] == (int32_t)i) ? 1 : 0;
+    lower[i] = (int32_t)(lo - s);
+    upper[i] = (int32_t)(e - lo - diag);
+  }
+}
+__global__ void k_mirror_extract(int32_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ ccol,
+                                 const double* __restrict__ cval, const int32_t* __restrict__ upper,
+                                 const int64_t* __restrict__ up_off, int32_t* __restrict__ keys,
+                                 double* __restrict__ vals) {
+  constexpr int G = 16;
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  for (int64_t j = (int64_t)blockIdx.x * gpb + threadIdx.x / G; j < n; j += (int64_t)gridDim.x * gpb) {
+    const int u = upper[j];
+    const int64_t src = off[j + 1] - u, dst = up_off[j];
+    for (int t = lane; t < u; t += G) {
+      keys[dst + t] = ccol[src + t];
+      vals[dst + t] = cval[src + t];
+    }
+  }
+}
+__global__ void k_mirror_place(int64_t count, const int32_t* __restrict__ keys, const double* __restrict__ vals,
+                               const int64_t* __restrict__ lo_off, const int64_t* __restrict__ off,
+                               double* __restrict__ cval) {
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < count; q += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t i = keys[q];
+    cval[off[i] + (q - lo_off[i])] = vals[q];
+  }
+}
+void launch_mirror_counts(int32_t n, const int64_t* off, const int32_t* ccol, int32_t* lower, int32_t* upper,
+                          hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_mirror_counts, dim3((unsigned)blocks), dim3(256), 0, s, n, off, ccol, lower, upper);
+}
+void launch_mirror_radix(int32_t n, int64_t count, const int64_t* off, const int32_t* ccol, double* cval,
+                         const int32_t* upper, const int64_t* up_off, const int64_t* lo_off, const RadixScratch& rx,
+                         double* vals2, hipStream_t s) {
+  if (n <= 0 || count <= 0) return;
+  // extract into the arrays the first pass does not write
+  const bool odd = radix_passes(n) % 2 == 1;
+  int32_t* ek = odd ? rx.keys : rx.keys2;
+  double* ev = odd ? rx.vals : vals2;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(k_mirror_extract, dim3((unsigned)blocks), dim3(256), 0, s, n, off, ccol, (const double*)cval,
+                     upper, up_off, ek, ev);
+  radix_passes_run(n, count, ek, nullptr, ev, rx.keys, nullptr, rx.vals, rx.keys2, nullptr, vals2, rx.hist, rx.hoff,
+                   rx.htmp, nullptr, s);
+  int64_t b2 = (count + 255) / 256;
+  if (b2 > 16384) b2 = 16384;
+  hipLaunchKernelGGL(k_mirror_place, dim3((unsigned)b2), dim3(256), 0, s, count, (const int32_t*)rx.keys2,
+                     (const double*)vals2, lo_off, off, cval);
 }
 
 void launch_mirror_pull(int32_t n, int64_t nnz, const int64_t* off, const int32_t* ccol, double* cval,

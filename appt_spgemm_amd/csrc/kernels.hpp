@@ -108,6 +108,15 @@ struct RadixScratch {
 void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
                       int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
                       int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s);
+// AAtS mirror by radix sort: per-row counts of entries left / right of the
+// diagonal, then (with their exclusive scans lo_off / up_off and the upper
+// total `count`) the strictly-upper entries sorted by column into the
+// lower halves.  rx sized for `count` entries (rows unused); vals2: count f64.
+void launch_mirror_counts(int32_t n, const int64_t* off, const int32_t* ccol, int32_t* lower, int32_t* upper,
+                          hipStream_t s);
+void launch_mirror_radix(int32_t n, int64_t count, const int64_t* off, const int32_t* ccol, double* cval,
+                         const int32_t* upper, const int64_t* up_off, const int64_t* lo_off, const RadixScratch& rx,
+                         double* vals2, hipStream_t s);
 // Atomic-cursor scatter; gate (optional): runs only when *gate != 0.
 void launch_transpose_scatter(int32_t n, const int32_t* ap, const int32_t* ac, const double* av,
                               int32_t* cursor, int32_t* bc, double* bv, const int32_t* gate, hipStream_t s);

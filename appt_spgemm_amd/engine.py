@@ -307,7 +307,8 @@ def _tuning_dict(t: Tuning) -> dict:
             "group_for_len": [int(x) for x in t.group_for_len[:3]], "heavy_window_cols": int(t.heavy_window_cols),
             "sort_span_factor": int(t.sort_span_factor),
             "sort_span_factor_symbolic": int(t.sort_span_factor_symbolic),
-            "transpose_radix_min_nnz": int(t.transpose_radix_min_nnz)}
+            "transpose_radix_min_nnz": int(t.transpose_radix_min_nnz),
+            "mirror_radix_min_nnz": int(t.mirror_radix_min_nnz)}
 
 
 def panels(rowptr: np.ndarray, max_bytes: int) -> list:

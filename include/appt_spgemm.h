@@ -95,7 +95,9 @@ typedef struct appt_tuning {
   int32_t sort_span_factor_symbolic; /* symbolic pass: the same by span alone (0 = default, < 0 = off) */
   int32_t transpose_radix_min_nnz; /* asymmetric A^T of at least this many entries by radix sort (0 = default
                                       262,144; < 0 = never) */
-  int32_t reserved[4];
+  int32_t mirror_radix_min_nnz;    /* AAtS mirror by radix sort when nnz(C) reaches this (0 = default 1,048,576;
+                                      < 0 = never) */
+  int32_t reserved[3];
 } appt_tuning;
 
 typedef struct appt_options {

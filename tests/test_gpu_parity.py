@@ -195,8 +195,9 @@ def test_scattered_rows_wide_columns(eng):
 
 @pytest.fixture(scope="module")
 def eng_radix():
-    # every asymmetric transpose through the radix sort, however small
-    e = ap.Engine(0, tuning={"transpose_radix_min_nnz": 1})
+    # every asymmetric transpose and every AAtS mirror through the radix
+    # sort, however small
+    e = ap.Engine(0, tuning={"transpose_radix_min_nnz": 1, "mirror_radix_min_nnz": 1})
     yield e
     e.close()
 
