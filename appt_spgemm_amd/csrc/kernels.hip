@@ -323,14 +323,16 @@ void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, i
 }
 
 // ---------------------------------------------------------------------------
-// Radix transpose: a stable LSD counting sort of A's entries by column, 8
-// bits per pass.  Entries come in row-major order, so a stable sort leaves
-// each column's entries (B's row) ordered by row id.  One wave per tile of
-// kRadixTile entries: a per-tile digit histogram (digit-major over tiles),
-// one exclusive scan of all histograms, then each wave walks its tile in
-// order, ranking equal digits inside a 64-entry chunk with ballots (lanes of
-// equal digit found bit by bit) and a per-digit running count in LDS.  No
-// global atomics; B's row pointers are the boundaries of the sorted keys.
+// Radix transpose: a stable LSD counting sort of A's entries by column, at
+// most 8 bits per pass.  Entries come in row-major order, so a stable sort
+// leaves each column's entries (B's row) ordered by row id.  One block per
+// tile of kRadixTile entries: a per-tile digit histogram (digit-major over
+// tiles), one exclusive scan of all histograms, then each block sorts its
+// tile in LDS -- every wave walks its quarter in order, ranking equal digits
+// inside a 64-entry chunk with ballots (lanes of equal digit found bit by
+// bit) and a per-digit running slot -- and copies each digit's run out to
+// consecutive addresses.  No global atomics; B's row pointers are the
+// boundaries of the sorted keys.
 // ---------------------------------------------------------------------------
 int64_t radix_tiles(int64_t nnz) { return (nnz + kRadixTile - 1) / kRadixTile; }
 static int key_bits(int32_t n) { return n > 1 ? 32 - __builtin_clz((unsigned)(n - 1)) : 1; }
@@ -355,23 +357,25 @@ constexpr int kRadixBuckets = 1 << kRadixMaxBits;
 __global__ void __launch_bounds__(256) k_radix_hist(int64_t nnz, const int32_t* __restrict__ keys, int shift, int bits,
                                                     int64_t ntiles, int32_t* __restrict__ hist, const int32_t* gate) {
   if (gated_off(gate)) return;
-  __shared__ int32_t cnt[4][kRadixBuckets];
-  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int64_t tile = (int64_t)blockIdx.x * 4 + w;
-  if (tile >= ntiles) return;
+  __shared__ int32_t cnt[kRadixBuckets];
+  const int64_t tile = blockIdx.x;
   const int nb = 1 << bits;
   const uint32_t dm = (uint32_t)nb - 1u;
-  for (int d = lane; d < nb; d += kWave) cnt[w][d] = 0;
-  gsync<kWave>();
+  for (int d = threadIdx.x; d < nb; d += blockDim.x) cnt[d] = 0;
+  __syncthreads();
   const int64_t base = tile * kRadixTile;
-  for (int c = 0; c < kRadixChunks; c++) {
-    const int64_t e = base + (int64_t)c * kWave + lane;
-    if (e < nnz) atomicAdd(&cnt[w][((uint32_t)keys[e] >> shift) & dm], 1);
+  for (int t = threadIdx.x; t < kRadixTile; t += blockDim.x) {
+    const int64_t e = base + t;
+    if (e < nnz) atomicAdd(&cnt[((uint32_t)keys[e] >> shift) & dm], 1);
   }
-  gsync<kWave>();
-  for (int d = lane; d < nb; d += kWave) hist[(int64_t)d * ntiles + tile] = cnt[w][d];
+  __syncthreads();
+  for (int d = threadIdx.x; d < nb; d += blockDim.x) hist[(int64_t)d * ntiles + tile] = cnt[d];
 }
 
+// One block per tile: (A) digit counts per wave over its kRadixChunks chunks,
+// (B) stable ranks (wave order, then chunk, then lane: the input order) place
+// every entry at its digit-sorted slot of the tile in LDS, (C) the block
+// copies the sorted tile out, each digit's run to consecutive addresses.
 __global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, int bits, int64_t ntiles,
                                                        const int64_t* __restrict__ hoff,
                                                        const int32_t* __restrict__ kin, const int32_t* __restrict__ rin,
@@ -379,51 +383,88 @@ __global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, i
                                                        int32_t* __restrict__ rout, double* __restrict__ vout,
                                                        const int32_t* gate) {
   if (gated_off(gate)) return;
-  // running output position per digit (positions < nnz < 2^31)
-  __shared__ int32_t run[4][kRadixBuckets];
+  constexpr int T = (int)kRadixTile, NB = kRadixBuckets;
+  // staging slots are skewed by one per 32 (slot + slot / 32): digit runs of
+  // clustered keys start a multiple of 32 slots apart and would otherwise
+  // land in one LDS bank
+  constexpr int TS = T + T / 32;
+  __shared__ int32_t sk[TS];
+  __shared__ int32_t sr[TS];
+  __shared__ double sv[TS];
+  __shared__ int32_t wcnt[4][NB];   // per-wave digit counts, then running slots
+  __shared__ int32_t bstart[NB];    // first slot of each digit in the tile
+  __shared__ int64_t gbase[NB];     // global position of each digit's run
+  __shared__ int64_t red[17];
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int64_t tile = (int64_t)blockIdx.x * 4 + w;
-  if (tile >= ntiles) return;
+  const int64_t tile = blockIdx.x;
   const int nb = 1 << bits;
   const uint32_t dm = (uint32_t)nb - 1u;
-  for (int d = lane; d < nb; d += kWave) run[w][d] = (int32_t)hoff[(int64_t)d * ntiles + tile];
-  gsync<kWave>();
   const int64_t base = tile * kRadixTile;
+  const int64_t wbase = base + (int64_t)w * kWave * kRadixChunks;
+  for (int d = threadIdx.x; d < 4 * NB; d += blockDim.x) (&wcnt[0][0])[d] = 0;
+  __syncthreads();
+  // (A) per-wave counts
+  for (int c = 0; c < kRadixChunks; c++) {
+    const int64_t e = wbase + (int64_t)c * kWave + lane;
+    if (e < nnz) atomicAdd(&wcnt[w][((uint32_t)kin[e] >> shift) & dm], 1);
+  }
+  __syncthreads();
+  // digit starts in the tile, and each wave's first slot per digit
+  for (int d0 = 0; d0 < NB; d0 += blockDim.x) {
+    const int d = d0 + threadIdx.x;
+    int tot = 0;
+    if (d < nb) {
+      for (int v = 0; v < 4; v++) {
+        const int x = wcnt[v][d];
+        wcnt[v][d] = tot;
+        tot += x;
+      }
+      gbase[d] = hoff[(int64_t)d * ntiles + tile];
+    }
+    int64_t all;
+    const int64_t st = block_scan_excl<int64_t>((int64_t)tot, red, &all);
+    if (d < nb) {
+      bstart[d] = (int32_t)st;
+      for (int v = 0; v < 4; v++) wcnt[v][d] += (int32_t)st;
+    }
+  }
+  __syncthreads();
+  // (B) stable ranks inside the wave, placed in LDS
   const uint64_t below = (1ull << lane) - 1ull;
-  constexpr int U = 4;  // chunks whose entries are loaded together
-  for (int c0 = 0; c0 < kRadixChunks; c0 += U) {
-    int32_t kk[U], rr[U];
-    double vv[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int64_t e = base + (int64_t)(c0 + u) * kWave + lane;
-      const bool valid = e < nnz;
-      kk[u] = valid ? kin[e] : -1;
-      rr[u] = (valid && rin) ? rin[e] : 0;
-      vv[u] = valid ? vin[e] : 0.0;
+  for (int c = 0; c < kRadixChunks; c++) {
+    const int64_t e = wbase + (int64_t)c * kWave + lane;
+    const bool valid = e < nnz;
+    const int32_t key = valid ? kin[e] : 0;
+    const uint32_t digit = ((uint32_t)key >> shift) & dm;
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < bits; b++) {
+      const bool bit = (digit >> b) & 1u;
+      const uint64_t bal = __ballot(bit);
+      peers &= bit ? bal : ~bal;
     }
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-      const bool valid = kk[u] >= 0;
-      const uint32_t digit = ((uint32_t)kk[u] >> shift) & dm;
-      // lanes holding the same digit: intersect the ballots of every digit bit
-      uint64_t peers = __ballot(valid);
-      for (int b = 0; b < bits; b++) {
-        const bool bit = (digit >> b) & 1u;
-        const uint64_t bal = __ballot(bit);
-        peers &= bit ? bal : ~bal;
-      }
-      const int rank = __popcll(peers & below);
-      const int32_t pos = run[w][digit] + rank;
-      gsync<kWave>();  // every lane has read its digit's count before it moves
-      if (valid && rank == 0) run[w][digit] += __popcll(peers);
-      gsync<kWave>();
-      if (valid) {
-        kout[pos] = kk[u];
-        if (rout) rout[pos] = rr[u];
-        vout[pos] = vv[u];
-      }
+    const int rank = __popcll(peers & below);
+    const int slot = wcnt[w][digit] + rank;
+    gsync<kWave>();  // every lane has read its digit's slot before it moves
+    if (valid && rank == 0) wcnt[w][digit] += __popcll(peers);
+    if (valid) {
+      const int ps = slot + (slot >> 5);
+      sk[ps] = key;
+      sr[ps] = rin ? rin[e] : 0;
+      sv[ps] = vin[e];
     }
+    gsync<kWave>();
+  }
+  __syncthreads();
+  // (C) copy out in slot order
+  const int valid_n = (int)((nnz - base) < (int64_t)T ? (nnz - base) : (int64_t)T);
+  for (int q = threadIdx.x; q < valid_n; q += blockDim.x) {
+    const int ps = q + (q >> 5);
+    const int32_t key = sk[ps];
+    const uint32_t digit = ((uint32_t)key >> shift) & dm;
+    const int64_t pos = gbase[digit] + (q - bstart[digit]);
+    kout[pos] = key;
+    if (rout) rout[pos] = sr[ps];
+    vout[pos] = sv[ps];
   }
 }
 
@@ -450,7 +491,7 @@ static void radix_passes_run(int32_t n, int64_t count, const int32_t* k0, const 
   const int P = radix_passes(n), bits = radix_bits(n);
   const int64_t nt = radix_tiles(count);
   if (nt <= 0) return;
-  const unsigned tb = (unsigned)((nt + 3) / 4);
+  const unsigned tb = (unsigned)nt;  // one block per tile
   const int32_t* sk = k0;
   const int32_t* sr = r0;
   const double* sv = v0;

@@ -80,10 +80,10 @@ void launch_csr_check(int32_t n, int64_t nnz, const int32_t* ptr, const int32_t*
 // increasing
 void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);
 // Radix transpose: digits of at most kRadixMaxBits bits (the fewest passes
-// that cover the column range, bits split evenly), one wave per tile of
-// kRadixTile entries.
-constexpr int kRadixMaxBits = 8, kRadixChunks = 8;
-constexpr int64_t kRadixTile = 64 * kRadixChunks;
+// that cover the column range, bits split evenly), one 256-thread block per
+// tile of kRadixTile entries (kRadixChunks chunks of 64 per wave).
+constexpr int kRadixMaxBits = 8, kRadixChunks = 16;
+constexpr int64_t kRadixTile = 4 * 64 * kRadixChunks;
 int64_t radix_tiles(int64_t nnz);
 int radix_passes(int32_t n);
 int radix_bits(int32_t n);  // digit bits per pass
