@@ -51,6 +51,7 @@ Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz
       throw Error(APPT_ERR_UNSUPPORTED, "batched ops are A*A and A*A^T");
     blk_row_h_[k] = (int32_t)rows;
     blk_nnz_h_[k] = nnz;
+    max_n_ = std::max(max_n_, n_each[k]);
     op_h[k] = ops[k] == APPT_OP_AAT ? 1 : 0;
     rows += n_each[k];
     nnz += nnz_each[k];
@@ -159,6 +160,7 @@ void Batch::operands() {
   O_.bv = in_val_;
   O_.bnnz = nnz_a_ + nnz_t_;
   O_.upper = 0;
+  O_.ncols = std::max(max_n_, 1);
   (void)E_.reset_ctl(W_);
   launch_fill_i32(tcnt_, (int64_t)N_ + 1, 0, s);
   launch_batch_acg(N_, in_rp_, in_col_, rowbase_, rowop_, acg_, tcnt_, s);

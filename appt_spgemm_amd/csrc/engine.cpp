@@ -245,6 +245,7 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
   O.bv = A.val;
   O.bnnz = nnz;
   O.upper = op == APPT_OP_AATS ? 1 : 0;
+  O.ncols = n;
   const int32_t* bptr = A.rowptr;
   int32_t* ctl = reset_ctl(W);
   if (op != APPT_OP_AA) {
@@ -370,7 +371,7 @@ static RowArgs row_args(const Workspace& W, const Operands& O, const int32_t* ro
   ra.bmeta = as<const int4>(W.bmeta.p);
   ra.wlist = as<uint2>(W.wlist.p);
   ra.lds_budget = bp.lds_budget;
-  ra.ncols = O.n;
+  ra.ncols = O.ncols > 0 ? O.ncols : O.nb;
   return ra;
 }
 

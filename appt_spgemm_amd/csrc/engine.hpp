@@ -96,6 +96,7 @@ struct Operands {
   const double* bv = nullptr;
   int64_t bnnz = 0;
   int upper = 0;  // AAtS: values formed for j >= i, lower part mirrored
+  int32_t ncols = 0;  // bound on output column indices (a batch's are local to each block)
 };
 
 // Bin statistics read back at the two host synchronisations; they fix the
@@ -241,6 +242,7 @@ class Batch {
   Plan plan_{};
   Operands O_{};
   int32_t K_ = 0, N_ = 0;
+  int32_t max_n_ = 0;  // largest block: bounds the (local) output columns
   int64_t nnz_a_ = 0, nnz_t_ = 0;
   std::vector<int32_t> blk_row_h_;
   std::vector<int64_t> blk_nnz_h_, blkoff_h_, blkprod_h_;

@@ -155,7 +155,7 @@ struct RowArgs {
   const int4* bmeta;     // per B row: {start, end, min column, max column}
   uint2* wlist;          // per output row: up to kWordListCap sorted (word, mask) pairs from the symbolic pass
   int32_t lds_budget;    // LDS bytes a block of the wave-level bins may take
-  int32_t ncols;         // column count of B (bounds every output column index)
+  int32_t ncols;         // bound on every output column index
 };
 constexpr int kWordListCap = 64;  // word-list slots per output row (stride of RowArgs::wlist)
 
