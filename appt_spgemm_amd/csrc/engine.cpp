@@ -224,8 +224,9 @@ int64_t result_bytes(int32_t n, int64_t nnz) {
 }
 
 static size_t stat_words() { return (size_t)kNumBins * kStatFields; }
-// stats layout: [sym stats | num stats | products (u64) | sym cursors | num cursors]
-static size_t stats_ints() { return stat_words() * 2 + 2 + 2 * kNumBins; }
+// stats layout: [sym stats | num stats | products (u64) | nnz(C) (i64) | sym cursors | num cursors];
+// the first four parts mirror HostStats, so each host read is one copy
+static size_t stats_ints() { return stat_words() * 2 + 4 + 2 * kNumBins; }
 
 // ---------------------------------------------------------------------------
 // stages
@@ -314,24 +315,23 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
   int32_t* rowwords = as<int32_t>(ensure(W.rowwords, sizeof(int32_t) * nn));
   int32_t* dstats = as<int32_t>(ensure(W.stats, sizeof(int32_t) * stats_ints()));
   unsigned long long* dprod = reinterpret_cast<unsigned long long*>(dstats + stat_words() * 2);
-  int32_t* dcursor = dstats + stat_words() * 2 + 2;
+  int32_t* dcursor = dstats + stat_words() * 2 + 4;
   int32_t* part = as<int32_t>(ensure(W.statpart, sizeof(int32_t) * (size_t)stats_part_elems() +
                                                       sizeof(unsigned long long) * kStatsMaxBlocks));
-  launch_fill_i32(dstats, (int64_t)stats_ints(), 0, s);
+  launch_fill3_i32(dstats, (int64_t)stats_ints(), rowcnt, n, rowwords, n, s);
   launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part, s);
-  launch_bin_cursors(dstats, dcursor, s);
+  launch_bin_cursors(dstats, dcursor, nullptr, nullptr, s);
   launch_bin_scatter(n, symbin, dcursor, symrows, s);
-  launch_fill_i32(rowcnt, n, 0, s);
-  launch_fill_i32(rowwords, n, 0, s);
 }
 
-void Engine::read_sym(Workspace& W, Plan& P) {
+void Engine::read_sym(Workspace& W, Plan& P, bool want_ctl) {
   hipStream_t s = stream_;
   const int32_t* dstats = as<int32_t>(W.stats.p);
-  APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, sizeof(int32_t) * stat_words(), hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipMemcpyAsync(&hs_->products, dstats + stat_words() * 2, sizeof(unsigned long long),
-                          hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipMemcpyAsync(hs_->ctl, W.ctl.p, sizeof(hs_->ctl), hipMemcpyDeviceToHost, s));
+  // symbolic stats and the products count in one copy (the numeric stats
+  // and total between them are not read here)
+  APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, offsetof(HostStats, total), hipMemcpyDeviceToHost, s));
+  if (want_ctl) APPT_HIP(hipMemcpyAsync(hs_->ctl, W.ctl.p, sizeof(hs_->ctl), hipMemcpyDeviceToHost, s));
+  else hs_->ctl[4] = 1;
   APPT_HIP(hipStreamSynchronize(s));
   std::memcpy(P.sym, hs_->sym, sizeof(P.sym));
   P.products = (int64_t)hs_->products;
@@ -407,20 +407,23 @@ void Engine::stage_offsets(Workspace& W, const Operands& O, const Config& cfg, i
   int32_t* numrows = as<int32_t>(ensure(W.numrows, sizeof(int32_t) * nn));
   int32_t* dstats = as<int32_t>(W.stats.p);
   int32_t* dnum = dstats + stat_words();
-  int32_t* dcursor = dstats + stat_words() * 2 + 2 + kNumBins;
+  int32_t* dcursor = dstats + stat_words() * 2 + 4 + kNumBins;
   launch_scan(as<int32_t>(W.rowcnt.p), n, rowoff, rp32, tmp, s);
   launch_numbin(n, O.ap, as<int64_t>(W.rowf.p), as<int32_t>(W.rowcnt.p), as<int32_t>(W.rowwords.p),
                 as<int32_t>(W.rowlo.p), as<int32_t>(W.rowhi.p), cfg.num, numbin, dnum, as<int32_t>(W.statpart.p), s);
-  launch_bin_cursors(dnum, dcursor, s);
+  // the cursor kernel also parks nnz(C) next to the products count
+  launch_bin_cursors(dnum, dcursor, rowoff + n, reinterpret_cast<int64_t*>(dstats + stat_words() * 2 + 2), s);
   launch_bin_scatter(n, numbin, dcursor, numrows, s);
 }
 
 void Engine::read_num(Workspace& W, const Operands& O, Plan& P) {
   hipStream_t s = stream_;
   const int32_t* dstats = as<int32_t>(W.stats.p);
-  APPT_HIP(hipMemcpyAsync(hs_->num, dstats + stat_words(), sizeof(int32_t) * stat_words(), hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipMemcpyAsync(&hs_->total, as<int64_t>(W.rowoff.p) + O.n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  // numeric stats, products and nnz(C) in one copy
+  APPT_HIP(hipMemcpyAsync(hs_->num, dstats + stat_words(),
+                          offsetof(HostStats, ctl) - offsetof(HostStats, num), hipMemcpyDeviceToHost, s));
   APPT_HIP(hipStreamSynchronize(s));
+  (void)O;
   std::memcpy(P.num, hs_->num, sizeof(P.num));
   P.nnz = hs_->total;
 }
@@ -502,7 +505,7 @@ void Engine::symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr
   Plan P;
   stage_operands(W, A, op, O);
   stage_prepass(W, O, cfg_);
-  read_sym(W, P);
+  read_sym(W, P, O.upper != 0);
   stage_symbolic(W, O, cfg_, P);
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)A.n + 1)));
   try {
@@ -539,7 +542,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   O.n = n;
   record(1);
   stage_prepass(W, O, cfg_);
-  read_sym(W, P);
+  read_sym(W, P, O.upper != 0);
   record(2);
   stage_symbolic(W, O, cfg_, P);
   record(3);

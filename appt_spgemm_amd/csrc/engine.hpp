@@ -1,6 +1,7 @@
 // engine.hpp -- host-side engine state (C++), behind the C ABI.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstddef>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -117,13 +118,17 @@ struct Result {  // what appt_csr_result::_priv points to
   void* val = nullptr;
 };
 
-struct HostStats {  // pinned staging for the small device->host reads
+struct HostStats {  // pinned staging for the small device->host reads; the
+                    // fields up to ctl mirror the device stats layout
   int32_t sym[kNumBins * kStatFields];
   int32_t num[kNumBins * kStatFields];
   unsigned long long products;
   int64_t total;
   int32_t ctl[8];
 };
+static_assert(offsetof(HostStats, products) == sizeof(int32_t) * 2 * kNumBins * kStatFields &&
+                  offsetof(HostStats, total) == offsetof(HostStats, products) + 8,
+              "HostStats mirrors the device stats layout");
 
 class Engine {
  public:
@@ -182,7 +187,7 @@ class Engine {
                      double* cval);
   void stage_mirror(Workspace& W, const Operands& O, const Plan& P, const int32_t* ccol, double* cval);
   // host synchronisations: bin statistics of the symbolic / numeric binning
-  void read_sym(Workspace& W, Plan& P);
+  void read_sym(Workspace& W, Plan& P, bool want_ctl = false);
   void read_num(Workspace& W, const Operands& O, Plan& P);
 
  private:

@@ -38,6 +38,22 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s, const int3
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v, gate);
 }
 
+__global__ void k_fill3_i32(int32_t* a, int64_t na, int32_t* b, int64_t nb, int32_t* c, int64_t nc) {
+  const int64_t tot = na + nb + nc;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
+    if (t < na) a[t] = 0;
+    else if (t < na + nb) b[t - na] = 0;
+    else c[t - na - nb] = 0;
+  }
+}
+void launch_fill3_i32(int32_t* a, int64_t na, int32_t* b, int64_t nb, int32_t* c, int64_t nc, hipStream_t s) {
+  const int64_t tot = na + nb + nc;
+  if (tot <= 0) return;
+  int64_t blocks = (tot + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill3_i32, dim3((unsigned)blocks), dim3(256), 0, s, a, na, b, nb, c, nc);
+}
+
 __global__ void k_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, const int32_t* gate) {
   if (gated_off(gate)) return;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
@@ -308,12 +324,54 @@ __global__ void k_scan_apply(const int32_t* __restrict__ in, int64_t n, const in
   }
 }
 
+// Small inputs: one block walks the tiles in order with a running carry
+// (one launch instead of three).
+constexpr int64_t kScanSmallTiles = 8;
+__global__ void __launch_bounds__(kScanThreads) k_scan_small(const int32_t* __restrict__ in, int64_t n,
+                                                             int64_t* __restrict__ out64, int32_t* __restrict__ out32,
+                                                             const int32_t* gate) {
+  if (gated_off(gate)) return;
+  __shared__ int64_t scratch[17];
+  int64_t carry = 0;
+  for (int64_t tb = 0; tb < n; tb += kScanTile) {
+    const int64_t base = tb + (int64_t)threadIdx.x * kScanItems;
+    int32_t v[kScanItems];
+    int64_t sum = 0;
+#pragma unroll
+    for (int t = 0; t < kScanItems; t++) {
+      const int64_t idx = base + t;
+      v[t] = idx < n ? in[idx] : 0;
+      sum += v[t];
+    }
+    int64_t total;
+    int64_t run = block_scan_excl<int64_t>(sum, scratch, &total) + carry;
+#pragma unroll
+    for (int t = 0; t < kScanItems; t++) {
+      const int64_t idx = base + t;
+      if (idx < n) {
+        out64[idx] = run;
+        if (out32) out32[idx] = (int32_t)run;
+      }
+      run += v[t];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) {
+    out64[n] = carry;
+    if (out32) out32[n] = (int32_t)carry;
+  }
+}
+
 void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, int64_t* tmp,
                  hipStream_t s, const int32_t* gate) {
   const int64_t nb = (n + kScanTile - 1) / kScanTile;
   if (nb == 0) {
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, (int32_t*)out64, (int64_t)2, 0, gate);
     if (out32) hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, out32, (int64_t)1, 0, gate);
+    return;
+  }
+  if (nb <= kScanSmallTiles) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(kScanThreads), 0, s, in, n, out64, out32, gate);
     return;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, tmp, gate);
@@ -972,7 +1030,9 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int32_t n, const uint8_t* _
 }
 // cursor[b] = first slot of bin b in the row list: exclusive scan of the
 // bin counts stats[b * kStatFields] over bins 1.. (bin 0 is not listed).
-__global__ void k_bin_cursors(const int32_t* __restrict__ stats, int32_t* __restrict__ cursor) {
+__global__ void k_bin_cursors(const int32_t* __restrict__ stats, int32_t* __restrict__ cursor,
+                              const int64_t* __restrict__ total_src, int64_t* __restrict__ total_dst) {
+  if (threadIdx.x == 0 && total_src) *total_dst = *total_src;
   if (threadIdx.x == 0) {
     int32_t run = 0;
     cursor[0] = 0;
@@ -982,8 +1042,9 @@ __global__ void k_bin_cursors(const int32_t* __restrict__ stats, int32_t* __rest
     }
   }
 }
-void launch_bin_cursors(const int32_t* stats, int32_t* cursor, hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_cursors, dim3(1), dim3(64), 0, s, stats, cursor);
+void launch_bin_cursors(const int32_t* stats, int32_t* cursor, const int64_t* total_src, int64_t* total_dst,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_bin_cursors, dim3(1), dim3(64), 0, s, stats, cursor, total_src, total_dst);
 }
 
 void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s) {

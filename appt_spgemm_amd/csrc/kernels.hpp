@@ -140,7 +140,11 @@ void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int3
                    uint8_t* bin, int32_t* stats, int32_t* part, hipStream_t s);
 // cursor[b] = start of bin b in the row list (exclusive scan of the counts
 // in a stats array of kNumBins * kStatFields).
-void launch_bin_cursors(const int32_t* stats, int32_t* cursor, hipStream_t s);
+// (total_src, optional: *total_dst = *total_src on the way)
+void launch_bin_cursors(const int32_t* stats, int32_t* cursor, const int64_t* total_src, int64_t* total_dst,
+                        hipStream_t s);
+// Zero three int32 arrays in one launch.
+void launch_fill3_i32(int32_t* a, int64_t na, int32_t* b, int64_t nb, int32_t* c, int64_t nc, hipStream_t s);
 void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s);
 
 struct RowArgs {
