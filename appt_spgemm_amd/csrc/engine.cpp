@@ -19,6 +19,8 @@
 // inside a HIP graph with the launch shapes of its first run.
 #include "engine.hpp"
 
+#include <chrono>
+
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -174,6 +176,7 @@ Engine::Engine(const appt_options* opt) : pool_(0) {
   set_lds_per_cu(lds_cu > 0 ? lds_cu : lds_max_);
   APPT_HIP(hipHostMalloc((void**)&hs_, sizeof(HostStats), hipHostMallocDefault));
   for (auto& e : ev_) APPT_HIP(hipEventCreate(&e));
+  APPT_HIP(hipEventCreateWithFlags(&ev_wait_, hipEventDisableTiming));
   cfg_ = make_config(opt ? &opt->tuning : nullptr, lds_max_);
 }
 
@@ -183,8 +186,21 @@ Engine::~Engine() {
   free_workspace(ws_);
   pool_.trim();
   for (auto& e : ev_) if (e) (void)hipEventDestroy(e);
+  if (ev_wait_) (void)hipEventDestroy(ev_wait_);
   if (hs_) (void)hipHostFree(hs_);
   if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::wait_stream() {
+  APPT_HIP(hipEventRecord(ev_wait_, stream_));
+  const auto t0 = std::chrono::steady_clock::now();
+  while (true) {
+    const hipError_t e = hipEventQuery(ev_wait_);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) APPT_HIP(e);
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(2000)) break;
+  }
+  APPT_HIP(hipEventSynchronize(ev_wait_));
 }
 
 void* Engine::ensure(Scratch& s, size_t bytes) {
@@ -332,7 +348,7 @@ void Engine::read_sym(Workspace& W, Plan& P, bool want_ctl) {
   APPT_HIP(hipMemcpyAsync(hs_->sym, dstats, offsetof(HostStats, total), hipMemcpyDeviceToHost, s));
   if (want_ctl) APPT_HIP(hipMemcpyAsync(hs_->ctl, W.ctl.p, sizeof(hs_->ctl), hipMemcpyDeviceToHost, s));
   else hs_->ctl[4] = 1;
-  APPT_HIP(hipStreamSynchronize(s));
+  wait_stream();
   std::memcpy(P.sym, hs_->sym, sizeof(P.sym));
   P.products = (int64_t)hs_->products;
   P.a_strict = hs_->ctl[4] == 0;
@@ -422,7 +438,7 @@ void Engine::read_num(Workspace& W, const Operands& O, Plan& P) {
   // numeric stats, products and nnz(C) in one copy
   APPT_HIP(hipMemcpyAsync(hs_->num, dstats + stat_words(),
                           offsetof(HostStats, ctl) - offsetof(HostStats, num), hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipStreamSynchronize(s));
+  wait_stream();
   (void)O;
   std::memcpy(P.num, hs_->num, sizeof(P.num));
   P.nnz = hs_->total;
@@ -581,7 +597,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   }
   record(6);
   APPT_HIP(hipGetLastError());
-  APPT_HIP(hipStreamSynchronize(s));
+  wait_stream();
 
   Result* R = new Result{rowptr_out, ccol, cval};
   C->n = n;

@@ -203,6 +203,10 @@ class Engine {
   HostStats* hs_ = nullptr;
   bool timing_ = false;
   hipEvent_t ev_[8] = {};
+  hipEvent_t ev_wait_ = nullptr;  // completion marker the host polls at a synchronisation
+  // Wait for everything enqueued on the engine's stream: poll a marker
+  // event (the engine's waits are short), blocking after about 2 ms.
+  void wait_stream();
   appt_phase_times times_ = {};
   appt_bin_report bins_ = {};
   int64_t lds_max_ = 65536;
