@@ -608,6 +608,9 @@ __device__ __forceinline__ int64_t span_words(int lo, int hi) {
 }
 
 constexpr int kTouchCap = 64;   // touched-word list length per row
+// products bound of the narrow-span symbolic rows kept wave-level, as a
+// multiple of the last class's bound
+constexpr int kSymStretch = 4;
 // rowwords[i] holds the row's distinct-word count; kNoWordList marks rows whose
 // symbolic pass left no sorted word list behind
 constexpr int32_t kNoWordList = 1 << 30, kWordsMask = kNoWordList - 1;
@@ -687,6 +690,15 @@ __device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t
       break;
     }
   if (sb >= 0) { *stat = nA; return sb; }
+  // long rows over a narrow span (banded operands): one 64-lane group of the
+  // last class per row rather than a workgroup per row, and the row keeps its
+  // word list (every touched word fits the list), which the numeric pass uses
+  const int lc = P.ncls - 1;
+  if (f <= kSymStretch * (int64_t)P.cls_max[lc] && words <= kTouchCap &&
+      sym_window_bytes(64, words) <= P.cls_bytes[lc]) {
+    *stat = words;
+    return 1 + (lc * 2 + 0) * 4 + 3;
+  }
   if (words * 4 <= P.wg_bytes) { *stat = words; return kBinWgBitmap; }
   if (hash_bytes <= P.wg_bytes) { *stat = keys; return kBinWgHash; }
   *stat = words;

@@ -75,6 +75,25 @@ def test_banded_and_er(eng, op):
         assert_product(got, ref(m, op), exact=True)
 
 
+def test_bin_report_narrow_span_rows(eng):
+    """A half-bandwidth-32 band gives rows of about 2000-2800 products over a
+    span of 5 words: past the last products class, yet each row stays in a
+    64-lane wave window (no workgroup-per-row bin) and keeps its word list,
+    so every numeric row runs in a list bin."""
+    m = ap.banded(4000, 32, 0.7, 3)
+    say("case bin report", m.meta)
+    got = eng.multiply(m, "AA")
+    assert_product(got, ref(m, "AA"), exact=True)
+    br = eng.bin_report()
+    say("bins", br)
+    sym, num = br["symbolic"], br["numeric"]
+    assert sum(v[0] for v in sym.values()) == m.n
+    assert sum(v[0] for v in num.values()) == m.n
+    assert "wg-window" not in sym and "wg-hash" not in sym
+    assert sym["window/c2/G64"][1] > 2048
+    assert all(k.startswith("list/") for k in num)
+
+
 @pytest.mark.parametrize("op", OPS)
 def test_rmat_small(eng, op):
     m = ap.rmat(12, 16, seed=1)
