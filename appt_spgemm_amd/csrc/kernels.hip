@@ -190,11 +190,17 @@ __global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, co
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
   for (int64_t k = (int64_t)blockIdx.x * gpb + threadIdx.x / G; k < n; k += (int64_t)gridDim.x * gpb) {
-    if (__hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-    const int s = ap[k], e = ap[k + 1];
+    // a row that starts after an asymmetry was found is skipped; the flag is
+    // read together with the row bounds (a select, not a branch), so its
+    // round trip does not delay them
+    const int stop = __hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int s = ap[k], e0 = ap[k + 1];
+    const int e = stop != 0 ? s : e0;
     bool miss = false;
     for (int p = s + lane; p < e; p += G) {
-      if ((p - s) % (16 * G) == lane && __hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+      if (p - s >= 16 * G && (p - s) % (16 * G) == lane &&
+          __hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+        break;
       const int i = ac[p];
       if (p > s && ac[p - 1] >= i) miss = true;   // descent or duplicate
       int lo = ap[i], hi = ap[i + 1];
