@@ -2523,6 +2523,48 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
   }
 }
 
+// Symbolic window for bins whose output spans are at most kNarrowWords
+// words (banded operands: a B row's consecutive columns put most lanes of a
+// product step in one or two words).  One shared bitmap would serialise
+// those lanes' atomic ORs on one address; each group instead keeps kNarrowC
+// copies of its row bitmap, lane l ORing into copy l % kNarrowC, so the
+// lanes of one word hit consecutive banks.  The copies are merged once per
+// row (copy order rotated per word, one bank per lane), and the row's word
+// list is the merged nonzero words, in order.  One row per group.
+constexpr int kNarrowWords = 8;
+template <int G>
+__global__ void __launch_bounds__(256) k_sym_narrow(RowArgs A, int32_t* __restrict__ rowcnt) {
+  constexpr int C = G < 32 ? G : 32;
+  __shared__ uint32_t copies[256 / G][kNarrowWords * C];
+  const int R = blockDim.x / G;
+  const int gi = threadIdx.x / G, lane = threadIdx.x % G;
+  const int r = xcd_block(blockIdx.x, gridDim.x) * R + gi;
+  if (r >= A.nrows) return;
+  const int i = A.rows[r];
+  uint32_t* bm = copies[gi];
+  for (int t = lane; t < kNarrowWords * C; t += G) bm[t] = 0u;
+  gsync<G>();
+  const int lo = A.rowlo[i];
+  const int cp = lane % C;
+  for_each_product<G, false>(i, lane, A, nullptr, [&](int j, double) {
+    const uint32_t b = (uint32_t)(j - lo);
+    atomicOr(&bm[(b >> 5) * C + cp], 1u << (b & 31u));
+  });
+  gsync<G>();
+  const int words = (int)span_words(lo, A.rowhi[i]);   // <= kNarrowWords by the launch
+  uint32_t m = 0u;
+  if (lane < words)
+    for (int c = 0; c < C; c++) m |= bm[lane * C + (c + lane) % C];
+  const uint64_t nz = group_bits<G>(__ballot(m != 0u));
+  const int rank = __popcll(nz & ((1ull << lane) - 1ull));
+  if (m != 0u) A.wlist[(size_t)i * kWordCap + rank] = make_uint2((uint32_t)lane, m);
+  const int cnt = gsum<G, int32_t>(__popc(m), nullptr);
+  if (lane == 0) {
+    rowcnt[i] = cnt;
+    A.rowwords[i] = __popcll(nz);
+  }
+}
+
 template <int G, bool UPPER, bool W64, bool LIST>
 __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_t Cm,
                                                     const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
@@ -2818,6 +2860,12 @@ static void num_window_launch(const RowArgs& a, int32_t W, int32_t Cm, const int
 }
 template <int G>
 static void sym_window_any(const RowArgs& a, int32_t W, int32_t* rowcnt, hipStream_t s) {
+  if (W <= kNarrowWords) {
+    const int R = 256 / G;
+    const unsigned blocks = (unsigned)((a.nrows + R - 1) / R);
+    hipLaunchKernelGGL(k_sym_narrow<G>, dim3(blocks), dim3(256), 0, s, a, rowcnt);
+    return;
+  }
   const bool wide = a.bnnz >= kNarrowIdx;
   if (a.upper) { if (wide) sym_window_launch<G, true, true>(a, W, rowcnt, s); else sym_window_launch<G, true, false>(a, W, rowcnt, s); }
   else { if (wide) sym_window_launch<G, false, true>(a, W, rowcnt, s); else sym_window_launch<G, false, false>(a, W, rowcnt, s); }
