@@ -44,6 +44,7 @@ Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz
   blk_row_h_.assign((size_t)K_ + 1, 0);
   blk_nnz_h_.assign((size_t)K_ + 1, 0);
   std::vector<uint8_t> op_h((size_t)std::max(K_, 1), 0);
+  std::vector<int32_t> toff_h((size_t)K_ + 1, 0);
   int64_t rows = 0, nnz = 0, tnnz = 0;
   for (int32_t k = 0; k < K_; k++) {
     if (n_each[k] < 0 || nnz_each[k] < 0) throw Error(APPT_ERR_INVALID, "negative batch matrix size");
@@ -51,7 +52,9 @@ Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz
       throw Error(APPT_ERR_UNSUPPORTED, "batched ops are A*A and A*A^T");
     blk_row_h_[k] = (int32_t)rows;
     blk_nnz_h_[k] = nnz;
+    toff_h[k] = (int32_t)tnnz;
     max_n_ = std::max(max_n_, n_each[k]);
+    if (ops[k] == APPT_OP_AAT) max_nt_ = std::max(max_nt_, n_each[k]);
     op_h[k] = ops[k] == APPT_OP_AAT ? 1 : 0;
     rows += n_each[k];
     nnz += nnz_each[k];
@@ -61,6 +64,7 @@ Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz
   }
   blk_row_h_[K_] = (int32_t)rows;
   blk_nnz_h_[K_] = nnz;
+  toff_h[K_] = (int32_t)tnnz;
   N_ = (int32_t)rows;
   nnz_a_ = nnz;
   nnz_t_ = tnnz;
@@ -73,6 +77,7 @@ Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz
   blk_row_ = as<int32_t>(alloc(sizeof(int32_t) * ((size_t)K_ + 1)));
   blk_nnz_ = as<int64_t>(alloc(sizeof(int64_t) * ((size_t)K_ + 1)));
   blk_op_ = as<uint8_t>(alloc((size_t)std::max(K_, 1)));
+  blk_toff_ = as<int32_t>(alloc(sizeof(int32_t) * ((size_t)K_ + 1)));
   rowbase_ = as<int32_t>(alloc(sizeof(int32_t) * nn));
   rowblk_ = as<int32_t>(alloc(sizeof(int32_t) * nn));
   rowop_ = as<uint8_t>(alloc(nn));
@@ -90,6 +95,7 @@ Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz
   APPT_HIP(hipMemcpyAsync(blk_row_, blk_row_h_.data(), sizeof(int32_t) * ((size_t)K_ + 1), hipMemcpyHostToDevice, s));
   APPT_HIP(hipMemcpyAsync(blk_nnz_, blk_nnz_h_.data(), sizeof(int64_t) * ((size_t)K_ + 1), hipMemcpyHostToDevice, s));
   if (K_ > 0) APPT_HIP(hipMemcpyAsync(blk_op_, op_h.data(), (size_t)K_, hipMemcpyHostToDevice, s));
+  APPT_HIP(hipMemcpyAsync(blk_toff_, toff_h.data(), sizeof(int32_t) * ((size_t)K_ + 1), hipMemcpyHostToDevice, s));
   if (K_ > 0) launch_batch_rows(N_, K_, blk_row_, blk_op_, rowbase_, rowblk_, rowop_, s);
   APPT_HIP(hipStreamSynchronize(s));
 }
@@ -162,9 +168,19 @@ void Batch::operands() {
   O_.upper = 0;
   O_.ncols = std::max(max_n_, 1);
   (void)E_.reset_ctl(W_);
-  launch_fill_i32(tcnt_, (int64_t)N_ + 1, 0, s);
-  launch_batch_acg(N_, in_rp_, in_col_, rowbase_, rowop_, acg_, tcnt_, s);
-  if (nnz_t_ > 0) {
+  // A*A^T matrices of at most kBatchTBlockN rows: one workgroup transposes
+  // each (LDS column counts); otherwise global counts and cursors
+  const bool tblock = nnz_t_ > 0 && max_nt_ <= kBatchTBlockN;
+  if (!tblock) launch_fill_i32(tcnt_, (int64_t)N_ + 1, 0, s);
+  // row ids of the entries (local to their matrix) for the workgroup transposes
+  int32_t* rid = tblock ? static_cast<int32_t*>(E_.ensure(W_.rkeys, sizeof(int32_t) * (size_t)nnz_a_)) : nullptr;
+  launch_batch_acg(N_, in_rp_, in_col_, rowbase_, rowop_, acg_, tblock ? nullptr : tcnt_, rid, s);
+  if (tblock) {
+    launch_batch_tblock(K_, N_, max_nt_, nnz_t_, blk_row_, blk_nnz_, blk_op_, blk_toff_, rid, in_col_, in_val_,
+                        tp32_, in_col_ + nnz_a_, in_val_ + nnz_a_, s);
+    int32_t* slist = static_cast<int32_t*>(E_.ensure(W_.sortlist, sizeof(int32_t) * (size_t)sort_rows_scratch_elems(N_)));
+    launch_sort_rows(N_, tp32_, in_col_ + nnz_a_, in_val_ + nnz_a_, slist, nullptr, s);
+  } else if (nnz_t_ > 0) {
     launch_scan(tcnt_, N_, toff_, tp32_, tscan_, s);
     launch_i64_to_i32(toff_, N_, tcur_, s);
     launch_batch_tscatter(N_, in_rp_, acg_, rowbase_, rowop_, tcur_, nnz_a_, in_col_, in_val_, s);

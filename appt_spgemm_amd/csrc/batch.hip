@@ -43,11 +43,12 @@ void launch_batch_rows(int32_t N, int32_t K, const int32_t* blk_row, const uint8
                      rowop);
 }
 
-// acg[p] = col[p] + rowbase(row of p); entries of A*A^T rows also count
-// their transpose row (tcnt, zero beforehand).
+// acg[p] = col[p] + rowbase(row of p); with tcnt, entries of A*A^T rows
+// also count their transpose row (tcnt, zero beforehand); with rid, they
+// record their row local to the matrix.
 __global__ void k_batch_acg(int32_t N, const int32_t* __restrict__ rp, const int32_t* __restrict__ col,
                             const int32_t* __restrict__ rowbase, const uint8_t* __restrict__ rowop,
-                            int32_t* __restrict__ acg, int32_t* __restrict__ tcnt) {
+                            int32_t* __restrict__ acg, int32_t* __restrict__ tcnt, int32_t* __restrict__ rid) {
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
@@ -57,16 +58,18 @@ __global__ void k_batch_acg(int32_t N, const int32_t* __restrict__ rp, const int
     for (int p = rp[i] + lane; p < rp[i + 1]; p += G) {
       const int k = col[p] + base;
       acg[p] = k;
-      if (t) atomicAdd(&tcnt[k], 1);
+      if (t && tcnt) atomicAdd(&tcnt[k], 1);
+      if (t && rid) rid[p] = (int32_t)(i - base);
     }
   }
 }
 void launch_batch_acg(int32_t N, const int32_t* rp, const int32_t* col, const int32_t* rowbase,
-                      const uint8_t* rowop, int32_t* acg, int32_t* tcnt, hipStream_t s) {
+                      const uint8_t* rowop, int32_t* acg, int32_t* tcnt, int32_t* rid, hipStream_t s) {
   if (N <= 0) return;
   int64_t blocks = ((int64_t)N * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_batch_acg, dim3((unsigned)blocks), dim3(256), 0, s, N, rp, col, rowbase, rowop, acg, tcnt);
+  hipLaunchKernelGGL(k_batch_acg, dim3((unsigned)blocks), dim3(256), 0, s, N, rp, col, rowbase, rowop, acg, tcnt,
+                     rid);
 }
 
 // Transpose entries of A*A^T rows into the region after the nnz(A) packed
@@ -96,6 +99,96 @@ void launch_batch_tscatter(int32_t N, const int32_t* rp, const int32_t* acg, con
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_batch_tscatter, dim3((unsigned)blocks), dim3(256), 0, s, N, rp, acg, rowbase, rowop, cursor,
                      nnz_a, bcol, bval);
+}
+
+// Transposes of the A*A^T matrices, one workgroup per matrix, for batches
+// whose A*A^T matrices have at most kBatchTBlockN rows: column counts in LDS
+// (LDS atomics, where the global-counter form serialises on the columns a
+// band's neighbouring rows share), their scan gives the transposed row
+// pointers (tp, relative to the transposed region; matrix b's region starts
+// at toff[b], the A*A^T entries before it), and entries are placed through
+// LDS cursors, walked in storage order kU at a time per thread with their
+// local rows from rid.  A*A matrices only write their empty transposed
+// rows.  Rows come out unordered within a transposed row (the per-row sort
+// follows).
+__global__ void __launch_bounds__(1024) k_batch_tblock(int32_t K, const int32_t* __restrict__ blk_row,
+                                                       const int64_t* __restrict__ blk_nnz,
+                                                       const uint8_t* __restrict__ blk_op,
+                                                       const int32_t* __restrict__ toff,
+                                                       const int32_t* __restrict__ rid,
+                                                       const int32_t* __restrict__ col,
+                                                       const double* __restrict__ val, int32_t* __restrict__ tp,
+                                                       int32_t* __restrict__ bcol, double* __restrict__ bval) {
+  constexpr int kU = 4;
+  extern __shared__ int32_t cnt[];
+  __shared__ int64_t red[17];
+  const int tid = threadIdx.x, T = blockDim.x;
+  for (int b = blockIdx.x; b < K; b += gridDim.x) {
+    const int r0 = blk_row[b], r1 = blk_row[b + 1], nb = r1 - r0;
+    const int base = toff[b];
+    if (!blk_op[b]) {
+      for (int t = tid; t < nb; t += T) tp[r0 + t] = base;
+      continue;
+    }
+    for (int t = tid; t < nb; t += T) cnt[t] = 0;
+    __syncthreads();
+    const int64_t e0 = blk_nnz[b], e1 = blk_nnz[b + 1];
+    for (int64_t p0 = e0 + tid; p0 < e1; p0 += (int64_t)T * kU) {
+      int c[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) c[u] = p0 + (int64_t)u * T < e1 ? col[p0 + (int64_t)u * T] : -1;
+#pragma unroll
+      for (int u = 0; u < kU; u++)
+        if (c[u] >= 0) atomicAdd(&cnt[c[u]], 1);
+    }
+    __syncthreads();
+    // exclusive scan of the counts: a contiguous chunk per thread
+    const int chunk = (nb + T - 1) / T;
+    const int k0 = tid * chunk, k1 = (k0 + chunk) < nb ? (k0 + chunk) : nb;
+    int64_t local = 0;
+    for (int k = k0; k < k1; k++) local += cnt[k];
+    int64_t tot;
+    int64_t run = block_scan_excl<int64_t>(local, red, &tot);
+    for (int k = k0; k < k1; k++) {
+      const int c = cnt[k];
+      cnt[k] = (int32_t)run;
+      tp[r0 + k] = base + (int32_t)run;
+      run += c;
+    }
+    __syncthreads();
+    // entries to their transposed rows
+    for (int64_t p0 = e0 + tid; p0 < e1; p0 += (int64_t)T * kU) {
+      int c[kU], li[kU];
+      double v[kU];
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const int64_t p = p0 + (int64_t)u * T;
+        const bool ok = p < e1;
+        c[u] = ok ? col[p] : -1;
+        li[u] = ok ? rid[p] : 0;
+        v[u] = ok ? val[p] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        if (c[u] < 0) continue;
+        const int q = base + atomicAdd(&cnt[c[u]], 1);
+        bcol[q] = li[u];
+        bval[q] = v[u];
+      }
+    }
+    __syncthreads();
+  }
+}
+void launch_batch_tblock(int32_t K, int32_t N, int32_t max_nt, int64_t nnz_t, const int32_t* blk_row,
+                         const int64_t* blk_nnz, const uint8_t* blk_op, const int32_t* toff, const int32_t* rid,
+                         const int32_t* col, const double* val, int32_t* tp, int32_t* bcol, double* bval,
+                         hipStream_t s) {
+  if (K <= 0) return;
+  const size_t lds = sizeof(int32_t) * ((size_t)max_nt + 1);
+  (void)hipFuncSetAttribute((const void*)k_batch_tblock, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  hipLaunchKernelGGL(k_batch_tblock, dim3((unsigned)K), dim3(1024), lds, s, K, blk_row, blk_nnz, blk_op, toff, rid,
+                     col, val, tp, bcol, bval);
+  launch_fill_i32(tp + N, 1, (int32_t)nnz_t, s);
 }
 
 // Per-matrix row pointers of the block-diagonal C: matrix b's n_b + 1

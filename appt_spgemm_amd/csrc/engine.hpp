@@ -251,7 +251,8 @@ class Batch {
   Plan plan_{};
   Operands O_{};
   int32_t K_ = 0, N_ = 0;
-  int32_t max_n_ = 0;  // largest block: bounds the (local) output columns
+  int32_t max_n_ = 0;   // largest block: bounds the (local) output columns
+  int32_t max_nt_ = 0;  // largest A*A^T block
   int64_t nnz_a_ = 0, nnz_t_ = 0;
   std::vector<int32_t> blk_row_h_;
   std::vector<int64_t> blk_nnz_h_, blkoff_h_, blkprod_h_;
@@ -259,6 +260,7 @@ class Batch {
   // device
   int32_t *in_rp_ = nullptr, *in_col_ = nullptr, *blk_row_ = nullptr, *rowbase_ = nullptr, *rowblk_ = nullptr;
   int32_t *acg_ = nullptr, *tcnt_ = nullptr, *tp32_ = nullptr, *tcur_ = nullptr, *rp32_ = nullptr, *crp_ = nullptr;
+  int32_t* blk_toff_ = nullptr;  // per block: its A*A^T entries' offset in the transposed region
   int32_t* ccol_ = nullptr;
   double *in_val_ = nullptr, *cval_ = nullptr;
   int64_t *blk_nnz_ = nullptr, *toff_ = nullptr, *tscan_ = nullptr, *blkoff_ = nullptr, *blkprod_ = nullptr;

@@ -204,10 +204,17 @@ struct BatchSrc {  // one separate device matrix to pack
 void launch_batch_rows(int32_t N, int32_t K, const int32_t* blk_row, const uint8_t* blk_op, int32_t* rowbase,
                        int32_t* rowblk, uint8_t* rowop, hipStream_t s);
 void launch_batch_acg(int32_t N, const int32_t* rp, const int32_t* col, const int32_t* rowbase,
-                      const uint8_t* rowop, int32_t* acg, int32_t* tcnt, hipStream_t s);
+                      const uint8_t* rowop, int32_t* acg, int32_t* tcnt, int32_t* rid, hipStream_t s);
 void launch_batch_tscatter(int32_t N, const int32_t* rp, const int32_t* acg, const int32_t* rowbase,
                            const uint8_t* rowop, int32_t* cursor, int64_t nnz_a, int32_t* bcol, double* bval,
                            hipStream_t s);
+// Largest A*A^T matrix (rows) the per-matrix workgroup transpose takes (its
+// column counts live in LDS).
+constexpr int32_t kBatchTBlockN = 16384;
+void launch_batch_tblock(int32_t K, int32_t N, int32_t max_nt, int64_t nnz_t, const int32_t* blk_row,
+                         const int64_t* blk_nnz, const uint8_t* blk_op, const int32_t* toff, const int32_t* rid,
+                         const int32_t* col, const double* val, int32_t* tp, int32_t* bcol, double* bval,
+                         hipStream_t s);
 void launch_batch_local_rowptr(int32_t N, int32_t K, const int64_t* rowoff, const int32_t* rowbase,
                                const int32_t* rowblk, const int32_t* blk_row, int32_t* crp, int64_t* blkoff,
                                hipStream_t s);

@@ -108,6 +108,21 @@ def test_batch_gather_and_edges(eng):
             d.free()
 
 
+def test_batch_transpose_paths(eng):
+    # A*A^T blocks of at most 16384 rows are transposed one workgroup each
+    # (LDS column counts); a larger one sends the whole batch to the global
+    # counts-and-cursors transpose.  Both must give the oracle's products.
+    small, sops = mixed(8, 6, nmax=2000)
+    for extra in (ap.banded(16384, 9, 0.7, 61), ap.erdos_renyi(20000, 3.0, 62)):
+        mats, ops = small + [extra], sops + ["AAt"]
+        say("case batch transpose path, largest A*A^T n =", extra.n)
+        with ap.Batch.from_host(eng, mats, ops) as b:
+            b.plan(True)
+            b.execute()
+            for k, (m, op) in enumerate(zip(mats, ops)):
+                assert_product(b.download(k), ref(m, op), exact=True)
+
+
 def test_batch_autotune(eng):
     mats, ops = mixed(12, 5)
     say("case batch autotune")
