@@ -96,6 +96,8 @@ void Pool::trim_locked() {
 // many times more words than the row has products (symbolic: by span alone,
 // so only far sparser spans qualify).
 constexpr int32_t kDefaultSortRatio = 2, kDefaultSymSortRatio = 16;
+// Device bytes the symbolic workgroup-bitmap rows may keep for the numeric pass.
+constexpr int64_t kWgBitsBudget = (int64_t)1 << 30;
 
 static BinParams default_params() {
   BinParams P{};
@@ -388,12 +390,29 @@ static RowArgs row_args(const Workspace& W, const Operands& O, const int32_t* ro
   ra.wlist = as<uint2>(W.wlist.p);
   ra.lds_budget = bp.lds_budget;
   ra.ncols = O.ncols > 0 ? O.ncols : O.nb;
+  if (W.wg_stride > 0) {
+    ra.wgbits = as<uint32_t>(W.wgbits.p);
+    ra.wgslot = as<int32_t>(W.wgslot.p);
+    ra.wgstride = W.wg_stride;
+  }
   return ra;
 }
 
 void Engine::stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, const Plan& P) {
   hipStream_t s = stream_;
   (void)ensure(W.wlist, sizeof(uint2) * kWordListCap * (size_t)std::max(O.n, 1));
+  // the workgroup-bitmap rows keep their bitmaps for the numeric workgroup
+  // pass (which then skips its structure pass) when they fit the budget
+  W.wg_stride = 0;
+  {
+    const int64_t cnt = P.sym[kBinWgBitmap * kStatFields], wmax = P.sym[kBinWgBitmap * kStatFields + 2];
+    if (cnt > 0 && wmax > 0 && cnt * wmax * 4 <= kWgBitsBudget) {
+      (void)ensure(W.wgbits, sizeof(uint32_t) * (size_t)(cnt * wmax));
+      int32_t* slot = as<int32_t>(ensure(W.wgslot, sizeof(int32_t) * (size_t)std::max(O.n, 1)));
+      launch_fill_i32(slot, O.n, -1, s);
+      W.wg_stride = (int32_t)wmax;
+    }
+  }
   const int32_t* symrows = as<int32_t>(W.symrows.p);
   int32_t* rowcnt = as<int32_t>(W.rowcnt.p);
   HeavyArgs heavy{};
@@ -408,6 +427,11 @@ void Engine::stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, 
     if (rb > lds_max_ - 8192) throw Error(APPT_ERR_UNSUPPORTED, "symbolic row exceeds LDS");
     // structure passes compute full rows (AAtS mirrors values only)
     RowArgs ra = row_args(W, O, symrows + off, cnt, 0, cfg.sym);
+    if (b != kBinWgBitmap) {
+      ra.wgbits = nullptr;
+      ra.wgslot = nullptr;
+      ra.wgstride = 0;
+    }
     launch_symbolic_bin(b, ra, wmax, rowcnt, heavy, s);
     off += cnt;
   }

@@ -72,13 +72,16 @@ struct Config {
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
       numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
-      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off;
+      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot;
+  // words per row of the stored workgroup-bin bitmaps of the current call
+  // (0: none stored)
+  int32_t wg_stride = 0;
   template <typename F>
   void each(F&& f) {
     Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
                       &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp,
-                      &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off};
+                      &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot};
     for (Scratch* s : all) f(*s);
   }
 };

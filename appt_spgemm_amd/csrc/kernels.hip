@@ -1299,6 +1299,14 @@ __global__ void __launch_bounds__(256) k_symbolic_lds(RowArgs A, int32_t row_wor
       cnt += (old & bit) ? 0 : 1;
       nw += old == 0u ? 1 : 0;
     });
+    if (G > kWave && A.wgbits) {
+      // keep the row's bitmap for the numeric workgroup pass (streaming
+      // stores: they would otherwise evict the B rows this pass gathers)
+      gsync<G>();
+      uint32_t* dst = A.wgbits + (size_t)r * A.wgstride;
+      for (int t = lane; t < words; t += G) __builtin_nontemporal_store(tab[t], &dst[t]);
+      if (lane == 0) A.wgslot[i] = r;
+    }
   } else {
     const int64_t keys_bound = A.rowf[i] < span_words(lo, hi) ? A.rowf[i] : span_words(lo, hi);
     const int64_t T = np2_i64((keys_bound * 4) / 3 + 1);
@@ -1503,14 +1511,23 @@ __device__ __forceinline__ void numeric_dense_row(int i, int lane, const RowArgs
   const int words = (int)span_words(lo, hi);
   const int64_t off = rowoff[i];
   const int c = (int)(rowoff[i + 1] - off);
-  for (int t = lane; t < words; t += G) info[t] = make_uint2(0u, 0u);
-  for (int t = lane; t < c; t += G) vals[t] = 0.0;
-  gsync<G>();
-  uint32_t* bits = (uint32_t*)info;
-  for_each_product<G, false>(i, lane, A, stage, [&](int j, double) {
-    const uint32_t b = (uint32_t)(j - lo);
-    atomicOr(&bits[(b >> 5) * 2], 1u << (b & 31u));
-  });
+  // the row's structure: the bitmap its symbolic pass kept, else a pass
+  // over the products
+  const int32_t slot = A.wgslot ? A.wgslot[i] : -1;
+  if (slot >= 0) {
+    const uint32_t* src = A.wgbits + (size_t)slot * A.wgstride;
+    for (int t = lane; t < words; t += G) info[t] = make_uint2(__builtin_nontemporal_load(&src[t]), 0u);
+    for (int t = lane; t < c; t += G) vals[t] = 0.0;
+  } else {
+    for (int t = lane; t < words; t += G) info[t] = make_uint2(0u, 0u);
+    for (int t = lane; t < c; t += G) vals[t] = 0.0;
+    gsync<G>();
+    uint32_t* bits = (uint32_t*)info;
+    for_each_product<G, false>(i, lane, A, stage, [&](int j, double) {
+      const uint32_t b = (uint32_t)(j - lo);
+      atomicOr(&bits[(b >> 5) * 2], 1u << (b & 31u));
+    });
+  }
   gsync<G>();
   const int chunk = (words + G - 1) / G;
   const int w0 = lane * chunk;

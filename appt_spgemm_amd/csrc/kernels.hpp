@@ -160,6 +160,12 @@ struct RowArgs {
   uint2* wlist;          // per output row: up to kWordListCap sorted (word, mask) pairs from the symbolic pass
   int32_t lds_budget;    // LDS bytes a block of the wave-level bins may take
   int32_t ncols;         // bound on every output column index
+  // bitmaps of the symbolic workgroup-bitmap rows, kept for the numeric
+  // workgroup pass: row i's span words at wgbits + wgslot[i] * wgstride
+  // (wgslot[i] < 0: none); null when not kept
+  uint32_t* wgbits;
+  int32_t* wgslot;
+  int32_t wgstride;
 };
 constexpr int kWordListCap = 64;  // word-list slots per output row (stride of RowArgs::wlist)
 
