@@ -1996,15 +1996,14 @@ struct HeavyStage {
   int32_t row;
 };
 
-template <bool VALS>
-__global__ void __launch_bounds__(kHeavyT) k_heavy(RowArgs A, int32_t wc, int32_t* __restrict__ work,
-                                                   const int32_t* __restrict__ bflag, int32_t* __restrict__ cur,
-                                                   int32_t* __restrict__ rowcnt, const int64_t* __restrict__ rowoff,
-                                                   int32_t* __restrict__ ccol, double* __restrict__ cval) {
+// Symbolic heavy rows: windows of `wc` columns, a bitmap each; the row's
+// count and distinct words are summed over the windows.
+__global__ void __launch_bounds__(kHeavyT) k_heavy_sym(RowArgs A, int32_t wc, int32_t* __restrict__ work,
+                                                       const int32_t* __restrict__ bflag, int32_t* __restrict__ cur,
+                                                       int32_t* __restrict__ rowcnt) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ HeavyStage hs;
-  double* vals = (double*)lds_raw;
-  uint32_t* bits = (uint32_t*)(lds_raw + (VALS ? (size_t)wc * 8 : 0));
+  uint32_t* bits = (uint32_t*)lds_raw;
   const int tid = threadIdx.x;
   const bool sorted = bflag[0] == 0;
   while (true) {
@@ -2018,46 +2017,35 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy(RowArgs A, int32_t wc, int32_
     const int s = A.ap[i], e = A.ap[i + 1];
     const int npass = (int)(((int64_t)hi - lo + wc) / wc);
     const bool carry = sorted && npass > 1;
-    int64_t out = 0, end = 0;
-    if (VALS) {
-      out = rowoff[i];
-      end = rowoff[i + 1];
-    }
     int64_t cnt = 0, nzw = 0;
     for (int pass = 0; pass < npass; pass++) {
-      const int w0 = lo + pass * wc;
-      const int w1 = (int)((int64_t)w0 + wc < (int64_t)hi + 1 ? (int64_t)w0 + wc : (int64_t)hi + 1);
-      const int width = w1 - w0;
-      const int words = (width + 31) >> 5;
+      const int64_t w0 = (int64_t)lo + (int64_t)pass * wc;
+      const int64_t w1 = (w0 + wc) < ((int64_t)hi + 1) ? (w0 + wc) : ((int64_t)hi + 1);
+      const int words = (int)((w1 - w0 + 31) >> 5);
       for (int t = tid; t < words; t += kHeavyT) bits[t] = 0u;
-      if (VALS)
-        for (int t = tid; t < width; t += kHeavyT) vals[t] = 0.0;
       for (int cb = s; cb < e; cb += kHeavyT) {
         const int p = cb + tid;
         int qs = 0, qe = 0;
-        double a = 0.0;
         if (p < e) {
           const int4 bm = A.bmeta[A.ac[p]];
           qs = (carry && pass > 0) ? cur[p] : bm.x;
           qe = bm.y;
-          if (carry && qs < qe && bm.w >= w1) {
+          if (carry && qs < qe && (int64_t)bm.w >= w1) {
             // first position of the B row at or past the window end
             int l = qs, h = qe;
             while (l < h) {
               const int mid = (int)(((unsigned)l + (unsigned)h) >> 1);
-              if (A.bc[mid] < w1) l = mid + 1;
+              if ((int64_t)A.bc[mid] < w1) l = mid + 1;
               else h = mid;
             }
             qe = l;
           }
           if (carry && pass + 1 < npass) cur[p] = qe;
-          if (VALS) a = A.av[p];
         }
         int64_t tot;
         const int64_t ex = block_scan_excl<int64_t>((int64_t)(qe - qs), hs.red, &tot);
         hs.off[tid] = ex;
         hs.qs[tid] = qs;
-        if (VALS) hs.a[tid] = a;
         const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
         __syncthreads();
         for (int64_t t = tid; t < tot; t += kHeavyT) {
@@ -2073,45 +2061,23 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy(RowArgs A, int32_t wc, int32_
           if (j < w0 || j >= w1) continue;   // only possible when B rows are rescanned
           const uint32_t b = (uint32_t)(j - w0);
           atomicOr(&bits[b >> 5], 1u << (b & 31u));
-          if (VALS && (!A.upper || j >= i)) atomicAdd(&vals[b], hs.a[l] * A.bv[q]);
         }
         __syncthreads();
       }
-      if (!VALS) {
-        int64_t c = 0, z = 0;
-        for (int t = tid; t < words; t += kHeavyT) {
-          const uint32_t x = bits[t];
-          c += __popc(x);
-          z += x != 0u ? 1 : 0;
-        }
-        int64_t tc, tz;
-        (void)block_scan_excl<int64_t>(c, hs.red, &tc);
-        (void)block_scan_excl<int64_t>(z, hs.red, &tz);
-        cnt += tc;
-        nzw += tz;
-      } else {
-        // compact the window into C: one word per thread, ascending columns
-        for (int wb = 0; wb < words; wb += kHeavyT) {
-          const int w = wb + tid;
-          const uint32_t x = w < words ? bits[w] : 0u;
-          int64_t tot;
-          int64_t o = out + block_scan_excl<int64_t>((int64_t)__popc(x), hs.red, &tot);
-          uint32_t y = x;
-          while (y) {
-            const int c = w * 32 + __builtin_ctz(y);
-            if (o < end) {
-              ccol[o] = w0 + c;
-              cval[o] = vals[c];
-            }
-            o++;
-            y &= y - 1u;
-          }
-          out += tot;
-        }
+      int64_t c = 0, z = 0;
+      for (int t = tid; t < words; t += kHeavyT) {
+        const uint32_t x = bits[t];
+        c += __popc(x);
+        z += x != 0u ? 1 : 0;
       }
+      int64_t tc, tz;
+      (void)block_scan_excl<int64_t>(c, hs.red, &tc);
+      (void)block_scan_excl<int64_t>(z, hs.red, &tz);
+      cnt += tc;
+      nzw += tz;
       __syncthreads();
     }
-    if (!VALS && tid == 0) {
+    if (tid == 0) {
       rowcnt[i] = (int32_t)cnt;
       A.rowwords[i] = (int32_t)nzw | kNoWordList;
     }
@@ -2132,7 +2098,7 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy(RowArgs A, int32_t wc, int32_
 // searches, scans and barriers).  Measured on R-MAT rows: rows spanning more
 // than kHeavySparseSpan dense windows run faster sparse (scale 21: 1.87 s
 // against 3.22 s), narrower ones dense (scale 16: 5.6 ms against 6.3 ms).
-// Cursors carry over between windows as in k_heavy.  LDS: bits (wcols / 32
+// Cursors carry over between windows as in k_heavy_sym.  LDS: bits (wcols / 32
 // words), then either prefix counts + vcap slots (sparse) or dcols slots
 // (dense).
 enum HeavyMode { kHeavyOr = 0, kHeavySlot = 1, kHeavyDense = 2 };
@@ -2994,9 +2960,8 @@ static void heavy_launch(bool vals, const RowArgs& a, const HeavyArgs& h, int32_
     hipLaunchKernelGGL(k_heavy_num, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_num, h.vcap, h.dcols, work,
                        h.bflag, h.cur, rowoff, ccol, cval);
   } else {
-    (void)hipFuncSetAttribute((const void*)k_heavy<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_heavy<false>, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_sym, work, h.bflag, h.cur, rowcnt,
-                       rowoff, ccol, cval);
+    (void)hipFuncSetAttribute((const void*)k_heavy_sym, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_heavy_sym, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_sym, work, h.bflag, h.cur, rowcnt);
   }
 }
 
