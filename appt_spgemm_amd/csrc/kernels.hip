@@ -2342,6 +2342,7 @@ template <int G, bool UPPER, bool W64, bool TRACK>
 __device__ __forceinline__ void or_chunk(int i, int lo, int lane, int m, bool lng, const RowArgs& A,
                                          const int2* mbe, uint32_t* bits, int W, uint32_t* touched, int* ctl) {
   constexpr int B = G < 32 ? G : 32;
+  static_assert(G % B == 0, "chunk depth must divide the group size");
   auto apply = [&](int j, bool ok) {
     const bool use = ok;
     const uint32_t b = (uint32_t)(j - lo);
@@ -2408,9 +2409,12 @@ __device__ __forceinline__ void value_chunk(int i, int lo, int lane, int m, bool
                                             const uint16_t* pre, const uint8_t* widx, const uint2* wtab,
                                             double* vals, int Cm, int W) {
   // gathers in flight per lane; the list variant trades depth for occupancy
-  // (fewer VGPRs: 4 waves per SIMD instead of 3)
-  constexpr int kDepth = LIST ? 8 : 16;
+  // (84 VGPRs, 5 waves per SIMD: 4 per chunk beat 8 at 4 waves per SIMD)
+  constexpr int kDepth = LIST ? 4 : 16;
   constexpr int B = G < kDepth ? G : kDepth;
+  // chunks tile the G staged entries exactly (entries past the row's count
+  // are zero-padded, never read past the staging)
+  static_assert(G % B == 0, "chunk depth must divide the group size");
   auto add = [&](int j, double prod, bool ok) {
     const bool use = ok && (!UPPER || j >= i);
     const uint32_t bb = use ? (uint32_t)(j - lo) : 0u;
@@ -2627,13 +2631,13 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
   constexpr int PL = (kCap + G - 1) / G;
   // per-row output head: offsets, word count, sorted word list
   struct Head {
-    int64_t off, end;
-    int rw;
+    int64_t off;
+    int c, rw;
     uint2 ent[PL];
   };
   auto load_head = [&](int i, Head& h) {  // scalars; the list one hop later
     h.off = rowoff[i];
-    h.end = rowoff[i + 1];
+    h.c = (int)(rowoff[i + 1] - h.off);
     h.rw = A.rowwords[i];
   };
   auto load_list = [&](int i, Head& h) {  // only the listed entries
@@ -2671,7 +2675,7 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
     const int i = q0.i;
     const int lo = q0.lo;
     const int64_t off = h0.off;
-    const int c = (int)(h0.end - h0.off);
+    const int c = h0.c;
     const int nt = h0.rw & kWordsMask;
     const bool have_list = (h0.rw & kNoWordList) == 0 && nt <= kCap;
     uint32_t myw[PL];
