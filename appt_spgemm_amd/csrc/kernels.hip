@@ -896,13 +896,22 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     }
     const bool lng = live && e - s > kPrepassLong;
     if (live && !lng) {
-      for (int p = s + lane; p < e; p += G) {
-        const int4 bm = bmeta[ac[p]];
-        const int len = bm.y - bm.x;
-        f += len;
-        if (len > 0) {
-          lo = bm.z < lo ? bm.z : lo;
-          hi = bm.w > hi ? bm.w : hi;
+      // 4 entries per lane per step: their column and B-row loads in flight together
+      for (int p0 = s + lane; p0 < e; p0 += 4 * G) {
+        int k[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) k[u] = p0 + u * G < e ? ac[p0 + u * G] : -1;
+        int4 bm[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) bm[u] = k[u] >= 0 ? bmeta[k[u]] : make_int4(0, 0, INT32_MAX, -1);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int len = bm[u].y - bm[u].x;
+          f += len;
+          if (len > 0) {
+            lo = bm[u].z < lo ? bm[u].z : lo;
+            hi = bm[u].w > hi ? bm[u].w : hi;
+          }
         }
       }
     }
