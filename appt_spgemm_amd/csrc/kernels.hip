@@ -1140,9 +1140,7 @@ __device__ __forceinline__ RowWalk row_walk(int nrows, int R, int gi) {
 // remainder (B rows longer than G) before the next entry, so every output
 // column still receives its contributions in A-row storage order.
 // G > 64: the whole workgroup owns the row; A-entry metadata is staged in LDS.
-// WALK (structure only): waves walk the B rows (the symbolic pass; the
-// numeric structure pass is faster with a lane per short B row).
-template <int G, bool VALS, int U = 8, bool WALK = false, typename Fn>
+template <int G, bool VALS, int U = 8, typename Fn>
 __device__ __forceinline__ void for_each_product(int i, int lane, const RowArgs& A, Stage* stage, Fn&& fn) {
   const int s = A.ap[i], e = A.ap[i + 1];
   for (int base = s; base < e; base += G) {
@@ -1178,53 +1176,11 @@ __device__ __forceinline__ void for_each_product(int i, int lane, const RowArgs&
           for (int q = qb[u] + G; q < qe[u]; q += G) fn(A.bc[q], VALS ? aa[u] * A.bv[q] : 0.0);
         }
       }
-    } else if constexpr (WALK && !VALS) {
-      // whole workgroup per row, structure only (the order of contributions
-      // does not matter): the chunk's A entries are staged in LDS and each
-      // wave walks their B rows U at a time, claimed from a counter: lanes
-      // over B positions, 64 per step, the U rows' steps interleaved (U
-      // gathers in flight per lane)
-      __syncthreads();
-      stage->bs[lane] = bs;
-      stage->be[lane] = be;
-      if (lane == 0) stage->nlong = 0;
-      __syncthreads();
-      const int wl = lane % kWave;
-      while (true) {
-        int g = 0;
-        if (wl == 0) g = atomicAdd(&stage->nlong, U);
-        g = __builtin_amdgcn_readfirstlane(__shfl(g, 0, kWave));
-        if (g >= m) break;
-        int q[U], qe[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-          const bool has = g + u < m;
-          q[u] = __builtin_amdgcn_readfirstlane(has ? stage->bs[g + u] : 0);
-          qe[u] = __builtin_amdgcn_readfirstlane(has ? stage->be[g + u] : 0);
-        }
-        while (true) {
-          bool any = false;
-#pragma unroll
-          for (int u = 0; u < U; u++) any |= q[u] < qe[u];
-          if (!any) break;
-          int jj[U];
-#pragma unroll
-          for (int u = 0; u < U; u++) {
-            const int qq = q[u] + wl;
-            jj[u] = qq < qe[u] ? A.bc[qq] : -1;
-          }
-#pragma unroll
-          for (int u = 0; u < U; u++) {
-            if (jj[u] >= 0) fn(jj[u], 0.0);
-            q[u] += kWave;
-          }
-        }
-      }
     } else {
-      // whole workgroup per row: products are balanced over the lanes: B
-      // rows of at most kShortRow entries are walked one lane per A entry,
-      // longer ones with all lanes over the B row (walking every B row by
-      // waves, as the symbolic pass does, was slower here)
+      // whole workgroup per row; the order of contributions is not fixed
+      // here (atomics), so products are balanced over the lanes: B rows of
+      // at most kShortRow entries are walked one lane per A entry, longer
+      // ones with all lanes over the B row
       constexpr int kShortRow = 32;
       __syncthreads();
       stage->bs[lane] = bs;
@@ -1301,7 +1257,7 @@ __global__ void __launch_bounds__(256) k_symbolic_lds(RowArgs A, int32_t row_wor
     const int words = (int)span_words(lo, hi);
     for (int t = lane; t < words; t += G) tab[t] = 0u;
     gsync<G>();
-    for_each_product<G, false, 8, true>(i, lane, A, stage_buf, [&](int j, double) {
+    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
       const uint32_t b = (uint32_t)(j - lo);
       const uint32_t bit = 1u << (b & 31u);
       const uint32_t old = atomicOr(&tab[b >> 5], bit);
@@ -1325,7 +1281,7 @@ __global__ void __launch_bounds__(256) k_symbolic_lds(RowArgs A, int32_t row_wor
     uint32_t* masks = tab + T;
     for (int t = lane; t < (int)T; t += G) { keys[t] = -1; masks[t] = 0u; }
     gsync<G>();
-    for_each_product<G, false, 8, true>(i, lane, A, stage_buf, [&](int j, double) {
+    for_each_product<G, false>(i, lane, A, stage_buf, [&](int j, double) {
       bool claimed;
       const uint32_t h = whash_insert(keys, logT, mask, j >> 5, &claimed);
       nw += claimed ? 1 : 0;
