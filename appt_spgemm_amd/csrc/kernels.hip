@@ -2441,6 +2441,9 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
   uint32_t* touched = (uint32_t*)((unsigned char*)bits + align16((skewed_words(W) + G) * 4));
   int* ctl = (int*)((unsigned char*)touched + align16(kTouchCap * 4));
   for (int t = lane; t < skewed_words(W) + G; t += G) bits[t] = 0u;
+  // unused touched entries hold a sentinel above every word index, so the
+  // rank loop reads them four at a time without bounds tests
+  for (int t = lane; t < kTouchCap; t += G) touched[t] = ~0u;
   const RowWalk walk = row_walk(A.nrows, R, gi);
   const int S = walk.step, r1 = walk.end;
   int r = walk.first;
@@ -2490,20 +2493,29 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
       }
       if (nt <= kWordCap) {
         uint2* wl = A.wlist + (size_t)i * kWordCap;
+        // rank = touched words below this lane's; each 16-byte broadcast
+        // read serves all of the lane's slots (entries past nt are ~0u)
+        const uint4* t4 = (const uint4*)touched;
+        int rank[PL];
 #pragma unroll
-        for (int k = 0; k < PL; k++) {
-          const int u = lane + k * G;
-          if (u < nt) {
-            int rank = 0;
-            for (int v = 0; v < nt; v++) rank += touched[v] < myw[k] ? 1 : 0;
-            wl[rank] = make_uint2(myw[k], mym[k]);
-          }
+        for (int k = 0; k < PL; k++) rank[k] = 0;
+        for (int v = 0; v < nt; v += 4) {
+          const uint4 t = t4[v >> 2];
+#pragma unroll
+          for (int k = 0; k < PL; k++)
+            rank[k] += (int)(t.x < myw[k]) + (int)(t.y < myw[k]) + (int)(t.z < myw[k]) + (int)(t.w < myw[k]);
         }
+#pragma unroll
+        for (int k = 0; k < PL; k++)
+          if (lane + k * G < nt) wl[rank[k]] = make_uint2(myw[k], mym[k]);
       }
       gsync<G>();
 #pragma unroll
       for (int k = 0; k < PL; k++)
-        if (lane + k * G < nt) bits[bank_skew(myw[k])] = 0u;
+        if (lane + k * G < nt) {
+          bits[bank_skew(myw[k])] = 0u;
+          touched[lane + k * G] = ~0u;
+        }
     } else {
       const int words = (int)span_words(lo, A.rowhi[i]);
       for (int w = lane; w < words; w += G) {
@@ -2511,6 +2523,7 @@ __global__ void __launch_bounds__(256) k_sym_window(RowArgs A, int32_t W, int32_
         c += __popc(bits[sw]);
         bits[sw] = 0u;
       }
+      for (int t = lane; t < kTouchCap; t += G) touched[t] = ~0u;
     }
     c = gsum<G, int32_t>(c, nullptr);
     if (lane == 0) { rowcnt[i] = c; A.rowwords[i] = nt; }
