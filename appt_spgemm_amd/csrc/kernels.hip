@@ -1951,7 +1951,8 @@ static void num_sort_launch(const RowArgs& a, int32_t na_max, const int64_t* row
 // each product is loaded once; otherwise each window rescans the B rows and
 // filters by column.  Inside a window the products of kHeavyT A entries are
 // flattened (exclusive scan of their segment lengths) and dealt to the
-// threads one product each, so short and long B rows balance.
+// threads, kHeavyU products per thread in flight, so short and long B rows
+// balance.
 // ---------------------------------------------------------------------------
 struct HeavyStage {
   int64_t off[kHeavyT];  // exclusive scan of the entries' segment lengths
@@ -1960,6 +1961,132 @@ struct HeavyStage {
   int64_t red[17];
   int32_t row;
 };
+
+// B ranges of a chunk of m A entries (from position cb) inside the window
+// ending at wend: entry k's range [qs, qe) goes to hs.qs[k] and hs.off[k]
+// (parked there until the caller's scan), a_ik to hs.a[k] when VALS.  With
+// carry (B rows sorted), qs is the entry's cursor after the first window and
+// qe the first B position at or past wend; keep moves the cursor there.
+template <bool VALS>
+__device__ __forceinline__ void heavy_stage_ranges(const RowArgs& A, HeavyStage& hs, int cb, int m, int64_t wend,
+                                                   bool carry, bool first, bool keep, int32_t* cur) {
+  const int tid = threadIdx.x;
+  // each entry's B range inside the window, T lanes per entry (T = 1024 /
+  // m rounded down to a power of two, at most 64): the window end is found
+  // by a T-ary search, log_(T+1) dependent loads instead of log_2.  The
+  // group's lane 0 stages qs, qe (parked in hs.off) and a_ik.
+  int lg = 0;
+  while (lg < 6 && m * (2 << lg) <= kHeavyT) lg++;
+  const int T = 1 << lg;
+  const int ent = tid >> lg, sub = tid & (T - 1);
+  if (ent < m) {
+    const int p = cb + ent;
+    const int4 bm = A.bmeta[A.ac[p]];
+    const int qs = (carry && !first) ? cur[p] : bm.x;
+    int l = qs, h = bm.y;
+    if (carry && l < h && (int64_t)bm.w >= wend) {
+      // first position of the B row at or past the window end; l, h are
+      // uniform over the group, whose lanes are active together
+      const int base = (tid & (kWave - 1)) & ~(T - 1);
+      const uint64_t gmask = T == kWave ? ~0ull : ((1ull << T) - 1ull);
+      while (l < h) {
+        const int len = h - l;
+        const bool small = len <= T;
+        const bool valid = !small || sub < len;
+        const int pos = small ? (valid ? l + sub : l) : l + (int)((int64_t)len * (sub + 1) / (T + 1));
+        const bool below = valid && (int64_t)A.bc[pos] < wend;
+        const int c = __popcll((__ballot(below) >> base) & gmask);
+        if (small) {
+          l += c;
+          h = l;
+        } else {
+          const int nl = c > 0 ? l + (int)((int64_t)len * c / (T + 1)) + 1 : l;
+          const int nh = c < T ? l + (int)((int64_t)len * (c + 1) / (T + 1)) : h;
+          l = nl;
+          h = nh;
+        }
+      }
+    } else {
+      l = h;
+    }
+    if (sub == 0) {
+      hs.qs[ent] = qs;
+      hs.off[ent] = l;
+      if (carry && keep) cur[p] = l;
+      if (VALS) hs.a[ent] = A.av[p];
+    }
+  }
+}
+
+enum HeavyMode { kHeavyOr = 0, kHeavySlot = 1, kHeavyDense = 2 };
+constexpr int64_t kHeavySparseSpan = 8;
+constexpr int kHeavyU = 4;  // products in flight per thread in heavy_products
+
+template <int MODE>
+__device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs, int i, int s, int e, int64_t w0,
+                                               int64_t wend, bool sorted, bool first, bool keep, int32_t* cur,
+                                               uint32_t* bits, const uint32_t* pre, double* vals) {
+  const int tid = threadIdx.x;
+  for (int cb = s; cb < e; cb += kHeavyT) {
+    const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
+    heavy_stage_ranges<MODE != kHeavyOr>(A, hs, cb, m, wend, sorted, first, keep, cur);
+    __syncthreads();
+    const int qs = tid < m ? hs.qs[tid] : 0;
+    const int qe = tid < m ? (int)hs.off[tid] : 0;
+    int64_t tot;
+    const int64_t ex = block_scan_excl<int64_t>((int64_t)(qe - qs), hs.red, &tot);
+    hs.off[tid] = ex;
+    __syncthreads();
+    // kHeavyU products per thread per step (t, t + 1024, ...): their owner
+    // searches, gathers and LDS updates are issued together.  The owner of
+    // t is the last entry whose segment starts at or before t, found by a
+    // fixed-depth search (l = 0 holds: hs.off[0] == 0).
+    int depth = 0;
+    while ((1 << depth) < m) depth++;
+    for (int64_t t0 = tid; t0 < tot; t0 += (int64_t)kHeavyU * kHeavyT) {
+      int64_t tt[kHeavyU];
+      int l[kHeavyU];
+#pragma unroll
+      for (int u = 0; u < kHeavyU; u++) {
+        tt[u] = t0 + (int64_t)u * kHeavyT;
+        l[u] = 0;
+      }
+      for (int d = depth - 1; d >= 0; d--) {
+#pragma unroll
+        for (int u = 0; u < kHeavyU; u++) {
+          const int cand = l[u] + (1 << d);
+          if (cand < m && hs.off[cand] <= tt[u]) l[u] = cand;
+        }
+      }
+      int q[kHeavyU], j[kHeavyU];
+      double pv[kHeavyU];
+#pragma unroll
+      for (int u = 0; u < kHeavyU; u++) {
+        // steps past tot read B position qs of their owner's (in-range) start
+        const bool ok = tt[u] < tot;
+        q[u] = hs.qs[l[u]] + (ok ? (int)(tt[u] - hs.off[l[u]]) : 0);
+        j[u] = ok ? A.bc[q[u]] : -1;
+        if (MODE != kHeavyOr) pv[u] = ok ? hs.a[l[u]] * A.bv[q[u]] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kHeavyU; u++) {
+        if (j[u] < w0 || j[u] >= wend) continue;  // past tot, or B rows rescanned
+        const uint32_t b = (uint32_t)(j[u] - w0);
+        if (MODE == kHeavyOr) {
+          atomicOr(&bits[b >> 5], 1u << (b & 31u));
+        } else if (MODE == kHeavyDense) {
+          atomicOr(&bits[b >> 5], 1u << (b & 31u));
+          if (!A.upper || j[u] >= i) atomicAdd(&vals[b], pv[u]);
+        } else if (!A.upper || j[u] >= i) {
+          const uint32_t w = b >> 5;
+          const uint32_t pos = pre[w] + __popc(bits[w] & ((1u << (b & 31u)) - 1u));
+          atomicAdd(&vals[pos], pv[u]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
 
 // Symbolic heavy rows: windows of `wc` columns, a bitmap each; the row's
 // count and distinct words are summed over the windows.
@@ -1988,47 +2115,8 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_sym(RowArgs A, int32_t wc, in
       const int64_t w1 = (w0 + wc) < ((int64_t)hi + 1) ? (w0 + wc) : ((int64_t)hi + 1);
       const int words = (int)((w1 - w0 + 31) >> 5);
       for (int t = tid; t < words; t += kHeavyT) bits[t] = 0u;
-      for (int cb = s; cb < e; cb += kHeavyT) {
-        const int p = cb + tid;
-        int qs = 0, qe = 0;
-        if (p < e) {
-          const int4 bm = A.bmeta[A.ac[p]];
-          qs = (carry && pass > 0) ? cur[p] : bm.x;
-          qe = bm.y;
-          if (carry && qs < qe && (int64_t)bm.w >= w1) {
-            // first position of the B row at or past the window end
-            int l = qs, h = qe;
-            while (l < h) {
-              const int mid = (int)(((unsigned)l + (unsigned)h) >> 1);
-              if ((int64_t)A.bc[mid] < w1) l = mid + 1;
-              else h = mid;
-            }
-            qe = l;
-          }
-          if (carry && pass + 1 < npass) cur[p] = qe;
-        }
-        int64_t tot;
-        const int64_t ex = block_scan_excl<int64_t>((int64_t)(qe - qs), hs.red, &tot);
-        hs.off[tid] = ex;
-        hs.qs[tid] = qs;
-        const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
-        __syncthreads();
-        for (int64_t t = tid; t < tot; t += kHeavyT) {
-          // owner: the last entry whose segment starts at or before t
-          int l = 0, h = m - 1;
-          while (l < h) {
-            const int mid = (l + h + 1) >> 1;
-            if (hs.off[mid] <= t) l = mid;
-            else h = mid - 1;
-          }
-          const int q = hs.qs[l] + (int)(t - hs.off[l]);
-          const int j = A.bc[q];
-          if (j < w0 || j >= w1) continue;   // only possible when B rows are rescanned
-          const uint32_t b = (uint32_t)(j - w0);
-          atomicOr(&bits[b >> 5], 1u << (b & 31u));
-        }
-        __syncthreads();
-      }
+      heavy_products<kHeavyOr>(A, hs, -1, s, e, w0, w1, carry, pass == 0, pass + 1 < npass, cur, bits, nullptr,
+                               nullptr);
       int64_t c = 0, z = 0;
       for (int t = tid; t < words; t += kHeavyT) {
         const uint32_t x = bits[t];
@@ -2066,67 +2154,6 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_sym(RowArgs A, int32_t wc, in
 // Cursors carry over between windows as in k_heavy_sym.  LDS: bits (wcols / 32
 // words), then either prefix counts + vcap slots (sparse) or dcols slots
 // (dense).
-enum HeavyMode { kHeavyOr = 0, kHeavySlot = 1, kHeavyDense = 2 };
-constexpr int64_t kHeavySparseSpan = 8;
-
-template <int MODE>
-__device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs, int i, int s, int e, int64_t w0,
-                                               int64_t wend, bool sorted, bool first, bool keep, int32_t* cur,
-                                               uint32_t* bits, const uint32_t* pre, double* vals) {
-  const int tid = threadIdx.x;
-  for (int cb = s; cb < e; cb += kHeavyT) {
-    const int p = cb + tid;
-    int qs = 0, qe = 0;
-    double a = 0.0;
-    if (p < e) {
-      const int4 bm = A.bmeta[A.ac[p]];
-      qs = (sorted && !first) ? cur[p] : bm.x;
-      qe = bm.y;
-      if (sorted && qs < qe && (int64_t)bm.w >= wend) {
-        int l = qs, h = qe;  // first position of the B row at or past the window end
-        while (l < h) {
-          const int mid = (int)(((unsigned)l + (unsigned)h) >> 1);
-          if ((int64_t)A.bc[mid] < wend) l = mid + 1;
-          else h = mid;
-        }
-        qe = l;
-      }
-      if (sorted && keep) cur[p] = qe;
-      if (MODE != kHeavyOr) a = A.av[p];
-    }
-    int64_t tot;
-    const int64_t ex = block_scan_excl<int64_t>((int64_t)(qe - qs), hs.red, &tot);
-    hs.off[tid] = ex;
-    hs.qs[tid] = qs;
-    if (MODE != kHeavyOr) hs.a[tid] = a;
-    const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
-    __syncthreads();
-    for (int64_t t = tid; t < tot; t += kHeavyT) {
-      int l = 0, h = m - 1;  // owner: the last entry whose segment starts at or before t
-      while (l < h) {
-        const int mid = (l + h + 1) >> 1;
-        if (hs.off[mid] <= t) l = mid;
-        else h = mid - 1;
-      }
-      const int q = hs.qs[l] + (int)(t - hs.off[l]);
-      const int j = A.bc[q];
-      if (j < w0 || j >= wend) continue;  // only when B rows are rescanned
-      const uint32_t b = (uint32_t)(j - w0);
-      if (MODE == kHeavyOr) {
-        atomicOr(&bits[b >> 5], 1u << (b & 31u));
-      } else if (MODE == kHeavyDense) {
-        atomicOr(&bits[b >> 5], 1u << (b & 31u));
-        if (!A.upper || j >= i) atomicAdd(&vals[b], hs.a[l] * A.bv[q]);
-      } else if (!A.upper || j >= i) {
-        const uint32_t w = b >> 5;
-        const uint32_t pos = pre[w] + __popc(bits[w] & ((1u << (b & 31u)) - 1u));
-        atomicAdd(&vals[pos], hs.a[l] * A.bv[q]);
-      }
-    }
-    __syncthreads();
-  }
-}
-
 __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols, int32_t vcap, int32_t dcols,
                                                        int32_t* __restrict__ work, const int32_t* __restrict__ bflag,
                                                        int32_t* __restrict__ cur, const int64_t* __restrict__ rowoff,
