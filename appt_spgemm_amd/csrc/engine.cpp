@@ -432,7 +432,10 @@ void Engine::stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, 
       ra.wgslot = nullptr;
       ra.wgstride = 0;
     }
-    launch_symbolic_bin(b, ra, wmax, rowcnt, heavy, s);
+    if (b == kBinHeavy && heavy_plan_cnt_ != nullptr)
+      launch_take_plan(ra, heavy_plan_r0_, heavy_plan_cnt_, heavy_plan_words_, rowcnt, s);
+    else
+      launch_symbolic_bin(b, ra, wmax, rowcnt, heavy, s);
     off += cnt;
   }
 }
@@ -546,10 +549,15 @@ void Engine::symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr
   stage_operands(W, A, op, O);
   stage_prepass(W, O, cfg_);
   read_sym(W, P, O.upper != 0);
+  plan_key_ = RowPlanKey{};
   stage_symbolic(W, O, cfg_, P);
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)A.n + 1)));
   try {
     stage_offsets(W, O, cfg_, rp32);
+    // keep the rows' counts and distinct words for the panels that follow
+    const size_t rb = sizeof(int32_t) * (size_t)std::max(A.n, 1);
+    APPT_HIP(hipMemcpyAsync(ensure(W.plancnt, rb), W.rowcnt.p, rb, hipMemcpyDeviceToDevice, stream_));
+    APPT_HIP(hipMemcpyAsync(ensure(W.planwords, rb), W.rowwords.p, rb, hipMemcpyDeviceToDevice, stream_));
     APPT_HIP(hipMemcpyAsync(rowptr_host, W.rowoff.p, sizeof(int64_t) * ((size_t)A.n + 1), hipMemcpyDeviceToHost,
                             stream_));
     APPT_HIP(hipStreamSynchronize(stream_));
@@ -558,6 +566,7 @@ void Engine::symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr
     throw;
   }
   pool_.put(rp32);
+  plan_key_ = RowPlanKey{A.rowptr, A.col, A.val, A.n, A.nnz, (int)op};
 }
 
 void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_t r1, appt_csr_result* C) {
@@ -584,7 +593,21 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   stage_prepass(W, O, cfg_);
   read_sym(W, P, O.upper != 0);
   record(2);
-  stage_symbolic(W, O, cfg_, P);
+  {
+    // heavy rows reuse the row-pointer plan of this view and op, if any
+    const bool planned = plan_key_.rowptr == A.rowptr && plan_key_.col == A.col && plan_key_.val == A.val &&
+                         plan_key_.n == A.n && plan_key_.nnz == A.nnz && plan_key_.op == (int)op;
+    struct Reset {
+      Engine* e;
+      ~Reset() { e->heavy_plan_cnt_ = nullptr; e->heavy_plan_words_ = nullptr; e->heavy_plan_r0_ = 0; }
+    } reset{this};
+    if (planned) {
+      heavy_plan_cnt_ = as<const int32_t>(W.plancnt.p);
+      heavy_plan_words_ = as<const int32_t>(W.planwords.p);
+      heavy_plan_r0_ = r0;
+    }
+    stage_symbolic(W, O, cfg_, P);
+  }
   record(3);
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)n + 1)));
   int32_t* ccol = nullptr;

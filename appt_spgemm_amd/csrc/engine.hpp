@@ -72,7 +72,7 @@ struct Config {
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
       numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
-      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot;
+      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot, plancnt, planwords;
   // words per row of the stored workgroup-bin bitmaps of the current call
   // (0: none stored)
   int32_t wg_stride = 0;
@@ -81,7 +81,7 @@ struct Workspace {
     Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
                       &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp,
-                      &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot};
+                      &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot, &plancnt, &planwords};
     for (Scratch* s : all) f(*s);
   }
 };
@@ -202,6 +202,21 @@ class Engine {
   bool own_stream_ = false;
   Pool pool_;
   Workspace ws_;
+  // Row-pointer plan of the last symbolic_rowptr call: the A view and op it
+  // was made for, and its per-row counts and distinct words (ws_.plancnt,
+  // ws_.planwords).  compute_rows on the same view and op takes its heavy
+  // rows' symbolic results from it (heavy_plan_ set for that call only).
+  struct RowPlanKey {
+    const void* rowptr = nullptr;
+    const void* col = nullptr;
+    const void* val = nullptr;
+    int32_t n = -1;
+    int64_t nnz = -1;
+    int op = -1;
+  } plan_key_;
+  const int32_t* heavy_plan_cnt_ = nullptr;
+  const int32_t* heavy_plan_words_ = nullptr;
+  int32_t heavy_plan_r0_ = 0;
   Config cfg_;
   HostStats* hs_ = nullptr;
   bool timing_ = false;

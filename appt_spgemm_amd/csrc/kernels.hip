@@ -2974,6 +2974,27 @@ static void heavy_launch(bool vals, const RowArgs& a, const HeavyArgs& h, int32_
   }
 }
 
+// Heavy rows of a row panel taking their symbolic results from a row-pointer
+// plan over all rows: panel row i is plan row r0 + i.  The plan's distinct-
+// word count is kept; such rows never have a word list.
+__global__ void k_take_plan(const int32_t* __restrict__ rows, int32_t nrows, int32_t r0,
+                            const int32_t* __restrict__ pcnt, const int32_t* __restrict__ pwords,
+                            int32_t* __restrict__ rowcnt, int32_t* __restrict__ rowwords) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < nrows; r += (int64_t)gridDim.x * blockDim.x) {
+    const int i = rows[r];
+    rowcnt[i] = pcnt[r0 + i];
+    rowwords[i] = pwords[r0 + i] | kNoWordList;
+  }
+}
+void launch_take_plan(const RowArgs& a, int32_t r0, const int32_t* pcnt, const int32_t* pwords, int32_t* rowcnt,
+                      hipStream_t s) {
+  if (a.nrows <= 0) return;
+  int64_t blocks = ((int64_t)a.nrows + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_take_plan, dim3((unsigned)blocks), dim3(256), 0, s, a.rows, a.nrows, r0, pcnt, pwords, rowcnt,
+                     a.rowwords);
+}
+
 void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* rowcnt, const HeavyArgs& h,
                          hipStream_t s) {
   if (a.nrows <= 0 || bin <= 0) return;

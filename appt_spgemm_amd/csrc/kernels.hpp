@@ -184,6 +184,11 @@ int64_t heavy_lds_bytes(bool vals, int32_t wc, int32_t vcap, int32_t dcols);
 
 // Symbolic: rowcnt[i] = distinct output columns of row i.
 // words_max: the bin's largest LDS statistic (span words or hash keys).
+// Symbolic results of a bin's rows (panel-local ids, a.rows) copied from a
+// row-pointer plan over all rows: rowcnt[i] = pcnt[r0 + i], rowwords[i] =
+// pwords[r0 + i] without a word list.
+void launch_take_plan(const RowArgs& a, int32_t r0, const int32_t* pcnt, const int32_t* pwords, int32_t* rowcnt,
+                      hipStream_t s);
 void launch_symbolic_bin(int bin, const RowArgs& a, int32_t words_max, int32_t* rowcnt, const HeavyArgs& h,
                          hipStream_t s);
 // Numeric: writes sorted col/val of each row at rowoff[i].

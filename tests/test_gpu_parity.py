@@ -493,6 +493,56 @@ def test_row_panels(eng, op):
         A.free()
 
 
+@pytest.mark.parametrize("op", ["AA", "AAt"])
+def test_row_panels_planned_heavy_rows(eng_heavy, op):
+    # heavy rows of panels take their counts from the row-pointer plan of the
+    # same view; a plan-free engine computes the same panels from scratch.
+    # Identity plus a dense row 0 and column-0 entries in 40 rows: those rows'
+    # outputs span all n columns, past this engine's workgroup bins (heavy)
+    n = 12000
+    rng = np.random.default_rng(12)
+    some = rng.choice(np.arange(1, n), 40, replace=False)
+    r = np.concatenate([np.arange(n), np.zeros(n - 1, np.int64), some])
+    c = np.concatenate([np.arange(n), np.arange(1, n), np.zeros(40, np.int64)])
+    key = np.unique(r * n + c)
+    r, c = key // n, key % n
+    rp = np.zeros(n + 1, np.int64); np.add.at(rp, r + 1, 1); rp = np.cumsum(rp)
+    m = host(n, rp, c, rng.uniform(0, 1, len(c)))
+    say("case planned heavy panels n=12000", op)
+    want = ref(m, op)
+    A = eng_heavy.upload(m)
+    fresh = ap.Engine(0, tuning={"wg_lds_bytes": 1024, "heavy_window_cols": 96})
+    B = fresh.upload(m)
+    try:
+        rp = eng_heavy.symbolic_rowptr(A, op)
+        np.testing.assert_array_equal(rp, want[0])
+        # about five panels, so plan rows are taken at non-zero panel offsets
+        parts = ap.panels(rp, max_bytes=(12 * int(rp[-1]) + 8 * (n + 1)) // 5)
+        assert len(parts) > 1
+        cols, vals, rps = [], [], [np.zeros(1, np.int64)]
+        heavy_rows = 0
+        for r0, r1 in parts:
+            R = eng_heavy.compute_rows(A, op, r0, r1)
+            heavy_rows += eng_heavy.bin_report()["symbolic"].get("heavy", (0, 0))[0]
+            F = fresh.compute_rows(B, op, r0, r1)
+            try:
+                h, f = R.download(), F.download()
+            finally:
+                R.release()
+                F.release()
+            np.testing.assert_array_equal(h.rowptr, f.rowptr)
+            np.testing.assert_array_equal(h.col, f.col)
+            cols.append(h.col); vals.append(h.val)
+            rps.append(np.asarray(h.rowptr[1:], np.int64) + rps[-1][-1])
+        assert heavy_rows > 0
+        got = ap.HostMatrix(m.n, np.concatenate(rps), np.concatenate(cols), np.concatenate(vals))
+        assert_product(got, want, exact=False)
+    finally:
+        A.free()
+        B.free()
+        fresh.close()
+
+
 def test_row_panels_reject_aats(eng):
     m = ap.laplace2d(16)
     A = eng.upload(m)
