@@ -535,7 +535,7 @@ void Engine::stage_mirror(Workspace& W, const Operands& O, const Plan& P, const 
 // one call
 // ---------------------------------------------------------------------------
 void Engine::compute(const appt_csr_view& A, appt_op op, appt_csr_result* C) {
-  compute_rows(A, op, 0, A.n, C);
+  compute_rows(A, op, 0, A.n, C, false);
 }
 
 void Engine::symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr_host) {
@@ -569,7 +569,8 @@ void Engine::symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr
   plan_key_ = RowPlanKey{A.rowptr, A.col, A.val, A.n, A.nnz, (int)op};
 }
 
-void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_t r1, appt_csr_result* C) {
+void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_t r1, appt_csr_result* C,
+                          bool use_plan) {
   validate(A);
   if (!C) throw Error(APPT_ERR_INVALID, "result pointer is NULL");
   if (op != APPT_OP_AA && op != APPT_OP_AAT && op != APPT_OP_AATS) throw Error(APPT_ERR_INVALID, "bad op");
@@ -595,8 +596,9 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   record(2);
   {
     // heavy rows reuse the row-pointer plan of this view and op, if any
-    const bool planned = plan_key_.rowptr == A.rowptr && plan_key_.col == A.col && plan_key_.val == A.val &&
-                         plan_key_.n == A.n && plan_key_.nnz == A.nnz && plan_key_.op == (int)op;
+    const bool planned = use_plan && plan_key_.rowptr == A.rowptr && plan_key_.col == A.col &&
+                         plan_key_.val == A.val && plan_key_.n == A.n && plan_key_.nnz == A.nnz &&
+                         plan_key_.op == (int)op;
     struct Reset {
       Engine* e;
       ~Reset() { e->heavy_plan_cnt_ = nullptr; e->heavy_plan_words_ = nullptr; e->heavy_plan_r0_ = 0; }

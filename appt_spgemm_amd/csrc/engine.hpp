@@ -140,7 +140,10 @@ class Engine {
 
   void compute(const appt_csr_view& A, appt_op op, appt_csr_result* C);
   // Rows [r0, r1) of op(A) (a row panel): C has r1 - r0 rows, global columns.
-  void compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_t r1, appt_csr_result* C);
+  // use_plan: heavy rows may take their counts from the last symbolic_rowptr
+  // of the same view and op (the panel API; compute() never does).
+  void compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_t r1, appt_csr_result* C,
+                    bool use_plan = true);
   // Exact row pointers of op(A) (symbolic phase only), copied to the host.
   void symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr_host);
   void release(appt_csr_result* C);

@@ -189,10 +189,11 @@ appt_status appt_spgemm_compute_rows(appt_engine* eng, const appt_csr_view* A, a
 /* Exact row pointers of op(A) (n+1 int64, host memory) from the symbolic
  * phase alone: sizes row panels before any numeric work.  The engine keeps
  * the rows' symbolic results (8 bytes per row) until its next call of this
- * function: appt_spgemm_compute_rows and appt_spgemm_compute on the same view
- * (rowptr, col, val, n, nnz) and op take their heavy rows' counts from them
- * instead of recomputing them, so A's contents must not change in between
- * (plan again after changing them). */
+ * function: appt_spgemm_compute_rows on the same view (rowptr, col, val, n,
+ * nnz) and op takes its heavy rows' counts from them instead of recomputing
+ * them, so A's contents must not change in between, nor may another matrix
+ * take over the same device pointers (plan again first).  appt_spgemm_compute
+ * never uses the plan. */
 appt_status appt_spgemm_symbolic_rowptr(appt_engine* eng, const appt_csr_view* A, appt_op op, int64_t* rowptr);
 
 /* run_ours(task, dOff, dCol, dVal, n, nnz, reps) (src/spgemm_engine.cuh:132-133):
