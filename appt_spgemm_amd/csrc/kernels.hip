@@ -2024,12 +2024,13 @@ constexpr int kHeavyU = 4;  // products in flight per thread in heavy_products
 
 template <int MODE>
 __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs, int i, int s, int e, int64_t w0,
-                                               int64_t wend, bool sorted, bool first, bool keep, int32_t* cur,
+                                               int64_t wend, bool carry, bool first, bool keep, int32_t* cur,
                                                uint32_t* bits, const uint32_t* pre, double* vals) {
+  // carry: B rows sorted and cursors kept across windows (heavy_stage_ranges)
   const int tid = threadIdx.x;
   for (int cb = s; cb < e; cb += kHeavyT) {
     const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
-    heavy_stage_ranges<MODE != kHeavyOr>(A, hs, cb, m, wend, sorted, first, keep, cur);
+    heavy_stage_ranges<MODE != kHeavyOr>(A, hs, cb, m, wend, carry, first, keep, cur);
     __syncthreads();
     const int qs = tid < m ? hs.qs[tid] : 0;
     const int qe = tid < m ? (int)hs.off[tid] : 0;
