@@ -260,14 +260,18 @@ class Engine:
         return DeviceResult(self, r)
 
     def compute_rows(self, A: DeviceMatrix, op: str, r0: int, r1: int) -> DeviceResult:
-        """Row panel [r0, r1) of op(A) (global column indices)."""
+        """Row panel [r0, r1) of op(A) (global column indices).  After
+        symbolic_rowptr(A, op) on this engine, heavy rows take their counts
+        from that plan: A must not change in between, and a matrix uploaded
+        into the freed buffers of a planned one must be planned again."""
         r = CsrResult()
         check(lib().appt_spgemm_compute_rows(self.handle, C.byref(A.view), OPS[op], r0, r1, C.byref(r)),
               f"appt_spgemm_compute_rows({op}, {r0}, {r1})")
         return DeviceResult(self, r)
 
     def symbolic_rowptr(self, A: DeviceMatrix, op: str = "AA") -> np.ndarray:
-        """Exact int64 row pointers of op(A) from the symbolic phase."""
+        """Exact int64 row pointers of op(A) from the symbolic phase; the
+        engine keeps the rows' symbolic results for compute_rows on A."""
         rp = np.empty(A.n + 1, dtype=np.int64)
         check(lib().appt_spgemm_symbolic_rowptr(self.handle, C.byref(A.view), OPS[op],
                                                 rp.ctypes.data_as(C.POINTER(C.c_int64))), "symbolic_rowptr")
