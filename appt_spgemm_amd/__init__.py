@@ -1,9 +1,12 @@
 """MI355X (gfx950) SpGEMM engine: C = A*A, A*A^T (plain and symmetric-Gram)
 for int32/fp64 CSR, behind the C ABI in include/appt_spgemm.h."""
 from ._abi import ApptError, lib  # noqa: F401
-from .engine import (Batch, DeviceMatrix, DeviceResult, Engine, HostMatrix, Server, banded,  # noqa: F401
+from .engine import (Batch, DeviceMatrix, DeviceResult, Engine, HostMatrix, Server, banded, bin_name,  # noqa: F401
                      csr1_read, csr1_write, erdos_renyi, laplace2d, pack, panels, rmat, run_ours, stencil27)
+from .engine import _num_bins
 
-__all__ = ["ApptError", "Batch", "DeviceMatrix", "DeviceResult", "Engine", "HostMatrix", "Server", "banded",
+NUM_BINS = _num_bins()
+
+__all__ = ["ApptError", "NUM_BINS", "Batch", "DeviceMatrix", "DeviceResult", "Engine", "HostMatrix", "Server", "banded", "bin_name",
            "csr1_read", "csr1_write", "erdos_renyi", "laplace2d", "lib", "pack", "panels", "rmat", "run_ours",
            "stencil27"]

@@ -25,6 +25,9 @@ appt_tuning tuning_of(const Config& c) {
   t.sort_span_factor_symbolic = c.sym.sort_ratio > 0 ? c.sym.sort_ratio : -1;
   t.transpose_radix_min_nnz = c.radix_min_nnz > INT32_MAX ? -1 : (int32_t)c.radix_min_nnz;
   t.mirror_radix_min_nnz = c.radix_mirror_min_nnz > INT32_MAX ? -1 : (int32_t)c.radix_mirror_min_nnz;
+  for (int k = 0; k < kNumCls; k++) t.class_lds_bytes[k] = c.sym.cls_bytes[k];
+  t.sort_max_products = c.sym.sort_max > 0 ? c.sym.sort_max : -1;
+  t.rowptr64_min_nnz = c.rowptr64_min_nnz > INT32_MAX ? 0 : (int32_t)c.rowptr64_min_nnz;
   return t;
 }
 

@@ -108,6 +108,7 @@ static BinParams default_params() {
   P.dense_ratio = 4;
   P.lds_budget = 64 * 1024;
   P.sort_ratio = kDefaultSortRatio;
+  P.sort_max = kSortMax;
   return P;
 }
 
@@ -139,6 +140,17 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
     if (t->sort_span_factor != 0) c.num.sort_ratio = t->sort_span_factor > 0 ? t->sort_span_factor : 0;
     if (t->sort_span_factor_symbolic != 0)
       c.sym.sort_ratio = t->sort_span_factor_symbolic > 0 ? t->sort_span_factor_symbolic : 0;
+    for (int k = 0; k < kNumCls; k++)
+      if (t->class_lds_bytes[k] > 0) c.sym.cls_bytes[k] = c.num.cls_bytes[k] = t->class_lds_bytes[k];
+    if (t->sort_max_products != 0) {
+      int32_t m = 0;  // a power of two in [16, kSortMax], or no sort bins
+      if (t->sort_max_products > 0) {
+        m = 16;
+        while (m < t->sort_max_products && m < kSortMax) m <<= 1;
+      }
+      c.sym.sort_max = c.num.sort_max = m;
+    }
+    if (t->rowptr64_min_nnz > 0) c.rowptr64_min_nnz = t->rowptr64_min_nnz;
   }
   if (group_budget > lds_max) group_budget = lds_max;
   c.sym.lds_budget = c.num.lds_budget = (int32_t)group_budget;
@@ -303,12 +315,10 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
   stage_bmeta(W, O, bptr, nullptr, nullptr, 0);
 }
 
-// ctl[0..1]: heavy-row claim counters; ctl[2]: B rows unsorted; ctl[3]: the
-// symmetric-pattern gather failed; ctl[4]: A rows not strictly increasing;
-// ctl[5..7]: structure check failures (ends, row pointer order, columns)
+// control words: see kCtlWords
 int32_t* Engine::reset_ctl(Workspace& W) {
-  int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * 8));
-  launch_fill_i32(ctl, 8, 0, stream_);
+  int32_t* ctl = as<int32_t>(ensure(W.ctl, sizeof(int32_t) * kCtlWords));
+  launch_fill_i32(ctl, kCtlWords, 0, stream_);
   return ctl;
 }
 
@@ -344,6 +354,8 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
 
 void Engine::read_sym(Workspace& W, Plan& P, bool want_ctl) {
   hipStream_t s = stream_;
+  // a launch that failed is reported here, before its statistics are used
+  APPT_HIP(hipGetLastError());
   const int32_t* dstats = as<int32_t>(W.stats.p);
   // symbolic stats and the products count in one copy (the numeric stats
   // and total between them are not read here)
@@ -366,6 +378,7 @@ HeavyArgs Engine::heavy_args(Workspace& W, const Operands& O, const Config& cfg)
   heavy_num_shape(dyn, cfg.heavy_cap, &h.wc_num, &h.vcap, &h.dcols);
   h.work = as<int32_t>(W.ctl.p);
   h.bflag = h.work + 2;
+  h.fault = h.work + kCtlHeavyFault;
   h.cur = as<int32_t>(ensure(W.cur, sizeof(int32_t) * (size_t)std::max<int64_t>(O.annz, 1)));
   return h;
 }
@@ -461,6 +474,7 @@ void Engine::stage_offsets(Workspace& W, const Operands& O, const Config& cfg, i
 
 void Engine::read_num(Workspace& W, const Operands& O, Plan& P) {
   hipStream_t s = stream_;
+  APPT_HIP(hipGetLastError());
   const int32_t* dstats = as<int32_t>(W.stats.p);
   // numeric stats, products and nnz(C) in one copy
   APPT_HIP(hipMemcpyAsync(hs_->num, dstats + stat_words(),
@@ -614,6 +628,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)n + 1)));
   int32_t* ccol = nullptr;
   double* cval = nullptr;
+  void* rowptr_out = rp32;
   try {
     stage_offsets(W, O, cfg_, rp32);
     read_num(W, O, P);
@@ -631,27 +646,33 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
     stage_numeric(W, O, cfg_, P, ccol, cval);
     record(5);
     stage_mirror(W, O, P, ccol, cval);
+    if (cfg_.rowptr64(P.nnz)) {
+      int64_t* rp64 = as<int64_t>(pool_.get(sizeof(int64_t) * ((size_t)n + 1)));
+      rowptr_out = rp64;
+      launch_copy_i64(as<int64_t>(W.rowoff.p), (int64_t)n + 1, rp64, s);
+    }
+    record(6);
+    APPT_HIP(hipGetLastError());
+    const bool heavy = P.num[kBinHeavy * kStatFields] > 0;
+    if (heavy) APPT_HIP(hipMemcpyAsync(hs_->ctl, W.ctl.p, sizeof(hs_->ctl), hipMemcpyDeviceToHost, s));
+    wait_stream();
+    if (heavy && hs_->ctl[kCtlHeavyFault] != 0)
+      throw Error(APPT_ERR_HIP, "heavy-row window could not advance (window shape below one word of outputs)");
   } catch (...) {
+    // nothing enqueued may still write these buffers once they are reused
+    (void)hipStreamSynchronize(s);
+    if (rowptr_out != rp32) pool_.put(rowptr_out);
     pool_.put(rp32);
     pool_.put(ccol);
     pool_.put(cval);
     throw;
   }
-  void* rowptr_out = rp32;
-  if (P.nnz > INT32_MAX) {
-    pool_.put(rp32);
-    int64_t* rp64 = as<int64_t>(pool_.get(sizeof(int64_t) * ((size_t)n + 1)));
-    launch_copy_i64(as<int64_t>(W.rowoff.p), (int64_t)n + 1, rp64, s);
-    rowptr_out = rp64;
-  }
-  record(6);
-  APPT_HIP(hipGetLastError());
-  wait_stream();
+  if (rowptr_out != rp32) pool_.put(rp32);
 
   Result* R = new Result{rowptr_out, ccol, cval};
   C->n = n;
   C->nnz = P.nnz;
-  C->rowptr_is_64 = P.nnz > INT32_MAX ? 1 : 0;
+  C->rowptr_is_64 = rowptr_out != rp32 ? 1 : 0;
   C->rowptr = rowptr_out;
   C->col = ccol;
   C->val = cval;
@@ -721,15 +742,26 @@ void Engine::eliminate_zeros(appt_csr_result* C) {
   APPT_HIP(hipStreamSynchronize(s));
   nnz = hs_->total;
   if (nnz == C->nnz) return;  // nothing to drop
-  const bool rp64 = nnz > INT32_MAX;
-  void* rp = pool_.get((rp64 ? sizeof(int64_t) : sizeof(int32_t)) * n1);
-  int32_t* col = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
-  double* val = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
-  launch_compact_nonzero(n, C->rowptr, C->rowptr_is_64 != 0, C->col, C->val, off, col, val, s);
-  if (rp64) launch_copy_i64(off, (int64_t)n + 1, as<int64_t>(rp), s);
-  else launch_i64_to_i32(off, (int64_t)n + 1, as<int32_t>(rp), s);
-  APPT_HIP(hipGetLastError());
-  APPT_HIP(hipStreamSynchronize(s));
+  const bool rp64 = cfg_.rowptr64(nnz);
+  void* rp = nullptr;
+  int32_t* col = nullptr;
+  double* val = nullptr;
+  try {
+    rp = pool_.get((rp64 ? sizeof(int64_t) : sizeof(int32_t)) * n1);
+    col = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1)));
+    val = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(nnz, 1)));
+    launch_compact_nonzero(n, C->rowptr, C->rowptr_is_64 != 0, C->col, C->val, off, col, val, s);
+    if (rp64) launch_copy_i64(off, (int64_t)n + 1, as<int64_t>(rp), s);
+    else launch_i64_to_i32(off, (int64_t)n + 1, as<int32_t>(rp), s);
+    APPT_HIP(hipGetLastError());
+    APPT_HIP(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    pool_.put(rp);
+    pool_.put(col);
+    pool_.put(val);
+    throw;
+  }
   release(C);
   Result* R = new Result{rp, col, val};
   C->n = n;

@@ -65,6 +65,8 @@ struct Config {
   int32_t heavy_cap = 0;  // column-window cap of the heavy-row bin (0: as wide as LDS allows)
   int64_t radix_min_nnz = 1 << 18;  // asymmetric transposes of at least this many entries use the radix sort
   int64_t radix_mirror_min_nnz = 1 << 20;  // AAtS mirrors of results this large use the radix sort
+  int64_t rowptr64_min_nnz = (int64_t)INT32_MAX + 1;  // results this large get int64 row pointers
+  bool rowptr64(int64_t nnz) const { return nnz > INT32_MAX || nnz >= rowptr64_min_nnz; }
 };
 
 // Device scratch of one pipeline.  Buffers only grow, so a HIP graph that
@@ -121,13 +123,21 @@ struct Result {  // what appt_csr_result::_priv points to
   void* val = nullptr;
 };
 
+// Control words of a pipeline run (zeroed by Engine::reset_ctl):
+// [0..1] heavy-row claim counters (symbolic, numeric); [2] B rows unsorted;
+// [3] the symmetric-pattern transpose gather failed; [4] A rows not strictly
+// increasing; [5..7] structure check failures (ends, row pointer order,
+// columns); [8] a heavy numeric window could not advance.
+constexpr int kCtlWords = 16;
+constexpr int kCtlHeavyFault = 8;
+
 struct HostStats {  // pinned staging for the small device->host reads; the
                     // fields up to ctl mirror the device stats layout
   int32_t sym[kNumBins * kStatFields];
   int32_t num[kNumBins * kStatFields];
   unsigned long long products;
   int64_t total;
-  int32_t ctl[8];
+  int32_t ctl[kCtlWords];
 };
 static_assert(offsetof(HostStats, products) == sizeof(int32_t) * 2 * kNumBins * kStatFields &&
                   offsetof(HostStats, total) == offsetof(HostStats, products) + 8,

@@ -666,10 +666,10 @@ __device__ __forceinline__ int group_index(const BinParams& P, int64_t f, int64_
 }
 
 // Register-sort bin of a row with f products over nA A entries: slots
-// N = np2(max(f, nA, 16)) <= kSortMax, or -1.
-__device__ __forceinline__ int sort_bin_of(int64_t f, int64_t nA) {
+// N = np2(max(f, nA, 16)) <= P.sort_max, or -1.
+__device__ __forceinline__ int sort_bin_of(const BinParams& P, int64_t f, int64_t nA) {
   const int64_t m = f > nA ? f : nA;
-  if (m > kSortMax) return -1;
+  if (m > P.sort_max) return -1;
   int b = kBinSort0;
   while ((int64_t)(16 << (b - kBinSort0)) < m) b++;
   return b;
@@ -685,7 +685,7 @@ __device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t
   const int gidx = group_index(P, f, nA);
   const int64_t keys = f < words ? f : words;
   const int64_t hash_bytes = np2_hd((keys * 4) / 3 + 1) * 8;
-  const int sb = sort_bin_of(f, nA);
+  const int sb = sort_bin_of(P, f, nA);
   // sort bins: the statistic is the A-row length (sizes the entry arrays)
   if (sb >= 0 && P.sort_ratio > 0 && words > (int64_t)P.sort_ratio * f) { *stat = nA; return sb; }
   for (int c = 0; c < P.ncls; c++)
@@ -725,7 +725,7 @@ __device__ __forceinline__ int num_bin_of(const BinParams& P, int64_t c, int64_t
   const int gidx = group_index(P, f, nA);
   const int64_t lim = (int64_t)P.dense_ratio * w;
   const bool dense_ok = words <= (lim > 64 ? lim : 64);
-  const int sb = sort_bin_of(f, nA);
+  const int sb = sort_bin_of(P, f, nA);
   // scattered: under 4 columns per 3 distinct words (clustered rows, such as
   // stencils, keep their windows however wide the span)
   if (sb >= 0 && P.sort_ratio > 0 && words > (int64_t)P.sort_ratio * f && 4 * w > 3 * c) { *stat = nA; return sb; }
@@ -2019,7 +2019,6 @@ __device__ __forceinline__ void heavy_stage_ranges(const RowArgs& A, HeavyStage&
 }
 
 enum HeavyMode { kHeavyOr = 0, kHeavySlot = 1, kHeavyDense = 2 };
-constexpr int64_t kHeavySparseSpan = 8;
 constexpr int kHeavyU = 4;  // products in flight per thread in heavy_products
 
 template <int MODE>
@@ -2158,7 +2157,8 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_sym(RowArgs A, int32_t wc, in
 __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols, int32_t vcap, int32_t dcols,
                                                        int32_t* __restrict__ work, const int32_t* __restrict__ bflag,
                                                        int32_t* __restrict__ cur, const int64_t* __restrict__ rowoff,
-                                                       int32_t* __restrict__ ccol, double* __restrict__ cval) {
+                                                       int32_t* __restrict__ ccol, double* __restrict__ cval,
+                                                       int32_t* __restrict__ fault) {
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   __shared__ HeavyStage hs;
   __shared__ int32_t cut_word;
@@ -2212,6 +2212,13 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
         }
         __syncthreads();
         cw = cut_word;
+        if (cw == 0) {
+          // the first word's outputs exceed vcap: no progress is possible.
+          // heavy_num_shape keeps vcap >= 32, so this is a guard only; the
+          // block abandons the row and the host reports the fault
+          if (tid == 0) *fault = 1;
+          break;
+        }
         if (cw < words) wcut = w0 + (int64_t)cw * 32;
         const int nvals = cw < words ? (int)pre[cw] : (int)(run < vcap ? run : vcap);
         for (int t = tid; t < nvals; t += kHeavyT) vals[t] = 0.0;
@@ -2920,44 +2927,6 @@ static void num_window_any(const RowArgs& a, int32_t W, int32_t Cm, const int64_
   }
 }
 
-// Symbolic heavy window for a dynamic LDS allowance of `bytes`: 1 bit per
-// column, a multiple of 32 (capped by `cap` when non-zero).
-int32_t heavy_window_cols(int64_t bytes, int32_t cap) {
-  int64_t wc = bytes * 8;
-  if (cap > 0 && wc > cap) wc = cap;
-  wc &= ~(int64_t)31;
-  if (wc < 32) wc = 32;
-  if (wc > ((int64_t)1 << 30)) wc = (int64_t)1 << 30;
-  return (int32_t)wc;
-}
-// Numeric heavy windows in `bytes` of LDS: sparse windows of `wcols` columns
-// (bit + 32-bit prefix per word) with `vcap` f64 slots, dense windows of
-// `dcols` columns (bit + f64 per column).  A cap (testing) sets wcols = cap,
-// vcap = max(32, cap / 3) and dcols = max(32, cap / 2).
-void heavy_num_shape(int64_t bytes, int32_t cap, int32_t* wcols, int32_t* vcap, int32_t* dcols) {
-  int64_t wc = cap > 0 ? cap : (bytes < 131072 ? bytes : 131072);
-  wc &= ~(int64_t)31;
-  if (wc < 64) wc = 64;
-  int64_t vc = (bytes - wc / 4) / 8;
-  int64_t dc = ((bytes - wc / 8) / 8) & ~(int64_t)31;
-  if (cap > 0) {
-    vc = (cap / 3) < vc ? (cap / 3) : vc;
-    dc = ((cap / 2) < dc ? (cap / 2) : dc) & ~(int64_t)31;
-  }
-  if (vc < 32) vc = 32;
-  if (dc < 32) dc = 32;
-  if (dc > wc) dc = wc;
-  *wcols = (int32_t)wc;
-  *vcap = (int32_t)vc;
-  *dcols = (int32_t)dc;
-}
-int64_t heavy_lds_bytes(bool vals, int32_t wc, int32_t vcap, int32_t dcols) {
-  if (!vals) return (int64_t)(wc / 32) * 4;
-  const int64_t sparse = (int64_t)(wc / 32) * 8 + (int64_t)vcap * 8;
-  const int64_t dense = (int64_t)(wc / 32) * 4 + (int64_t)dcols * 8;
-  return sparse > dense ? sparse : dense;
-}
-
 static void heavy_launch(bool vals, const RowArgs& a, const HeavyArgs& h, int32_t* work, int32_t* rowcnt,
                          const int64_t* rowoff, int32_t* ccol, double* cval, hipStream_t s) {
   const size_t lds =
@@ -2968,7 +2937,7 @@ static void heavy_launch(bool vals, const RowArgs& a, const HeavyArgs& h, int32_
   if (vals) {
     (void)hipFuncSetAttribute((const void*)k_heavy_num, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_heavy_num, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_num, h.vcap, h.dcols, work,
-                       h.bflag, h.cur, rowoff, ccol, cval);
+                       h.bflag, h.cur, rowoff, ccol, cval, h.fault);
   } else {
     (void)hipFuncSetAttribute((const void*)k_heavy_sym, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(k_heavy_sym, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_sym, work, h.bflag, h.cur, rowcnt);

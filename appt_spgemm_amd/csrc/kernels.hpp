@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "heavy_shape.hpp"
+
 namespace appt {
 
 constexpr int kNumCls = 3;
@@ -44,6 +46,7 @@ struct BinParams {
   int32_t lds_budget;             // LDS bytes a block of the wave-level bins may take
   int32_t sort_ratio;             // register sort first when span words > sort_ratio * products (0: never);
                                   // numeric pass: scattered rows only
+  int32_t sort_max;               // slots of the largest register-sort bin (16 .. kSortMax; 0: no sort bins)
 };
 
 inline bool bin_is_sort(int bin) { return bin >= kBinSort0; }
@@ -171,16 +174,15 @@ constexpr int kWordListCap = 64;  // word-list slots per output row (stride of R
 
 // Heavy-row bin: column windows of wc_sym / wc_num columns, two row-claim
 // counters (work[0] symbolic, work[1] numeric), the B-sortedness flag of
-// launch_rowstats and a cursor per A entry (nnz(A) int32).
+// launch_rowstats, a cursor per A entry (nnz(A) int32) and a fault flag the
+// numeric kernel sets (to 1) if a sparse window cannot advance.
 struct HeavyArgs {
   int32_t wc_sym, wc_num, vcap, dcols;  // window columns (symbolic / numeric sparse), slots, dense width
   int32_t* work;
   const int32_t* bflag;
   int32_t* cur;
+  int32_t* fault;
 };
-int32_t heavy_window_cols(int64_t lds_bytes, int32_t cap);
-void heavy_num_shape(int64_t lds_bytes, int32_t cap, int32_t* wcols, int32_t* vcap, int32_t* dcols);
-int64_t heavy_lds_bytes(bool vals, int32_t wc, int32_t vcap, int32_t dcols);
 
 // Symbolic: rowcnt[i] = distinct output columns of row i.
 // words_max: the bin's largest LDS statistic (span words or hash keys).

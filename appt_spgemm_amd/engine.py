@@ -105,6 +105,19 @@ def csr1_write(path: str, m: HostMatrix) -> None:
     check(lib().appt_csr1_write(str(path).encode(), C.byref(h)), "appt_csr1_write")
 
 
+def bin_name(b: int) -> str | None:
+    """Name of row bin ``b`` (``appt_bin_name``), None past the last bin."""
+    p = lib().appt_bin_name(b)
+    return p.decode() if p else None
+
+
+def _num_bins() -> int:
+    b = 0
+    while lib().appt_bin_name(b):
+        b += 1
+    return b
+
+
 # ---------------------------------------------------------------------------
 # device engine
 # ---------------------------------------------------------------------------
@@ -191,7 +204,7 @@ class Engine:
         if tuning:
             t = Tuning()
             for k, v in tuning.items():
-                if k in ("f_class_max", "group_for_len"):
+                if k in ("f_class_max", "group_for_len", "class_lds_bytes"):
                     arr = getattr(t, k)
                     for i, x in enumerate(v):
                         arr[i] = int(x)
@@ -312,7 +325,9 @@ def _tuning_dict(t: Tuning) -> dict:
             "sort_span_factor": int(t.sort_span_factor),
             "sort_span_factor_symbolic": int(t.sort_span_factor_symbolic),
             "transpose_radix_min_nnz": int(t.transpose_radix_min_nnz),
-            "mirror_radix_min_nnz": int(t.mirror_radix_min_nnz)}
+            "mirror_radix_min_nnz": int(t.mirror_radix_min_nnz),
+            "class_lds_bytes": [int(x) for x in t.class_lds_bytes if x > 0],
+            "sort_max_products": int(t.sort_max_products), "rowptr64_min_nnz": int(t.rowptr64_min_nnz)}
 
 
 def panels(rowptr: np.ndarray, max_bytes: int) -> list:
@@ -417,7 +432,7 @@ class Batch:
 def _tuning_struct(tuning: dict | None) -> Tuning:
     t = Tuning()
     for k, v in (tuning or {}).items():
-        if k in ("f_class_max", "group_for_len"):
+        if k in ("f_class_max", "group_for_len", "class_lds_bytes"):
             arr = getattr(t, k)
             for i, x in enumerate(v):
                 arr[i] = int(x)

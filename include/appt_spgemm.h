@@ -97,7 +97,13 @@ typedef struct appt_tuning {
                                       262,144; < 0 = never) */
   int32_t mirror_radix_min_nnz;    /* AAtS mirror by radix sort when nnz(C) reaches this (0 = default 1,048,576;
                                       < 0 = never) */
-  int32_t reserved[3];
+  int32_t class_lds_bytes[APPT_MAX_CLASSES]; /* LDS bytes one row of products class c may use in the
+                                      wave-level bins (0 = default 16 KB, 24 KB, 32 KB) */
+  int32_t sort_max_products;  /* largest register-sort bin (products and A entries of a row, rounded up to
+                                 a power of two in [16, 2048]; 0 = 2048; < 0 = no register-sort bins) */
+  int32_t rowptr64_min_nnz;   /* results with at least this many entries get int64 row pointers
+                                 (0 = only when nnz(C) >= 2^31, where int32 cannot hold them) */
+  int32_t reserved[2];
 } appt_tuning;
 
 typedef struct appt_options {

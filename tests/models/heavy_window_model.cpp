@@ -1,0 +1,104 @@
+// CPU model of the numeric heavy-row window sequence (test infrastructure).
+//
+// Restates the window loop of k_heavy_num (appt_spgemm_amd/csrc/kernels.hip)
+// over a row's output columns, using the product's own shape rules
+// (heavy_shape.hpp), and checks that every window advances, that a sparse
+// window never holds more than vcap outputs, and that the windows together
+// emit every output exactly once.  Random rows: span up to 2^22 columns,
+// densities from one column in 10^4 to fully dense, clustered or uniform.
+// Prints one line per configuration and "OK <rows>" at the end; exits 1 on
+// the first violated invariant.
+#include <cstdint>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "heavy_shape.hpp"
+
+using namespace appt;
+
+static int fail(const char* what, int64_t a, int64_t b) {
+  std::printf("FAIL %s (%lld, %lld)\n", what, (long long)a, (long long)b);
+  return 1;
+}
+
+struct Row {
+  int64_t lo, hi;
+  std::vector<int32_t> pre;  // pre[c]: outputs in [lo, lo + c)
+};
+
+int main() {
+  // dynamic LDS of a heavy block: the device's LDS per block (64 KB .. 160 KB)
+  // less the kernel's static staging (24 KB)
+  const int64_t lds_blocks[] = {65536, 163840};
+  const int32_t caps[] = {0, 1, 31, 32, 33, 63, 64, 65, 96, 100, 128, 1000, 4096, 65536, 131072, 1 << 20};
+  std::mt19937_64 rng(12345);
+  // rows: spans up to 5000 columns (dense windows) and up to 2^21 (sparse),
+  // densities from one column in 10^4 to fully dense, clustered or uniform
+  std::vector<Row> rows;
+  const double dens[] = {1e-4, 1e-2, 0.3, 1.0};
+  for (int t = 0; t < 32; t++) {
+    Row r;
+    const int64_t span = 1 + (int64_t)(rng() % (t < 20 ? 5000 : (1u << 21)));
+    r.lo = (int64_t)(rng() % 1000000) & ~(int64_t)31;  // the pre-pass aligns lo to 32 columns
+    r.hi = r.lo + span - 1;
+    const double d = dens[t % 4];
+    const bool clustered = (t / 4) % 2;
+    std::uniform_real_distribution<double> u(0, 1);
+    r.pre.assign((size_t)span + 1, 0);
+    for (int64_t c = 0; c < span; c++) {
+      const double p = clustered ? (((c / 4096) % 3 == 0) ? 1.0 : d * 0.01) : d;
+      const bool on = c == 0 || c == span - 1 || u(rng) < p;  // lo and hi are output columns
+      r.pre[(size_t)c + 1] = r.pre[(size_t)c] + (on ? 1 : 0);
+    }
+    rows.push_back(std::move(r));
+  }
+  int64_t checked = 0;
+  for (int64_t lds : lds_blocks) {
+    const int64_t dyn = lds - 24 * 1024;
+    for (int32_t cap : caps) {
+      int32_t wcols, vcap, dcols;
+      heavy_num_shape(dyn, cap, &wcols, &vcap, &dcols);
+      if (vcap < kHeavyMinSlots || dcols < kHeavyMinSlots || wcols < kHeavyMinCols || (wcols & 31) || (dcols & 31))
+        return fail("shape floor", vcap, dcols);
+      if (cap == 0 && heavy_lds_bytes(true, wcols, vcap, dcols) > dyn) return fail("shape exceeds LDS", wcols, dyn);
+      std::printf("lds %lld cap %d: wcols %d vcap %d dcols %d\n", (long long)lds, cap, wcols, vcap, dcols);
+      for (const Row& r : rows) {
+        const int64_t lo = r.lo, hi = r.hi, span = hi - lo + 1;
+        auto outputs = [&](int64_t a, int64_t b) { return (int64_t)r.pre[(size_t)(b - lo)] - r.pre[(size_t)(a - lo)]; };
+        const int64_t total = r.pre[(size_t)span];
+        const bool sparse = hi - lo + 1 > kHeavySparseSpan * dcols;
+        int64_t w0 = lo, emitted = 0, windows = 0;
+        while (w0 <= hi) {
+          const int64_t wspan = sparse ? wcols : dcols;
+          const int64_t w1 = (w0 + wspan) < (hi + 1) ? (w0 + wspan) : (hi + 1);
+          const int64_t words = (w1 - w0 + 31) >> 5;
+          int64_t wcut = w1;
+          if (sparse) {
+            // the cut: the first word whose outputs would not all fit in vcap
+            int64_t cut = words, run = 0;
+            for (int64_t w = 0; w < words; w++) {
+              const int64_t a = w0 + w * 32, b = (a + 32) < w1 ? (a + 32) : w1;
+              const int64_t pc = outputs(a, b);
+              if (run <= vcap && run + pc > vcap) { cut = w; break; }
+              run += pc;
+            }
+            if (cut == 0) return fail("sparse window cannot advance", w0, vcap);
+            if (cut < words) wcut = w0 + cut * 32;
+          }
+          if (wcut <= w0) return fail("window does not advance", w0, wcut);
+          const int64_t in = outputs(w0, wcut);
+          if (sparse && in > vcap) return fail("sparse window over capacity", in, vcap);
+          if (!sparse && wcut - w0 > dcols) return fail("dense window over width", wcut - w0, dcols);
+          emitted += in;
+          w0 = wcut;
+          if (++windows > span / 32 + 2) return fail("too many windows", windows, span);
+        }
+        if (emitted != total) return fail("outputs lost or repeated", emitted, total);
+        checked++;
+      }
+    }
+  }
+  std::printf("OK %lld\n", (long long)checked);
+  return 0;
+}
