@@ -130,6 +130,8 @@ _SIGNATURES = {
     "appt_engine_trim": (C.c_int, [C.c_void_p]),
     "appt_engine_device_bytes": (C.c_int64, [C.c_void_p]),
     "appt_spgemm_compute": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, P(CsrResult)]),
+    "appt_spgemm_compute_noalloc": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, C.c_void_p, C.c_void_p,
+                                              C.c_void_p, C.c_int64, P(C.c_int64)]),
     "appt_csr_release": (C.c_int, [C.c_void_p, P(CsrResult)]),
     "appt_csr_eliminate_zeros": (C.c_int, [C.c_void_p, P(CsrResult)]),
     "appt_spgemm_compute_rows": (C.c_int, [C.c_void_p, P(CsrView), C.c_int, C.c_int32, C.c_int32,

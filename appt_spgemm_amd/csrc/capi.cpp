@@ -129,6 +129,12 @@ appt_status appt_spgemm_compute(appt_engine* eng, const appt_csr_view* A, appt_o
   return guard([&] { eng->impl->compute(*A, op, C); });
 }
 
+appt_status appt_spgemm_compute_noalloc(appt_engine* eng, const appt_csr_view* A, appt_op op, int32_t* rowptr,
+                                        int32_t* col, double* val, int64_t capacity, int64_t* nnz_out) {
+  if (!eng || !A || !nnz_out) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->compute_noalloc(*A, op, rowptr, col, val, capacity, nnz_out); });
+}
+
 appt_status appt_spgemm_compute_rows(appt_engine* eng, const appt_csr_view* A, appt_op op, int32_t row_begin,
                                      int32_t row_end, appt_csr_result* C) {
   if (!eng || !A || !C) return APPT_ERR_INVALID;

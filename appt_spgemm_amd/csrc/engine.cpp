@@ -18,6 +18,7 @@
 // runs them with the two synchronisations between, and a batch replays them
 // inside a HIP graph with the launch shapes of its first run.
 #include "engine.hpp"
+#include "noalloc.hpp"
 
 #include <chrono>
 
@@ -167,6 +168,9 @@ Engine::Engine(const appt_options* opt) : pool_(0) {
   device_ = opt ? opt->device : 0;
   if (device_ < 0 || device_ >= ndev) throw Error(APPT_ERR_INVALID, "device ordinal out of range");
   APPT_HIP(hipSetDevice(device_));
+  // device address of the no-allocation entry point's static scratch, taken
+  // once per device while it is current
+  APPT_HIP(static_scratch_init(device_));
   if (opt && opt->stream) {
     stream_ = (hipStream_t)opt->stream;
   } else {
@@ -220,6 +224,16 @@ void Engine::wait_stream() {
 void* Engine::ensure(Scratch& s, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (s.cap >= bytes) return s.p;
+  if (arena_) {
+    const size_t want = (bytes + 255) & ~(size_t)255;
+    if (arena_->used + want > arena_->cap)
+      throw Error(APPT_ERR_UNSUPPORTED, "static scratch exhausted: " + std::to_string(arena_->used + want) +
+                                            " of " + std::to_string(arena_->cap) + " bytes");
+    s.p = arena_->base + arena_->used;
+    s.cap = want;
+    arena_->used += want;
+    return s.p;
+  }
   if (s.p) pool_.put(s.p);
   size_t want = bytes + bytes / 8;
   s.p = pool_.get(want);
@@ -419,7 +433,7 @@ void Engine::stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, 
   W.wg_stride = 0;
   {
     const int64_t cnt = P.sym[kBinWgBitmap * kStatFields], wmax = P.sym[kBinWgBitmap * kStatFields + 2];
-    if (cnt > 0 && wmax > 0 && cnt * wmax * 4 <= kWgBitsBudget) {
+    if (!arena_ && cnt > 0 && wmax > 0 && cnt * wmax * 4 <= kWgBitsBudget) {
       (void)ensure(W.wgbits, sizeof(uint32_t) * (size_t)(cnt * wmax));
       int32_t* slot = as<int32_t>(ensure(W.wgslot, sizeof(int32_t) * (size_t)std::max(O.n, 1)));
       launch_fill_i32(slot, O.n, -1, s);
@@ -513,7 +527,8 @@ void Engine::stage_mirror(Workspace& W, const Operands& O, const Plan& P, const 
   if (!O.upper || P.direct_sym) return;
   const int32_t n = O.n;
   const int64_t* off = as<int64_t>(W.rowoff.p);
-  if (P.nnz < cfg_.radix_mirror_min_nnz) {
+  // (the static scratch holds no result-sized mirror buffers)
+  if (arena_ || P.nnz < cfg_.radix_mirror_min_nnz) {
     launch_mirror_pull(n, P.nnz, off, ccol, cval, stream_);
     return;
   }
@@ -701,6 +716,88 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
     times_.mirror_ms = ms[5];
     APPT_HIP(hipEventElapsedTime(&times_.total_ms, ev_[0], ev_[6]));
   }
+}
+
+void Engine::compute_noalloc(const appt_csr_view& A, appt_op op, int32_t* rowptr, int32_t* col, double* val,
+                             int64_t capacity, int64_t* nnz_out) {
+  validate(A);
+  if (op != APPT_OP_AA && op != APPT_OP_AAT && op != APPT_OP_AATS) throw Error(APPT_ERR_INVALID, "bad op");
+  // sizes first: an input too large for the static scratch is refused before any work
+  const int64_t need = static_scratch_need(A.n, A.nnz);
+  if (need > (int64_t)kStaticScratchBytes)
+    throw Error(APPT_ERR_UNSUPPORTED, "n = " + std::to_string(A.n) + ", nnz = " + std::to_string(A.nnz) +
+                                          " needs up to " + std::to_string(need) + " bytes of scratch; the "
+                                          "no-allocation entry point has " +
+                                          std::to_string(kStaticScratchBytes) + " (use appt_spgemm_compute)");
+  if (!nnz_out || (A.n > 0 && !rowptr) || capacity < 0 || (capacity > 0 && (!col || !val)))
+    throw Error(APPT_ERR_INVALID, "output buffers missing");
+  APPT_HIP(hipSetDevice(device_));
+  void* base = static_scratch(device_);
+  if (!base) throw Error(APPT_ERR_HIP, "static scratch of this device has no address");
+  *nnz_out = 0;
+  times_ = appt_phase_times{};
+  // every buffer is carved afresh from the start of the array: the previous
+  // call's contents are stale and each stage clears what it needs
+  Arena arena{static_cast<unsigned char*>(base), kStaticScratchBytes, 0};
+  ws_static_.each([](Scratch& x) { x.p = nullptr; x.cap = 0; });
+  ws_static_.wg_stride = 0;
+  struct Leave {
+    Engine* e;
+    ~Leave() { e->arena_ = nullptr; }
+  } leave{this};
+  arena_ = &arena;
+  const int32_t n = A.n;
+  hipStream_t s = stream_;
+  Workspace& W = ws_static_;
+  Operands O;
+  Plan P;
+  try {
+    record(0);
+    stage_operands(W, A, op, O);
+    record(1);
+    stage_prepass(W, O, cfg_);
+    read_sym(W, P, O.upper != 0);
+    record(2);
+    stage_symbolic(W, O, cfg_, P);
+    record(3);
+    stage_offsets(W, O, cfg_, rowptr);
+    read_num(W, O, P);
+    *nnz_out = P.nnz;
+    if (P.nnz > INT32_MAX) throw Error(APPT_ERR_OVERFLOW, "nnz(C) needs 64-bit row pointers");
+    if (P.nnz > capacity)
+      throw Error(APPT_ERR_OVERFLOW, "nnz(C) = " + std::to_string(P.nnz) + " exceeds the capacity " +
+                                         std::to_string(capacity));
+    P.direct_sym = O.upper && P.a_strict && P.num[kBinWgBitmap * kStatFields] == 0 &&
+                   P.num[kBinWgHash * kStatFields] == 0 && P.num[kBinHeavy * kStatFields] == 0;
+    record(4);
+    stage_numeric(W, O, cfg_, P, col, val);
+    record(5);
+    stage_mirror(W, O, P, col, val);
+    record(6);
+    APPT_HIP(hipGetLastError());
+    const bool heavy = P.num[kBinHeavy * kStatFields] > 0;
+    if (heavy) APPT_HIP(hipMemcpyAsync(hs_->ctl, W.ctl.p, sizeof(hs_->ctl), hipMemcpyDeviceToHost, s));
+    wait_stream();
+    if (heavy && hs_->ctl[kCtlHeavyFault] != 0)
+      throw Error(APPT_ERR_HIP, "heavy-row window could not advance (window shape below one word of outputs)");
+  } catch (...) {
+    // the next call reuses the array: nothing of this one may still run
+    (void)hipStreamSynchronize(s);
+    throw;
+  }
+  bins_ = appt_bin_report{};
+  bins_.n_bins = kNumBins;
+  for (int b = 0; b < kNumBins; b++) {
+    bins_.sym_rows[b] = P.sym[b * kStatFields];
+    bins_.num_rows[b] = P.num[b * kStatFields];
+    bins_.sym_metric_max[b] = P.sym[b * kStatFields + 1];
+    bins_.num_metric_max[b] = P.num[b * kStatFields + 1];
+  }
+  times_.products = P.products;
+  times_.nnz_c = P.nnz;
+  times_.n_bins_used = P.bins_used;
+  times_.sync_points = 3;
+  (void)n;
 }
 
 void Engine::release(appt_csr_result* C) {

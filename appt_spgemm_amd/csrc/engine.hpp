@@ -156,6 +156,13 @@ class Engine {
                     bool use_plan = true);
   // Exact row pointers of op(A) (symbolic phase only), copied to the host.
   void symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr_host);
+  // C = op(A) into caller-owned device buffers (int32 row pointers, n + 1;
+  // col / val of `capacity` entries), with every scratch buffer taken from
+  // the device's static scratch array: no device allocation.  *nnz_out
+  // receives nnz(C); APPT_ERR_OVERFLOW (nothing written to col / val) when it
+  // exceeds capacity.
+  void compute_noalloc(const appt_csr_view& A, appt_op op, int32_t* rowptr, int32_t* col, double* val,
+                       int64_t capacity, int64_t* nnz_out);
   void release(appt_csr_result* C);
   // Drop entries whose value is exactly zero (numeric semantics); C must be
   // an engine-owned result.
@@ -176,6 +183,8 @@ class Engine {
   int64_t lds_max() const { return lds_max_; }
   static void validate(const appt_csr_view& A);
 
+  // Scratch buffer of at least `bytes`: from the pool, or, inside
+  // compute_noalloc, carved from the static scratch array.
   void* ensure(Scratch& s, size_t bytes);
   void free_workspace(Workspace& W);
   // Called with the exact bytes of C once the symbolic phase has sized it,
@@ -209,6 +218,14 @@ class Engine {
  private:
   void record(int idx);
   HeavyArgs heavy_args(Workspace& W, const Operands& O, const Config& cfg);
+
+  // Bump allocator over the static scratch array (compute_noalloc only).
+  struct Arena {
+    unsigned char* base = nullptr;
+    size_t cap = 0, used = 0;
+  };
+  Arena* arena_ = nullptr;
+  Workspace ws_static_;  // scratch views into the static array, reset every call
 
   int device_ = 0;
   hipStream_t stream_ = nullptr;

@@ -180,6 +180,24 @@ appt_status appt_spgemm_compute(appt_engine* eng, const appt_csr_view* A, appt_o
                                 appt_csr_result* C);
 appt_status appt_csr_release(appt_engine* eng, appt_csr_result* C);
 
+/* C = op(A) with no device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  The caller passes the output buffers: rowptr (n+1
+ * int32), col and val of `capacity` entries, all device memory on the
+ * engine's device.  Every scratch buffer comes from one static device array
+ * of 64 MiB per device (taken when an engine is created there), so n and nnz
+ * are limited to what fits in it: about 606 bytes per row plus 44 per entry
+ * plus 3 MiB; larger inputs are refused with APPT_ERR_UNSUPPORTED before any
+ * work.  *nnz_out receives nnz(C); when it exceeds capacity (or needs 64-bit
+ * row pointers) the call returns APPT_ERR_OVERFLOW with col/val untouched, so
+ * the caller can size the buffers and call again.  Same kernels and results
+ * as appt_spgemm_compute.  The array is shared by every call on one device
+ * and keeps the last call's contents: calls on one device that overlap in
+ * time (from several host threads) race; calls from one thread are
+ * synchronous and never overlap. */
+appt_status appt_spgemm_compute_noalloc(appt_engine* eng, const appt_csr_view* A, appt_op op, int32_t* rowptr,
+                                        int32_t* col, double* val, int64_t capacity, int64_t* nnz_out);
+
 /* Numeric semantics on request: drop the entries of an engine-owned result
  * whose value is exactly zero (scipy's eliminate_zeros; the default is the
  * reference's structural semantics, report/CORRECTNESS_NOTES.md:3-16).  C is
