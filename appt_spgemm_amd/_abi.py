@@ -173,6 +173,7 @@ _SIGNATURES = {
     "appt_device_free_view": (C.c_int, [C.c_void_p, P(CsrView)]),
     "appt_csr_download_rows": (C.c_int, [C.c_void_p, P(CsrResult), C.c_int32, C.c_int32, P(C.c_int64),
                                           P(C.c_int32), P(C.c_double)]),
+    "appt_csr_digest": (C.c_int, [C.c_void_p, P(CsrResult), P(C.c_uint64)]),
     "appt_csr_download": (C.c_int, [C.c_void_p, P(CsrResult), P(C.c_int64), P(C.c_int32),
                                      P(C.c_double)]),
 }

@@ -310,6 +310,11 @@ appt_status appt_csr_download_rows(appt_engine* eng, const appt_csr_result* C, i
   });
 }
 
+appt_status appt_csr_digest(appt_engine* eng, const appt_csr_result* C, uint64_t* out) {
+  if (!eng || !C || !out) return APPT_ERR_INVALID;
+  return guard([&] { eng->impl->digest(*C, out); });
+}
+
 appt_status appt_csr_download(appt_engine* eng, const appt_csr_result* C, int64_t* rowptr, int32_t* col,
                               double* val) {
   if (!C) return APPT_ERR_INVALID;

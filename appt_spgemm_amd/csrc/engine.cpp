@@ -810,6 +810,20 @@ void Engine::release(appt_csr_result* C) {
   std::memset(C, 0, sizeof(*C));
 }
 
+void Engine::digest(const appt_csr_result& C, uint64_t out[2]) {
+  if (C.n < 0 || C.nnz < 0 || (C.nnz > 0 && (!C.col || !C.val)) || !C.rowptr)
+    throw Error(APPT_ERR_INVALID, "not a result");
+  APPT_HIP(hipSetDevice(device_));
+  auto* d = as<unsigned long long>(ensure(ws_.digest, 2 * sizeof(unsigned long long)));
+  launch_digest(C.n, C.nnz, C.rowptr, C.rowptr_is_64 != 0, C.col, C.val, d, stream_);
+  APPT_HIP(hipGetLastError());
+  APPT_HIP(hipMemcpyAsync(&hs_->total, d, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+  APPT_HIP(hipMemcpyAsync(&hs_->products, d + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, stream_));
+  APPT_HIP(hipStreamSynchronize(stream_));
+  out[0] = (uint64_t)hs_->total;
+  out[1] = (uint64_t)hs_->products;
+}
+
 void Engine::check(const appt_csr_view& A) {
   validate(A);
   APPT_HIP(hipSetDevice(device_));

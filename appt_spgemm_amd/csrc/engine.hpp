@@ -74,7 +74,7 @@ struct Config {
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
       numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
-      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot, plancnt, planwords;
+      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot, plancnt, planwords, digest;
   // words per row of the stored workgroup-bin bitmaps of the current call
   // (0: none stored)
   int32_t wg_stride = 0;
@@ -83,7 +83,8 @@ struct Workspace {
     Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
                       &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp,
-                      &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot, &plancnt, &planwords};
+                      &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot, &plancnt, &planwords,
+                      &digest};
     for (Scratch* s : all) f(*s);
   }
 };
@@ -167,6 +168,8 @@ class Engine {
   // Drop entries whose value is exactly zero (numeric semantics); C must be
   // an engine-owned result.
   void eliminate_zeros(appt_csr_result* C);
+  // Structure and value digests of a device result (see launch_digest).
+  void digest(const appt_csr_result& C, uint64_t out[2]);
   // Device-side structure check of A (throws APPT_ERR_INVALID).
   void check(const appt_csr_view& A);
   void run_timed(const char* task, const appt_csr_view& A, int reps, appt_eng_result* out);

@@ -38,6 +38,10 @@ void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s, const int3
   hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v, gate);
 }
 
+__global__ void k_zero_u64x2(unsigned long long* p) {
+  if (threadIdx.x < 2) p[threadIdx.x] = 0ull;
+}
+
 __global__ void k_fill3_i32(int32_t* a, int64_t na, int32_t* b, int64_t nb, int32_t* c, int64_t nc) {
   const int64_t tot = na + nb + nc;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
@@ -3046,6 +3050,55 @@ void launch_numeric_bin(int bin, const RowArgs& a, int32_t words_max, int32_t c_
     case 128: num_lds<64, false>(a, words_max, c_max, rowoff, ccol, cval, s); break;
     default: break;
   }
+}
+
+// ---------------------------------------------------------------------------
+// Digest of a CSR result: out[0] hashes the structure (each row pointer with
+// its index, each column with its position), out[1] the value bits with their
+// positions.  Each term is a splitmix64 mix of (position, content); terms are
+// summed modulo 2^64, so the digest does not depend on the reduction order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+template <typename RP>
+__global__ void __launch_bounds__(256) k_digest(int32_t n, int64_t nnz, const RP* __restrict__ rp,
+                                                const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                unsigned long long* __restrict__ out) {
+  unsigned long long hs = 0, hv = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t <= n; t += stride)
+    hs += mix64(mix64((unsigned long long)t) ^ (unsigned long long)(int64_t)rp[t]);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nnz; t += stride) {
+    const unsigned long long pos = mix64((unsigned long long)t + 0x5bd1e995ull);
+    hs += mix64(pos ^ (unsigned long long)(uint32_t)col[t]);
+    hv += mix64(pos ^ (unsigned long long)__double_as_longlong(val[t]));
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    hs += __shfl_xor(hs, m, 64);
+    hv += __shfl_xor(hv, m, 64);
+  }
+  if (threadIdx.x % kWave == 0) {
+    atomicAdd(&out[0], hs);
+    atomicAdd(&out[1], hv);
+  }
+}
+void launch_digest(int32_t n, int64_t nnz, const void* rp, bool rp64, const int32_t* col, const double* val,
+                   unsigned long long* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_zero_u64x2, dim3(1), dim3(64), 0, s, out);
+  const int64_t work = std::max<int64_t>((int64_t)n + 1, nnz);
+  int64_t blocks = (work + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  if (rp64)
+    hipLaunchKernelGGL(k_digest<int64_t>, dim3((unsigned)blocks), dim3(256), 0, s, n, nnz,
+                       static_cast<const int64_t*>(rp), col, val, out);
+  else
+    hipLaunchKernelGGL(k_digest<int32_t>, dim3((unsigned)blocks), dim3(256), 0, s, n, nnz,
+                       static_cast<const int32_t*>(rp), col, val, out);
 }
 
 // ---------------------------------------------------------------------------

@@ -241,6 +241,11 @@ void launch_count_nonzero(int32_t n, const void* rp, bool rp64, const double* va
 void launch_compact_nonzero(int32_t n, const void* rp, bool rp64, const int32_t* col, const double* val,
                             const int64_t* off, int32_t* ocol, double* oval, hipStream_t s);
 
+// Order-independent digest of a CSR result into out[0] (structure) and
+// out[1] (values): sums of per-entry 64-bit mixes of (position, content).
+void launch_digest(int32_t n, int64_t nnz, const void* rp, bool rp64, const int32_t* col, const double* val,
+                   unsigned long long* out, hipStream_t s);
+
 void set_num_cus(int n);
 void set_lds_per_cu(int64_t b);
 void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s, const int32_t* gate = nullptr);

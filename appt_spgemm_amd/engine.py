@@ -182,6 +182,13 @@ class DeviceResult:
                                        val.ctypes.data_as(C.POINTER(C.c_double))), "csr_download_rows")
         return HostMatrix(r1 - r0, rp, col[:cnt], val[:cnt])
 
+    def digest(self, engine: "Engine | None" = None) -> tuple:
+        """(structure, values) 64-bit digests computed on the device."""
+        eng = engine or self._engine
+        out = (C.c_uint64 * 2)()
+        check(lib().appt_csr_digest(eng.handle, C.byref(self.res), out), "csr_digest")
+        return int(out[0]), int(out[1])
+
     def eliminate_zeros(self) -> "DeviceResult":
         """Numeric semantics: drop entries whose value is exactly zero (in place)."""
         check(lib().appt_csr_eliminate_zeros(self._engine.handle, C.byref(self.res)), "csr_eliminate_zeros")
@@ -483,6 +490,13 @@ class Server:
         i = JobInfo()
         check(lib().appt_server_wait(self.handle, job, C.byref(i)), "appt_server_wait")
         return self._info(i)
+
+    def result(self, job: int) -> DeviceResult:
+        """Device view of a finished job's result (valid until ``release``)."""
+        i = JobInfo()
+        check(lib().appt_server_wait(self.handle, job, C.byref(i)), "appt_server_wait")
+        check(i.status, f"job {job}")
+        return DeviceResult(None, i.C)
 
     def download(self, job: int) -> HostMatrix:
         i = JobInfo()

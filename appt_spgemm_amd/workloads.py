@@ -10,23 +10,39 @@ import numpy as np
 from .engine import HostMatrix, banded, erdos_renyi, laplace2d, rmat, stencil27
 
 
-def batch_mix(count: int = 4096, seed: int = 1, nmin: int = 64, nmax: int = 16384):
-    """Uneven batch: n log-uniform in [nmin, nmax]; even index random-banded
-    (half-bandwidth U{1..32}, 70 % band fill), odd index Erdos-Renyi (mean row
-    nnz U(2, 20)); each matrix A*A or A*A^T with probability 1/2.  FP64
-    values U(-1,1).  Returns (matrices, ops)."""
+def batch_specs(count: int = 4096, seed: int = 1, nmin: int = 64, nmax: int = 16384):
+    """The uneven batch as generator calls: n log-uniform in [nmin, nmax];
+    even index random-banded (half-bandwidth U{1..32}, 70 % band fill), odd
+    index Erdos-Renyi (mean row nnz U(2, 20)); each matrix A*A or A*A^T with
+    probability 1/2.  FP64 values U(-1,1).  Returns (callables that build the
+    matrices, ops), so callers can generate them in parallel."""
     rng = np.random.default_rng(seed)
-    mats, ops = [], []
+    specs = []
     for k in range(count):
         n = int(np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
         s = seed * 1_000_003 + k
         if k % 2 == 0:
-            m = banded(n, int(rng.integers(1, 33)), 0.7, s)
+            bw = int(rng.integers(1, 33))
+            specs.append(lambda n=n, bw=bw, s=s: banded(n, bw, 0.7, s))
         else:
-            m = erdos_renyi(n, float(rng.uniform(2.0, 20.0)), s)
-        mats.append(m)
-        ops.append("AA" if rng.random() < 0.5 else "AAt")
-    return mats, ops
+            mu = float(rng.uniform(2.0, 20.0))
+            specs.append(lambda n=n, mu=mu, s=s: erdos_renyi(n, mu, s))
+    ops = ["AA" if rng.random() < 0.5 else "AAt" for _ in range(count)]
+    return specs, ops
+
+
+def generate(specs, workers: int = 16) -> list:
+    """Run generator callables on a thread pool (the C++ generators release
+    the interpreter lock), keeping their order."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max(1, workers)) as ex:
+        return list(ex.map(lambda f: f(), specs))
+
+
+def batch_mix(count: int = 4096, seed: int = 1, nmin: int = 64, nmax: int = 16384):
+    """The uneven batch (batch_specs) generated: returns (matrices, ops)."""
+    specs, ops = batch_specs(count, seed, nmin, nmax)
+    return generate(specs), ops
 
 
 def serve_job(rng: np.random.Generator, nmin: int = 1000, nmax: int = 2_000_000, seed: int = 0):

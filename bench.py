@@ -172,14 +172,6 @@ def copy_bandwidth(mib: int = 1024, reps: int = 5) -> float:
     return 2.0 * nbytes / (ms.value / reps * 1e-3) / 1e9 if ms.value > 0 else float("nan")
 
 
-def gen_parallel(fns, workers: int = 16):
-    """Run host generator calls on a thread pool (the C++ generators release
-    the GIL inside ctypes calls)."""
-    from concurrent.futures import ThreadPoolExecutor
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        return list(ex.map(lambda f: f(), fns))
-
-
 def products_of(m, op: str) -> int:
     deg = np.diff(np.asarray(m.rowptr, np.int64))
     if op == "AA":
@@ -193,21 +185,11 @@ def bench_batch(args) -> dict:
     random-banded, half Erdos-Renyi, each A*A or A*A^T.  Plain per-matrix calls
     against one planned, graph-replayed batch (autotuned bins)."""
     import appt_spgemm_amd as ap
-    rng = np.random.default_rng(args.seed)
+    from appt_spgemm_amd.workloads import batch_specs, generate
     count = args.batch_count
-    specs = []
-    for k in range(count):
-        n = int(np.floor(np.exp(rng.uniform(np.log(64), np.log(16384)))))
-        s = args.seed * 1_000_003 + k
-        if k % 2 == 0:
-            bw = int(rng.integers(1, 33))
-            specs.append(lambda n=n, bw=bw, s=s: ap.banded(n, bw, 0.7, s))
-        else:
-            mu = float(rng.uniform(2.0, 20.0))
-            specs.append(lambda n=n, mu=mu, s=s: ap.erdos_renyi(n, mu, s))
-    ops = ["AA" if rng.random() < 0.5 else "AAt" for _ in range(count)]
+    specs, ops = batch_specs(count, args.seed)
     t0 = time.perf_counter()
-    mats = gen_parallel(specs)
+    mats = generate(specs)
     print(f"# generated {count} matrices in {time.perf_counter() - t0:.1f} s, "
           f"sum n {sum(m.n for m in mats)}, sum nnz {sum(m.nnz for m in mats)}", file=sys.stderr, flush=True)
     products = sum(products_of(m, op) for m, op in zip(mats, ops))
@@ -351,10 +333,10 @@ def bench_serve(args) -> dict:
     log-uniform in [1e3, 2e6], A*A or A*A^T, resident on the device; a closed
     burst measures saturation, then Poisson arrivals at 50 % and 80 % of it."""
     import appt_spgemm_amd as ap
-    from appt_spgemm_amd.workloads import serve_job
+    from appt_spgemm_amd.workloads import generate, serve_job
     rng = np.random.default_rng(args.seed)
     pool_n = args.serve_pool
-    jobs = gen_parallel([lambda r=np.random.default_rng(args.seed * 7919 + k), k=k:
+    jobs = generate([lambda r=np.random.default_rng(args.seed * 7919 + k), k=k:
                          serve_job(r, 1000, args.serve_nmax, seed=k) for k in range(pool_n)], workers=8)
     eng = ap.Engine(0)
     dev = [eng.upload(m) for m, _, _ in jobs]

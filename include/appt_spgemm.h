@@ -404,6 +404,14 @@ appt_status appt_device_free_view(appt_engine* eng, appt_csr_view* v);
  * that range's entries (sized by the caller from the row pointers). */
 appt_status appt_csr_download_rows(appt_engine* eng, const appt_csr_result* C, int32_t row_begin, int32_t row_end,
                                    int64_t* rowptr, int32_t* col, double* val);
+/* Digests of a device result computed on the GPU (one pass, one
+ * synchronisation): out[0] hashes the structure (row pointers as int64 with
+ * their row index, columns with their position), out[1] the value bits with
+ * their position; each term is a 64-bit mix, summed modulo 2^64.  Equal
+ * results give equal digests; compares results too large to download.  No
+ * reference counterpart (the reference compares host checksums,
+ * src/spgemm_engine.cuh:116-118). */
+appt_status appt_csr_digest(appt_engine* eng, const appt_csr_result* C, uint64_t* out);
 /* Copy a device result to caller-provided host arrays (rowptr as int64);
  * eng may be NULL (the calling thread's current device is used). */
 appt_status appt_csr_download(appt_engine* eng, const appt_csr_result* C, int64_t* rowptr,

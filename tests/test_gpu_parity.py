@@ -436,14 +436,15 @@ def test_budget_rejects_cleanly():
 
 
 def test_full_size_stencil_c2(eng):
-    """27-point stencil 96^3 (n = 884,736): closed-form counts, sorted rows,
-    row sums, and a sample of rows bit-equal to the oracle."""
+    """27-point stencil 96^3 (n = 884,736), A*A, A*A^T and AAtS: closed-form
+    counts, sorted rows, row sums, and a sample of rows bit-equal to the
+    oracle."""
     for drop, seed in ((0.0, 1), (0.1, 1)):
         m = ap.stencil27(96, drop, seed)
         say("case stencil27 g=96 drop", drop)
         A = eng.upload(m)
         try:
-            for op in ("AA", "AAt"):
+            for op in ("AA", "AAt", "AAtS"):
                 R = eng.compute(A, op)
                 got = R.download()
                 R.release()
@@ -455,8 +456,12 @@ def test_full_size_stencil_c2(eng):
                 C = got.to_scipy() if False else ap.HostMatrix(m.n, got.rowptr, got.col, got.val).to_scipy()
                 ones = np.ones(m.n)
                 np.testing.assert_allclose(C @ ones, S @ (B @ ones), rtol=1e-9, atol=1e-9)
+                # AAtS: the mirrored upper triangle; with strictly sorted rows
+                # c_ij and c_ji sum the same products in the same order, so the
+                # A*A^T rows of the oracle are the reference bit for bit
+                oop = "AAt" if op == "AAtS" else op
                 for r0 in (0, 400000, m.n - 3000):
-                    off, c2, v2 = oracle.rows_range(m.n, m.rowptr, m.col, m.val, r0, r0 + 3000, op)
+                    off, c2, v2 = oracle.rows_range(m.n, m.rowptr, m.col, m.val, r0, r0 + 3000, oop)
                     np.testing.assert_array_equal(got.rowptr[r0:r0 + 3001] - got.rowptr[r0], off)
                     lo, hi = got.rowptr[r0], got.rowptr[r0 + 3000]
                     np.testing.assert_array_equal(got.col[lo:hi], c2)
