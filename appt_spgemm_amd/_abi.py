@@ -47,7 +47,7 @@ class Tuning(C.Structure):
                 ("sort_span_factor", C.c_int32), ("sort_span_factor_symbolic", C.c_int32),
                 ("transpose_radix_min_nnz", C.c_int32), ("mirror_radix_min_nnz", C.c_int32),
                 ("class_lds_bytes", C.c_int32 * MAX_CLASSES), ("sort_max_products", C.c_int32),
-                ("rowptr64_min_nnz", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("rowptr64_min_nnz", C.c_int32), ("row_templates", C.c_int32), ("reserved", C.c_int32 * 1)]
 
 
 class Options(C.Structure):

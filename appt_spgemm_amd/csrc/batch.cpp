@@ -28,6 +28,7 @@ appt_tuning tuning_of(const Config& c) {
   for (int k = 0; k < kNumCls; k++) t.class_lds_bytes[k] = c.sym.cls_bytes[k];
   t.sort_max_products = c.sym.sort_max > 0 ? c.sym.sort_max : -1;
   t.rowptr64_min_nnz = c.rowptr64_min_nnz > INT32_MAX ? 0 : (int32_t)c.rowptr64_min_nnz;
+  t.row_templates = c.templates ? 0 : -1;
   return t;
 }
 
@@ -42,6 +43,8 @@ void* Batch::alloc(size_t bytes) {
 
 Batch::Batch(Engine& e, int32_t count, const int32_t* n_each, const int64_t* nnz_each, const appt_op* ops)
     : E_(e), cfg_(e.config()) {
+  // block-local columns: relative row patterns are not comparable across blocks
+  cfg_.templates = false;
   if (count < 0 || (count > 0 && (!n_each || !nnz_each || !ops))) throw Error(APPT_ERR_INVALID, "bad batch arguments");
   K_ = count;
   blk_row_h_.assign((size_t)K_ + 1, 0);

@@ -105,8 +105,9 @@ const char* appt_bin_name(int32_t bin) {
       "list/c0/G8", "list/c0/G16", "list/c0/G32", "list/c0/G64",
       "list/c1/G8", "list/c1/G16", "list/c1/G32", "list/c1/G64",
       "list/c2/G8", "list/c2/G16", "list/c2/G32", "list/c2/G64",
-      "sort/N16", "sort/N32", "sort/N64", "sort/N128", "sort/N256", "sort/N512", "sort/N1024", "sort/N2048"};
-  static_assert(appt::kNumBins == 48, "bin names cover every bin id");
+      "sort/N16", "sort/N32", "sort/N64", "sort/N128", "sort/N256", "sort/N512", "sort/N1024", "sort/N2048",
+      "template"};
+  static_assert(appt::kNumBins == 49, "bin names cover every bin id");
   if (bin < 0 || bin >= appt::kNumBins) return nullptr;
   return names[bin];
 }

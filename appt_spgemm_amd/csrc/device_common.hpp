@@ -114,6 +114,37 @@ __device__ __forceinline__ uint32_t hash_slot(int32_t key, int logT) {
   return ((uint32_t)key * 2654435769u) >> (32 - logT);
 }
 
+// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8).  Renumber them so that each XCD walks one contiguous eighth of the
+// row list: neighbouring rows share B rows, and those then hit the same L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  constexpr int kXcd = 8;
+  const int x = b % kXcd, k = b / kXcd;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
+// Rows of a persistent window block.  The bin's row list is split into eight
+// contiguous ranges, one per XCD (block b runs on XCD b % 8), sized by the
+// XCD's block count; inside its range the blocks of one XCD take tiles of R
+// rows in turn (group gi owns row gi of a tile).  At any moment the blocks
+// of an XCD work on neighbouring rows, so the B rows they gather are shared
+// in that XCD's L2.
+struct RowWalk {
+  int first, step, end;
+};
+__device__ __forceinline__ RowWalk row_walk(int nrows, int R, int gi) {
+  constexpr int kXcd = 8;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b % kXcd, k = b / kXcd;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  const int nbx = per + (x < rem ? 1 : 0);
+  const int before = x * per + (x < rem ? x : rem);
+  const int x0 = (int)((int64_t)nrows * before / nb);
+  const int x1 = (int)((int64_t)nrows * (before + nbx) / nb);
+  return RowWalk{x0 + k * R + gi, nbx * R, x1};
+}
+
 __device__ __forceinline__ int64_t np2_i64(int64_t x) {
   if (x < 2) return 2;
   int64_t p = 2;

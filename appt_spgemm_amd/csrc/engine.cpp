@@ -152,6 +152,7 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
       c.sym.sort_max = c.num.sort_max = m;
     }
     if (t->rowptr64_min_nnz > 0) c.rowptr64_min_nnz = t->rowptr64_min_nnz;
+    if (t->row_templates < 0) c.templates = false;
   }
   if (group_budget > lds_max) group_budget = lds_max;
   c.sym.lds_budget = c.num.lds_budget = (int32_t)group_budget;
@@ -291,6 +292,7 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
   O.bnnz = nnz;
   O.upper = op == APPT_OP_AATS ? 1 : 0;
   O.ncols = n;
+  O.a_is_b = op == APPT_OP_AA;
   const int32_t* bptr = A.rowptr;
   int32_t* ctl = reset_ctl(W);
   if (op != APPT_OP_AA) {
@@ -361,7 +363,34 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
   int32_t* part = as<int32_t>(ensure(W.statpart, sizeof(int32_t) * (size_t)stats_part_elems() +
                                                       sizeof(unsigned long long) * kStatsMaxBlocks));
   launch_fill3_i32(dstats, (int64_t)stats_ints(), rowcnt, n, rowwords, n, s);
-  launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part, s);
+  // row templates: not in the static scratch (no-allocation calls) nor where
+  // a batch's columns are local to its blocks (cfg.templates off)
+  W.tpl_on = cfg.templates && !arena_ && n > 0;
+  if (W.tpl_on) {
+    TplArgs& T = W.tpl;
+    const size_t nb = (size_t)std::max(O.nb, 1);
+    T.ckey = as<unsigned long long>(ensure(W.ckey, sizeof(unsigned long long) * kClassSlots));
+    T.crep = as<int32_t>(ensure(W.crep, sizeof(int32_t) * kClassSlots));
+    T.bclass = as<int32_t>(ensure(W.bclass, sizeof(int32_t) * nb));
+    const bool a_is_b = O.a_is_b && O.row0 == 0 && O.n == O.nb;
+    T.aclass = a_is_b ? T.bclass : as<int32_t>(ensure(W.aclass, sizeof(int32_t) * nn));
+    T.akey = a_is_b ? T.ckey : as<unsigned long long>(ensure(W.akey, sizeof(unsigned long long) * kClassSlots));
+    T.arep = a_is_b ? T.crep : as<int32_t>(ensure(W.arep, sizeof(int32_t) * kClassSlots));
+    T.tkey = as<unsigned long long>(ensure(W.tkey, sizeof(unsigned long long) * kTplSlots));
+    T.trep = as<int32_t>(ensure(W.trep, sizeof(int32_t) * kTplSlots));
+    T.tplid = as<int32_t>(ensure(W.tplid, sizeof(int32_t) * nn));
+    T.info = as<int4>(ensure(W.tinfo, sizeof(int4) * kTplSlots));
+    T.posoff = as<int32_t>(ensure(W.tposoff, sizeof(int32_t) * (size_t)kTplSlots * (kTplMaxLen + 1)));
+    T.pos = as<uint16_t>(ensure(W.tpos, sizeof(uint16_t) * (size_t)kTplSlots * kTplMaxProducts));
+    T.soff = as<int32_t>(ensure(W.tsoff, sizeof(int32_t) * (size_t)kTplSlots * kTplMaxProducts));
+    T.tuple = as<int32_t>(ensure(W.ttuple, sizeof(int32_t) * (size_t)kTplSlots * kTplMaxLen));
+    T.pos2 = as<uint32_t>(ensure(W.tpos2, sizeof(uint32_t) * (size_t)kTplSlots * 2 * kTplMaxLen * kTplPairs));
+    T.matched = as<int32_t>(ensure(W.tmatched, sizeof(int32_t) * 2));
+    T.row0 = O.row0;
+    launch_tpl_detect(T, n, O.ap, O.ac, O.nb, bmeta, O.bc, a_is_b, s);
+  }
+  launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part,
+                 W.tpl_on ? &W.tpl : nullptr, rowcnt, rowwords, s);
   launch_bin_cursors(dstats, dcursor, nullptr, nullptr, s);
   launch_bin_scatter(n, symbin, dcursor, symrows, s);
 }
@@ -448,6 +477,10 @@ void Engine::stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, 
   for (int b = 1; b < kNumBins; b++) {
     const int32_t cnt = P.sym[b * kStatFields];
     if (cnt == 0) continue;
+    if (b == kBinTemplate) {  // counts written by the pre-pass
+      off += cnt;
+      continue;
+    }
     const int32_t mmax = P.sym[b * kStatFields + 1];
     const int32_t wmax = P.sym[b * kStatFields + 2];
     const int64_t rb = symbolic_row_lds_bytes(b, mmax, wmax);
@@ -480,7 +513,8 @@ void Engine::stage_offsets(Workspace& W, const Operands& O, const Config& cfg, i
   int32_t* dcursor = dstats + stat_words() * 2 + 4 + kNumBins;
   launch_scan(as<int32_t>(W.rowcnt.p), n, rowoff, rp32, tmp, s);
   launch_numbin(n, O.ap, as<int64_t>(W.rowf.p), as<int32_t>(W.rowcnt.p), as<int32_t>(W.rowwords.p),
-                as<int32_t>(W.rowlo.p), as<int32_t>(W.rowhi.p), cfg.num, numbin, dnum, as<int32_t>(W.statpart.p), s);
+                as<int32_t>(W.rowlo.p), as<int32_t>(W.rowhi.p), cfg.num, numbin, dnum, as<int32_t>(W.statpart.p),
+                W.tpl_on ? &W.tpl : nullptr, s);
   // the cursor kernel also parks nnz(C) next to the products count
   launch_bin_cursors(dnum, dcursor, rowoff + n, reinterpret_cast<int64_t*>(dstats + stat_words() * 2 + 2), s);
   launch_bin_scatter(n, numbin, dcursor, numrows, s);
@@ -517,7 +551,10 @@ void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, c
     // AAtS: values are formed for j >= i only (unless computed directly);
     // the structure is the full row
     RowArgs ra = row_args(W, O, numrows + off, cnt, P.direct_sym ? 0 : O.upper, cfg.num);
-    launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, heavy, s);
+    if (b == kBinTemplate)
+      launch_tpl_num(ra, W.tpl, wmax, cmax, rowoff, ccol, cval, num_cus(), lds_per_cu(), s);
+    else
+      launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, heavy, s);
     off += cnt;
   }
 }
@@ -619,6 +656,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   // a panel: output rows r0..r1-1 of A against all of B
   O.ap = A.rowptr + r0;
   O.n = n;
+  O.row0 = r0;
   record(1);
   stage_prepass(W, O, cfg_);
   read_sym(W, P, O.upper != 0);

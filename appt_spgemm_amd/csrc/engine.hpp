@@ -66,6 +66,7 @@ struct Config {
   int64_t radix_min_nnz = 1 << 18;  // asymmetric transposes of at least this many entries use the radix sort
   int64_t radix_mirror_min_nnz = 1 << 20;  // AAtS mirrors of results this large use the radix sort
   int64_t rowptr64_min_nnz = (int64_t)INT32_MAX + 1;  // results this large get int64 row pointers
+  bool templates = true;  // row templates (templates.hip) for rows sharing a relative pattern
   bool rowptr64(int64_t nnz) const { return nnz > INT32_MAX || nnz >= rowptr64_min_nnz; }
 };
 
@@ -74,17 +75,22 @@ struct Config {
 struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
       numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
-      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot, plancnt, planwords, digest;
+      rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot, plancnt, planwords, digest,
+      ckey, crep, akey, arep, bclass, aclass, tkey, trep, tplid, tinfo, tposoff, tpos, tsoff, ttuple, tpos2, tmatched;
   // words per row of the stored workgroup-bin bitmaps of the current call
   // (0: none stored)
   int32_t wg_stride = 0;
+  // row templates of the current call (tpl_on: detection ran)
+  TplArgs tpl{};
+  bool tpl_on = false;
   template <typename F>
   void each(F&& f) {
     Scratch* all[] = {&bptr,   &bcol,    &bval,    &cursor,   &bmeta, &rowf,     &rowlo,    &rowhi, &symbin, &numbin,
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
                       &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp,
                       &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot, &plancnt, &planwords,
-                      &digest};
+                      &digest, &ckey,    &crep,  &akey,   &arep,   &bclass,  &aclass, &tkey,  &trep,   &tplid,
+                      &tinfo,  &tposoff, &tpos,  &tsoff,  &ttuple, &tpos2, &tmatched};
     for (Scratch* s : all) f(*s);
   }
 };
@@ -104,6 +110,8 @@ struct Operands {
   int64_t bnnz = 0;
   int upper = 0;  // AAtS: values formed for j >= i, lower part mirrored
   int32_t ncols = 0;  // bound on output column indices (a batch's are local to each block)
+  int32_t row0 = 0;   // global row index of A-side row 0 (row panels)
+  bool a_is_b = false;  // the A-side rows are B's rows (A*A over all rows)
 };
 
 // Bin statistics read back at the two host synchronisations; they fix the

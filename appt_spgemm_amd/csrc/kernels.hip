@@ -854,7 +854,8 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
                           const int4* __restrict__ bmeta, int upper, BinParams P,
                           int64_t* __restrict__ rowf, int32_t* __restrict__ rowlo, int32_t* __restrict__ rowhi,
                           uint8_t* __restrict__ bin, int32_t* __restrict__ part,
-                          unsigned long long* __restrict__ prodpart) {
+                          unsigned long long* __restrict__ prodpart, TplArgs T, int tpl_on,
+                          int32_t* __restrict__ rowcnt, int32_t* __restrict__ rowwords) {
   // kPrepassLanes lanes per row (more rows in flight, short reductions);
   // rows of more than kPrepassLong entries are left to the whole wave
   constexpr int G = kPrepassLanes;
@@ -867,22 +868,37 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
   (void)upper;
   // one row's bin from its reduced products and span; `mine`: the lane that
   // writes the row
-  auto finish = [&](int64_t i, int64_t f, int lo, int hi, int nA, bool mine) {
+  // tpl: the row's checked template slot, or -1
+  auto finish = [&](int64_t i, int64_t f, int lo, int hi, int nA, bool mine, int tpl) {
     // bitmap windows start on a 32-column boundary, so a window word and a
     // word-hash key (j >> 5) cover the same columns and distinct-word counts
     // agree between accumulator modes
     if (lo <= hi) lo &= ~31;
     const int64_t words = span_words(lo, hi);
     int64_t stat;
-    const int b = sym_bin_of(P, f, words, (int64_t)nA, &stat);
+    int b;
+    if (tpl >= 0) {
+      // template rows: the count is the template's slot count, no symbolic
+      // pass; the statistics carry the slot count and the longest B row
+      const int4 inf = T.info[tpl];
+      b = kBinTemplate;
+      stat = inf.w;
+      if (mine) {
+        rowcnt[i] = inf.z;
+        rowwords[i] = kNoWordList;
+      }
+    } else {
+      b = sym_bin_of(P, f, words, (int64_t)nA, &stat);
+    }
     if (mine) {
+      if (tpl_on) T.tplid[i] = tpl;
       rowf[i] = f;
       rowlo[i] = lo;
       rowhi[i] = hi;
       bin[i] = (uint8_t)b;
       myprod += (unsigned long long)f;
     }
-    return std::pair<int, int64_t>(b, stat);
+    return std::pair<int, int64_t>(b, tpl >= 0 ? (int64_t)T.info[tpl].z : stat);
   };
   // every wave runs the same number of iterations so the wave-wide statistics
   // updates below are executed by converged waves
@@ -899,6 +915,13 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
       e = ap[i + 1];
     }
     const bool lng = live && e - s > kPrepassLong;
+    // a template candidate (template rows have at most kTplMaxLen <= kPrepassLong
+    // entries): valid template, same A class as its representative, and the
+    // same B class at every entry
+    int tpl = (tpl_on && live && !lng) ? T.tplid[i] : -1;
+    if (tpl >= 0 && (T.info[tpl].x == 0 || T.aclass[i] != T.aclass[T.trep[tpl]])) tpl = -1;
+    const int32_t* tup = tpl >= 0 ? T.tuple + (size_t)tpl * kTplMaxLen - s : nullptr;
+    int same = 1;
     if (live && !lng) {
       // 4 entries per lane per step: their column and B-row loads in flight together
       for (int p0 = s + lane; p0 < e; p0 += 4 * G) {
@@ -908,6 +931,11 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
         int4 bm[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) bm[u] = k[u] >= 0 ? bmeta[k[u]] : make_int4(0, 0, INT32_MAX, -1);
+        if (tpl >= 0) {
+#pragma unroll
+          for (int u = 0; u < 4; u++)
+            if (k[u] >= 0) same &= T.bclass[k[u]] == tup[p0 + u * G] ? 1 : 0;
+        }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int len = bm[u].y - bm[u].x;
@@ -922,9 +950,12 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     f = gsum<G, int64_t>(f, nullptr);
     lo = gmin<G>(lo);
     hi = gmax<G>(hi);
+    same = gmin<G>(same);
+    if (!same) tpl = -1;
     const bool leader = live && !lng && lane == 0;
-    const auto bs = finish(i, f, lo, hi, e - s, leader);
-    stats_add_wave(st, leader ? bs.first : -1, f, bs.second);
+    const auto bs = finish(i, f, lo, hi, e - s, leader, tpl);
+    // template rows: metric = products (their slot-table size), words = longest B row
+    stats_add_wave(st, leader ? bs.first : -1, f, tpl >= 0 ? T.info[tpl].w : bs.second);
     // this wave's long rows, one at a time with all 64 lanes
     uint64_t longs = __ballot(lng && lane == 0);
     while (longs) {
@@ -946,7 +977,7 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
       lf = gsum<kWave, int64_t>(lf, nullptr);
       llo = gmin<kWave>(llo);
       lhi = gmax<kWave>(lhi);
-      const auto lb = finish(li, lf, llo, lhi, le - ls, wl == 0);
+      const auto lb = finish(li, lf, llo, lhi, le - ls, wl == 0, -1);
       stats_add_wave(st, wl == 0 ? lb.first : -1, lf, lb.second);
     }
   }
@@ -961,13 +992,15 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
 }
 void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4* bmeta, int upper,
                     const BinParams& P, int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin,
-                    int32_t* stats, unsigned long long* total_products, int32_t* part, hipStream_t s) {
+                    int32_t* stats, unsigned long long* total_products, int32_t* part, const TplArgs* T,
+                    int32_t* rowcnt, int32_t* rowwords, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * kPrepassLanes + 255) / 256;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   unsigned long long* prodpart = reinterpret_cast<unsigned long long*>(part + stats_part_elems());
+  const TplArgs none{};
   hipLaunchKernelGGL(k_prepass, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, ac, bmeta, upper, P, rowf, rowlo,
-                     rowhi, bin, part, prodpart);
+                     rowhi, bin, part, prodpart, T ? *T : none, T ? 1 : 0, rowcnt, rowwords);
   stats_reduce(part, prodpart, (int)blocks, stats, total_products, s);
 }
 
@@ -975,7 +1008,8 @@ void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4*
 __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_t* __restrict__ rowf,
                          const int32_t* __restrict__ rowcnt, const int32_t* __restrict__ rowwords,
                          const int32_t* __restrict__ rowlo, const int32_t* __restrict__ rowhi, BinParams P,
-                         uint8_t* __restrict__ bin, int32_t* __restrict__ part) {
+                         uint8_t* __restrict__ bin, int32_t* __restrict__ part, const int32_t* __restrict__ tplid,
+                         const int4* __restrict__ tinfo) {
   __shared__ BinStatLds st;
   stats_init(st);
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -990,7 +1024,13 @@ __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_
       const int64_t nA = ap[i + 1] - ap[i];
       const int rw = rowwords[i];
       const int64_t listed = (rw & kNoWordList) == 0 && (rw & kWordsMask) <= kWordListCap ? (rw & kWordsMask) : -1;
-      b = num_bin_of(P, c, words, rw & kWordsMask, rowf[i], nA, listed, &stat);
+      const int t = tplid ? tplid[i] : -1;
+      if (t >= 0) {
+        b = kBinTemplate;
+        stat = tinfo[t].w;  // longest B row: lanes per row
+      } else {
+        b = num_bin_of(P, c, words, rw & kWordsMask, rowf[i], nA, listed, &stat);
+      }
       bin[i] = (uint8_t)b;
     }
     stats_add_wave(st, b, c, stat);
@@ -999,12 +1039,12 @@ __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_
 }
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
                    const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
-                   uint8_t* bin, int32_t* stats, int32_t* part, hipStream_t s) {
+                   uint8_t* bin, int32_t* stats, int32_t* part, const TplArgs* T, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n + 255) / 256;
   if (blocks > kStatsMaxBlocks) blocks = kStatsMaxBlocks;
   hipLaunchKernelGGL(k_numbin, dim3((unsigned)blocks), dim3(256), 0, s, n, ap, rowf, rowcnt, rowwords, rowlo,
-                     rowhi, P, bin, part);
+                     rowhi, P, bin, part, T ? T->tplid : nullptr, T ? T->info : nullptr);
   stats_reduce(part, nullptr, (int)blocks, stats, nullptr, s);
 }
 
@@ -1102,37 +1142,6 @@ struct Stage {
   int32_t longs[256];
   int32_t nlong;
 };
-
-// Blocks are dispatched round-robin over the 8 XCDs (block b runs on XCD
-// b % 8).  Renumber them so that each XCD walks one contiguous eighth of the
-// row list: neighbouring rows share B rows, and those then hit the same L2.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-  constexpr int kXcd = 8;
-  const int x = b % kXcd, k = b / kXcd;
-  const int per = nb / kXcd, rem = nb % kXcd;
-  return x * per + (x < rem ? x : rem) + k;
-}
-
-// Rows of a persistent window block.  The bin's row list is split into eight
-// contiguous ranges, one per XCD (block b runs on XCD b % 8), sized by the
-// XCD's block count; inside its range the blocks of one XCD take tiles of R
-// rows in turn (group gi owns row gi of a tile).  At any moment the blocks
-// of an XCD work on neighbouring rows, so the B rows they gather are shared
-// in that XCD's L2.
-struct RowWalk {
-  int first, step, end;
-};
-__device__ __forceinline__ RowWalk row_walk(int nrows, int R, int gi) {
-  constexpr int kXcd = 8;
-  const int nb = gridDim.x, b = blockIdx.x;
-  const int x = b % kXcd, k = b / kXcd;
-  const int per = nb / kXcd, rem = nb % kXcd;
-  const int nbx = per + (x < rem ? 1 : 0);
-  const int before = x * per + (x < rem ? x : rem);
-  const int x0 = (int)((int64_t)nrows * before / nb);
-  const int x1 = (int)((int64_t)nrows * (before + nbx) / nb);
-  return RowWalk{x0 + k * R + gi, nbx * R, x1};
-}
 
 // fn(j, prod) is called once per product with j = B column and prod = a*b
 // (rounded product; 0 when VALS is false).
@@ -2802,6 +2811,7 @@ __global__ void __launch_bounds__(256) k_num_window(RowArgs A, int32_t W, int32_
 
 static int g_num_cus = 256;
 void set_num_cus(int n) { g_num_cus = n > 0 ? n : 256; }
+int num_cus() { return g_num_cus; }
 
 // ---------------------------------------------------------------------------
 // launch helpers
@@ -2811,7 +2821,7 @@ static bool bin_dense(int bin) {
 }
 
 int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
-  if (bin == kBinHeavy || bin == 0) return 0;
+  if (bin == kBinHeavy || bin == 0 || bin == kBinTemplate) return 0;
   if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), words_max, 4, false);
   (void)metric_max;
   if (bin == kBinWgBitmap) return (int64_t)words_max * 4;
@@ -2820,6 +2830,7 @@ int64_t symbolic_row_lds_bytes(int bin, int32_t metric_max, int32_t words_max) {
 }
 int64_t numeric_row_lds_bytes(int bin, int32_t c_max, int32_t words_max) {
   if (bin == kBinHeavy || bin == 0) return 0;
+  if (bin == kBinTemplate) return tpl_group_bytes(c_max);
   if (bin_is_sort(bin)) return sort_row_bytes(sort_bin_slots(bin), words_max, 8, true);
   if (bin == kBinWgBitmap) return dense_row_bytes(words_max, c_max);
   if (bin_is_list(bin)) return num_list_bytes(bin_group(bin), words_max, c_max);
@@ -2863,6 +2874,7 @@ static void num_lds(const RowArgs& a, int32_t words_max, int32_t c_max, const in
 
 static int64_t g_lds_per_cu = 160 * 1024;
 void set_lds_per_cu(int64_t b) { g_lds_per_cu = b > 0 ? b : 65536; }
+int64_t lds_per_cu() { return g_lds_per_cu; }
 
 // Groups per block for a window kernel and the persistent grid.
 static void window_shape(int G, int64_t group_bytes, int32_t nrows, int64_t lds_budget, int* R_out,

@@ -18,6 +18,9 @@
 //                     products (and A entries) whose output is too scattered
 //                     for a window; the products are sorted by column in
 //                     registers (bitonic network over G lanes x E slots)
+//   48                row templates: rows whose structure is a shifted copy of
+//                     a shared pattern (templates.hip); no symbolic pass,
+//                     products go straight to their precomputed slots
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -27,8 +30,8 @@
 namespace appt {
 
 constexpr int kNumCls = 3;
-constexpr int kNumBins = 48;
-constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27, kBinList0 = 28, kBinSort0 = 40;
+constexpr int kNumBins = 49;
+constexpr int kBinWgBitmap = 25, kBinWgHash = 26, kBinHeavy = 27, kBinList0 = 28, kBinSort0 = 40, kBinTemplate = 48;
 constexpr int kSortMax = 2048;  // products (and A entries) of the largest sort bin
 constexpr int kHeavyT = 1024;   // threads of a heavy-row workgroup
 constexpr int kStatFields = 3;  // count, max metric, max words
@@ -49,7 +52,7 @@ struct BinParams {
   int32_t sort_max;               // slots of the largest register-sort bin (16 .. kSortMax; 0: no sort bins)
 };
 
-inline bool bin_is_sort(int bin) { return bin >= kBinSort0; }
+inline bool bin_is_sort(int bin) { return bin >= kBinSort0 && bin < kBinSort0 + 8; }
 inline int sort_bin_slots(int bin) { return 16 << (bin - kBinSort0); }
 inline bool bin_is_list(int bin) { return bin >= kBinList0 && bin < kBinSort0; }
 inline int bin_group(int bin) { return bin_is_list(bin) ? 8 << ((bin - kBinList0) % 4) : 8 << ((bin - 1) % 4); }
@@ -134,13 +137,17 @@ void launch_sort_rows(int32_t n, const int32_t* bp, int32_t* bc, double* bv, int
 int64_t scan_tmp_elems(int64_t n);
 void launch_scan(const int32_t* in, int64_t n, int64_t* out64, int32_t* out32, int64_t* tmp,
                  hipStream_t s, const int32_t* gate = nullptr);
+struct TplArgs;
+// T (optional): template candidates are checked against their template's
+// representative and take the template bin, with rowcnt / rowwords written.
 void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4* bmeta, int upper,
                     const BinParams& P,
                     int64_t* rowf, int32_t* rowlo, int32_t* rowhi, uint8_t* bin, int32_t* stats,
-                    unsigned long long* total_products, int32_t* part, hipStream_t s);
+                    unsigned long long* total_products, int32_t* part, const TplArgs* T, int32_t* rowcnt,
+                    int32_t* rowwords, hipStream_t s);
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
                    const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
-                   uint8_t* bin, int32_t* stats, int32_t* part, hipStream_t s);
+                   uint8_t* bin, int32_t* stats, int32_t* part, const TplArgs* T, hipStream_t s);
 // cursor[b] = start of bin b in the row list (exclusive scan of the counts
 // in a stats array of kNumBins * kStatFields).
 // (total_src, optional: *total_dst = *total_src on the way)
@@ -183,6 +190,50 @@ struct HeavyArgs {
   int32_t* cur;
   int32_t* fault;
 };
+
+// ---- row templates (templates.hip) ----
+constexpr int kClassSlots = 4096;      // relative-pattern classes per call
+constexpr int kTplSlots = 256;         // row templates per call
+constexpr int kTplMaxLen = 64;         // entries of a template row's A row and of each of its B rows
+constexpr int kTplMaxProducts = kTplMaxLen * kTplMaxLen;
+struct TplArgs {
+  unsigned long long* ckey;  // B-row class table: kClassSlots hash keys (0 = empty)
+  int32_t* crep;             // class representative (a B row)
+  unsigned long long* akey;  // A-side class table (unused when A's rows are B's rows)
+  int32_t* arep;             // class representative (an A-side row)
+  int32_t* bclass;           // per B row: class slot, or -1
+  int32_t* aclass;           // per A-side row: class slot, or -1 (== bclass when A's rows are B's rows)
+  unsigned long long* tkey;  // template table: kTplSlots keys
+  int32_t* trep;             // template representative (A-side row)
+  int32_t* tplid;            // per A-side row: template slot, or -1
+  int4* info;                // per template: {valid, A entries m, slots nS, longest B row}
+  int32_t* posoff;           // per template: kTplMaxLen + 1 product offsets of the A entries
+  uint16_t* pos;             // per template: kTplMaxProducts output slots of the products
+  int32_t* soff;             // per template: kTplMaxProducts output offsets (column - row), ascending
+  int32_t* tuple;            // per template: kTplMaxLen B classes of its representative's entries
+  uint32_t* pos2;            // per template: [parity][entry u][pair l] slots of B entries 2l - par, 2l - par + 1
+  int32_t* matched;          // [0]: B rows whose pattern repeats another's; [1]: the same for A-side rows
+  int32_t row0;              // global row index of A-side row 0 (row panels)
+};
+constexpr int kTplPairs = kTplMaxLen / 2 + 1;  // B-value pairs of one B row (either parity)
+// LDS of one template row: its A entries' values and {B-row start, first
+// product, products} (kTplMaxLen each), then nS f64 slots.
+__host__ __device__ inline int64_t tpl_group_bytes(int64_t nsmax) {
+  return (int64_t)kTplMaxLen * 24 + ((nsmax * 8 + 15) & ~(int64_t)15);
+}
+// The same for the pair-load kernel: values and {B-row start, length}.
+__host__ __device__ inline int64_t tpl_pair_group_bytes(int64_t nsmax) {
+  return (int64_t)kTplMaxLen * 16 + ((nsmax * 8 + 15) & ~(int64_t)15);
+}
+// Classes of B's rows (and of A's rows unless a_is_b), template candidates
+// of A's rows (tplid, pending the pre-pass's check against the template's
+// representative) and the template tables.
+void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb,
+                       const int4* bmeta, const int32_t* bc, bool a_is_b, hipStream_t s);
+// Numeric pass of the template bin (maxb: longest B row, nsmax: most slots
+// of a row of the bin).
+void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t nsmax, const int64_t* rowoff,
+                    int32_t* ccol, double* cval, int num_cus, int64_t lds_per_cu, hipStream_t s);
 
 // Symbolic: rowcnt[i] = distinct output columns of row i.
 // words_max: the bin's largest LDS statistic (span words or hash keys).
@@ -248,6 +299,8 @@ void launch_digest(int32_t n, int64_t nnz, const void* rp, bool rp64, const int3
 
 void set_num_cus(int n);
 void set_lds_per_cu(int64_t b);
+int num_cus();
+int64_t lds_per_cu();
 void launch_i64_to_i32(const int64_t* in, int64_t n, int32_t* out, hipStream_t s, const int32_t* gate = nullptr);
 void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s);
 

@@ -1,0 +1,744 @@
+// templates.hip -- row templates: output rows whose structure is a shifted
+// copy of a shared relative pattern (stencils, banded operators, structured
+// grids), computed without a symbolic pass and without per-product index
+// lookups.
+//
+// Relative pattern of a row r with entries at columns c_0 .. c_{m-1} (storage
+// order): the sequence (c_0 - r, ..., c_{m-1} - r).  Output row i of
+// C = A * B depends only on
+//   * the relative pattern of A's row i,
+//   * the relative patterns of the B rows k_u = c_u it references, in order.
+// Rows that agree in all of these have the same relative output structure S
+// (sorted, distinct offsets) and the same map from each product (u, v) (A
+// entry u, B entry v) to its output slot.  Such rows share one template:
+// nS = |S|, S, and pos[u][v].  Their counts need no symbolic pass (nS), and
+// their values are accumulated straight into slot pos[u][v] in A-row storage
+// order -- the order of the row-wise Gustavson loop -- so they are the same
+// bit for bit as in the other wave-level bins.
+//
+// Detection is exact.  Relative patterns are grouped into classes through a
+// table keyed by a 64-bit hash, and every row is compared entry by entry with
+// its class representative: a row that differs (a hash collision) gets no
+// class.  Output rows are grouped into templates by a hash of (A class,
+// B classes in order), and every row is compared class by class with its
+// template's representative row before it takes the template bin.  Rows
+// longer than kTplMaxLen entries, with B rows longer than that, or whose
+// template has more than kTplMaxProducts products, stay in the other bins;
+// so do all rows of a matrix whose patterns do not repeat (the tables fill
+// and stop admitting).
+#include "kernels.hpp"
+#include "device_common.hpp"
+
+namespace appt {
+
+namespace {
+
+__device__ __forceinline__ unsigned long long tmix(unsigned long long x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// Open addressing with at most kProbes probes; returns the slot holding key h
+// (inserted here if absent) or -1 when the neighbourhood is full, and sets
+// *mine when this call inserted the key.  Key 0 marks an empty slot (hashes
+// are made odd).  Probes are plain (cached) loads: most rows of a structured
+// matrix find their key already there, and a stale empty slot only costs a
+// compare-and-swap that returns the key.  Nothing here is a same-address
+// atomic per row (those serialise in one L2 channel).
+constexpr int kProbes = 8;
+__device__ __forceinline__ int table_insert(unsigned long long* keys, int nslots, unsigned long long h, bool* mine) {
+  int slot = (int)((h >> 17) & (unsigned long long)(nslots - 1));
+  *mine = false;
+  for (int p = 0; p < kProbes; p++) {
+    // device-coherent read (the L2): a key inserted by another CU is seen
+    const unsigned long long k = __hip_atomic_load(&keys[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (k == h) return slot;
+    if (k == 0ull) {
+      const unsigned long long old = atomicCAS(&keys[slot], 0ull, h);
+      if (old == 0ull) {
+        *mine = true;
+        return slot;
+      }
+      if (old == h) return slot;
+    }
+    slot = (slot + 1) & (nslots - 1);
+  }
+  return -1;
+}
+
+// Block-local cache in front of table_insert: rows of one block (neighbouring
+// rows of a structured matrix) mostly share their key, so only the first row
+// of each key in a block probes the global table.  An entry packs the key's
+// high 32 bits with slot + 1 in one 64-bit LDS word (written and read whole);
+// a false hit between two keys that share their high bits only costs the row
+// its class or template, which the exact checks that follow would drop.
+constexpr int kCacheSlots = 64;
+struct KeyCache {
+  unsigned long long e[kCacheSlots];
+};
+__device__ __forceinline__ void cache_init(KeyCache& c) {
+  for (int t = threadIdx.x; t < kCacheSlots; t += blockDim.x) c.e[t] = 0ull;
+  __syncthreads();
+}
+__device__ __forceinline__ int cached_insert(KeyCache& c, unsigned long long* keys, int nslots, unsigned long long h,
+                                             bool* mine) {
+  const int idx = (int)((h >> 7) & (kCacheSlots - 1));
+  const unsigned long long e = c.e[idx];
+  *mine = false;
+  if ((e >> 32) == (h >> 32) && (e & 0xffffffffull) != 0ull) return (int)(e & 0xffffffffull) - 1;
+  const int slot = table_insert(keys, nslots, h, mine);
+  if (slot >= 0) c.e[idx] = (h & 0xffffffff00000000ull) | (unsigned long long)(slot + 1);
+  return slot;
+}
+
+// Entries of row r: [start, end) and its relative base.  B rows are read
+// through bmeta (B row k is global row k); A-side rows through ap (panel row
+// i is global row row0 + i).
+struct Rows {
+  const int32_t* ap;   // null: B rows through bmeta
+  const int4* bmeta;
+  int32_t row0;
+  __device__ __forceinline__ int2 range(int r) const {
+    if (ap) return make_int2(ap[r], ap[r + 1]);
+    const int4 b = bmeta[r];
+    return make_int2(b.x, b.y);
+  }
+  __device__ __forceinline__ int64_t base(int r) const { return ap ? (int64_t)row0 + r : (int64_t)r; }
+};
+
+// Insertion for the group leaders of a wavefront (lanes with want): leaders
+// holding the same key are served by one probe (the first of them), so a
+// wavefront of rows that share a pattern costs one cache lookup and at most
+// one probe of the global table; the table's device-wide reads and
+// compare-and-swaps go past the XCD's L2 and are kept rare.  *mine is set
+// for the lane whose probe inserted the key.
+__device__ __forceinline__ int wave_insert(KeyCache& kc, unsigned long long* keys, int nslots, unsigned long long h,
+                                           bool want, bool* mine) {
+  const int lane = threadIdx.x % kWave;
+  unsigned long long pending = __ballot(want);
+  int result = -1;
+  *mine = false;
+  while (pending) {
+    const int leader = __builtin_ctzll(pending);
+    const unsigned long long hl = __shfl(h, leader, kWave);
+    const unsigned long long same = __ballot(want && h == hl) & pending;
+    int slot = -1;
+    bool ins = false;
+    if (lane == leader) slot = cached_insert(kc, keys, nslots, h, &ins);
+    slot = __shfl(slot, leader, kWave);
+    if ((same >> lane) & 1ull) {
+      result = slot;
+      *mine = lane == leader && ins;
+    }
+    pending &= ~same;
+  }
+  return result;
+}
+
+// The class and template kernels take kClsLanes lanes per row, each lane
+// loading kClsUnroll entries per step (their loads in flight together), so a
+// wavefront keeps many rows' loads outstanding.
+constexpr int kClsLanes = 4;
+constexpr int kClsUnroll = 4;
+constexpr int64_t kClsMaxBlocks = 4096;
+
+__device__ __forceinline__ unsigned long long entry_hash(int u, int64_t rel) {
+  return tmix(((unsigned long long)(u + 1) << 32) ^ (unsigned long long)(uint32_t)(int32_t)rel);
+}
+
+// Pattern class of each row: hash of its relative pattern, inserted into the
+// class table; the row that inserts a key is the class representative (any
+// row of the class would do: every other row is compared with it exactly).
+__global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, const int32_t* __restrict__ col,
+                                                      unsigned long long* __restrict__ keys,
+                                                      int32_t* __restrict__ rep, int32_t* __restrict__ cls) {
+  constexpr int G = kClsLanes;
+  __shared__ KeyCache kc;
+  cache_init(kc);
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  const int64_t stride = (int64_t)gridDim.x * gpb;
+  // every wave runs the same number of iterations (group shuffles stay converged)
+  const int64_t wave_first = (int64_t)blockIdx.x * gpb + (threadIdx.x / kWave) * (kWave / G);
+  for (int64_t w0 = wave_first; w0 < nrows; w0 += stride) {
+    const int64_t r = w0 + (threadIdx.x % kWave) / G;
+    int2 se = make_int2(0, 0);
+    if (r < nrows) se = R.range((int)r);
+    const int m = se.y - se.x;
+    const bool ok = r < nrows && m > 0 && m <= kTplMaxLen;
+    const int64_t b = ok ? R.base((int)r) : 0;
+    unsigned long long h = 0;
+    if (ok)
+      for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
+        int c[kClsUnroll];
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++) c[x] = u0 + x * G < m ? col[se.x + u0 + x * G] : 0;
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++)
+          if (u0 + x * G < m) h += entry_hash(u0 + x * G, (int64_t)c[x] - b);
+      }
+    h = gsum<G, unsigned long long>(h, nullptr);
+    h = (h + tmix((unsigned long long)m)) | 1ull;
+    bool mine;
+    const int slot = wave_insert(kc, keys, kClassSlots, h, lane == 0 && ok, &mine);
+    if (lane == 0 && r < nrows) {
+      if (mine) rep[slot] = (int32_t)r;
+      cls[r] = ok ? slot : -1;
+    }
+  }
+}
+
+// Entry-by-entry comparison of each classed row with its class
+// representative; a row that differs loses its class.
+// Rows that matched a class other than their own (repeated patterns) are
+// counted into *matched (one atomic per block).
+__global__ void __launch_bounds__(256) k_class_verify(int32_t nrows, Rows R, const int32_t* __restrict__ col,
+                                                      const int32_t* __restrict__ rep, int32_t* __restrict__ cls,
+                                                      int32_t* __restrict__ matched) {
+  constexpr int G = kClsLanes;
+  __shared__ int32_t block_matched;
+  if (threadIdx.x == 0) block_matched = 0;
+  __syncthreads();
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  const int64_t stride = (int64_t)gridDim.x * gpb;
+  const int64_t wave_first = (int64_t)blockIdx.x * gpb + (threadIdx.x / kWave) * (kWave / G);
+  for (int64_t w0 = wave_first; w0 < nrows; w0 += stride) {
+    const int64_t r = w0 + (threadIdx.x % kWave) / G;
+    const int c = r < nrows ? cls[r] : -1;
+    const int q = c >= 0 ? rep[c] : -1;
+    int same = 1;
+    if (c >= 0 && q != (int)r) {
+      const int2 se = R.range((int)r), sq = R.range(q);
+      const int64_t b = R.base((int)r), bq = R.base(q);
+      const int m = se.y - se.x;
+      if (m != sq.y - sq.x) {
+        same = 0;
+      } else {
+        for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
+          int x0[kClsUnroll], x1[kClsUnroll];
+#pragma unroll
+          for (int x = 0; x < kClsUnroll; x++) {
+            const bool in = u0 + x * G < m;
+            x0[x] = in ? col[se.x + u0 + x * G] : 0;
+            x1[x] = in ? col[sq.x + u0 + x * G] : 0;
+          }
+#pragma unroll
+          for (int x = 0; x < kClsUnroll; x++)
+            if (u0 + x * G < m && (int64_t)x0[x] - b != (int64_t)x1[x] - bq) same = 0;
+        }
+      }
+    }
+    same = gmin<G>(same);
+    if (lane == 0 && c >= 0 && !same) cls[r] = -1;
+    const int hits = __popcll(__ballot(lane == 0 && c >= 0 && same && q != (int)r));
+    if (threadIdx.x % kWave == 0 && hits) atomicAdd(&block_matched, hits);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && block_matched) atomicAdd(matched, block_matched);
+}
+
+// Template candidate of each A-side row: A class and the classes of its B
+// rows, in order.  Writes the template slot (or -1) to tplid; the row that
+// inserts a template's key is its representative.
+// Templates only pay when B's row patterns repeat: unless at least a quarter
+// of B's rows matched another row's pattern (*matched of nb), every row gets
+// tplid -1 and no table is built.
+__device__ __forceinline__ bool templates_worth(const int32_t* matched, int32_t nb) {
+  return (int64_t)matched[0] * 4 >= (int64_t)nb;
+}
+
+__global__ void __launch_bounds__(256) k_tpl_insert(int32_t n, const int32_t* __restrict__ ap,
+                                                    const int32_t* __restrict__ ac, const int32_t* __restrict__ aclass,
+                                                    const int32_t* __restrict__ bclass,
+                                                    unsigned long long* __restrict__ keys, int32_t* __restrict__ rep,
+                                                    int32_t* __restrict__ tplid, const int32_t* __restrict__ matched,
+                                                    int32_t nb) {
+  constexpr int G = kClsLanes;
+  if (!templates_worth(matched, nb)) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+      tplid[i] = -1;
+    return;
+  }
+  __shared__ KeyCache kc;
+  cache_init(kc);
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  const int64_t stride = (int64_t)gridDim.x * gpb;
+  const int64_t wave_first = (int64_t)blockIdx.x * gpb + (threadIdx.x / kWave) * (kWave / G);
+  for (int64_t w0 = wave_first; w0 < n; w0 += stride) {
+    const int64_t i = w0 + (threadIdx.x % kWave) / G;
+    const int a = i < n ? aclass[i] : -1;
+    unsigned long long h = 0;
+    int ok = a >= 0 ? 1 : 0;
+    if (a >= 0) {
+      const int s = ap[i], e = ap[i + 1];
+      for (int u0 = lane; u0 < e - s; u0 += G * kClsUnroll) {
+        int k[kClsUnroll];
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++) k[x] = u0 + x * G < e - s ? ac[s + u0 + x * G] : -1;
+        int bc[kClsUnroll];
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++) bc[x] = k[x] >= 0 ? bclass[k[x]] : 0;
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++)
+          if (k[x] >= 0) {
+            ok &= bc[x] >= 0 ? 1 : 0;
+            h += tmix(((unsigned long long)(u0 + x * G + 1) << 32) ^ (unsigned long long)(uint32_t)bc[x]);
+          }
+      }
+    }
+    h = gsum<G, unsigned long long>(h, nullptr);
+    ok = gmin<G>(ok);
+    h = (h + tmix(0x7a3ull ^ ((unsigned long long)(uint32_t)a << 8))) | 1ull;
+    bool mine;
+    const int slot = wave_insert(kc, keys, kTplSlots, h, lane == 0 && ok, &mine);
+    if (lane == 0 && i < n) {
+      if (mine) rep[slot] = (int32_t)i;
+      tplid[i] = ok ? slot : -1;
+    }
+  }
+}
+
+// Template tables, one block per template slot, from its representative row:
+// the products' output offsets (B column - row), sorted; the distinct ones
+// are S, and pos[(u, v)] is each product's rank among them.  The template is
+// marked invalid when it has more than kTplMaxProducts products.
+constexpr int kBuildT = 1024;
+__global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
+                                                       const int4* __restrict__ bmeta, const int32_t* __restrict__ bc,
+                                                       int32_t row0, const unsigned long long* __restrict__ keys,
+                                                       const int32_t* __restrict__ rep, const int32_t* __restrict__ bclass,
+                                                       int4* __restrict__ info, int32_t* __restrict__ posoff,
+                                                       uint16_t* __restrict__ pos, int32_t* __restrict__ soff,
+                                                       int32_t* __restrict__ tuple, uint32_t* __restrict__ pos2) {
+  __shared__ unsigned long long key[kTplMaxProducts];
+  __shared__ uint16_t rank_of[kTplMaxProducts];
+  __shared__ int32_t off[kTplMaxLen + 1];
+  __shared__ int32_t bstart[kTplMaxLen];
+  __shared__ int32_t scan[kBuildT / kWave + 1];
+  __shared__ int32_t maxb_s;
+  const int t = blockIdx.x, tid = threadIdx.x;
+  if (keys[t] == 0ull) {
+    if (tid == 0) info[t] = make_int4(0, 0, 0, 0);
+    return;
+  }
+  const int r = rep[t];
+  const int s = ap[r], m = ap[r + 1] - s;
+  const int64_t gi = (int64_t)row0 + r;
+  if (tid == 0) {
+    int run = 0, maxb = 0;
+    for (int u = 0; u < m; u++) {
+      const int4 b = bmeta[ac[s + u]];
+      off[u] = run;
+      bstart[u] = b.x;
+      run += b.y - b.x;
+      maxb = b.y - b.x > maxb ? b.y - b.x : maxb;
+    }
+    off[m] = run;
+    maxb_s = maxb;
+  }
+  __syncthreads();
+  const int P = off[m];
+  if (P > kTplMaxProducts || P <= 0) {
+    if (tid == 0) info[t] = make_int4(0, m, 0, 0);
+    return;
+  }
+  int32_t* po = posoff + (size_t)t * (kTplMaxLen + 1);
+  for (int u = tid; u <= m; u += kBuildT) po[u] = off[u];
+  // the representative's B classes, entry by entry: the pre-pass compares
+  // every candidate row against them
+  for (int u = tid; u < m; u += kBuildT) tuple[(size_t)t * kTplMaxLen + u] = bclass[ac[s + u]];
+  // keys: (offset + 2^31) << 32 | product index; padding sorts last
+  int np2 = 1;
+  while (np2 < P) np2 <<= 1;
+  for (int p = tid; p < np2; p += kBuildT) {
+    unsigned long long k = ~0ull;
+    if (p < P) {
+      int u = 0;
+      while (off[u + 1] <= p) u++;
+      const int v = p - off[u];
+      const int64_t d = (int64_t)bc[bstart[u] + v] - gi;
+      k = ((unsigned long long)(d + 2147483648LL) << 32) | (unsigned long long)p;
+    }
+    key[p] = k;
+  }
+  __syncthreads();
+  // bitonic sort of np2 keys in LDS
+  for (int kk = 2; kk <= np2; kk <<= 1)
+    for (int j = kk >> 1; j > 0; j >>= 1) {
+      for (int p = tid; p < np2; p += kBuildT) {
+        const int q = p ^ j;
+        if (q > p) {
+          const bool up = (p & kk) == 0;
+          const unsigned long long x = key[p], y = key[q];
+          if ((x > y) == up) {
+            key[p] = y;
+            key[q] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  // rank of each distinct offset: block scan of "first of its offset" flags,
+  // kBuildT elements per round
+  int base = 0;
+  uint16_t* pp = pos + (size_t)t * kTplMaxProducts;
+  int32_t* so = soff + (size_t)t * kTplMaxProducts;
+  for (int p0 = 0; p0 < P; p0 += kBuildT) {
+    const int p = p0 + tid;
+    const bool head = p < P && (p == 0 || (key[p] >> 32) != (key[p - 1] >> 32));
+    int tot;
+    const int ex = block_scan_excl<int>(head ? 1 : 0, scan, &tot);
+    if (p < P) {
+      const int rank = base + ex + (head ? 1 : 0) - 1;
+      pp[key[p] & 0xffffffffull] = (uint16_t)rank;
+      rank_of[key[p] & 0xffffffffull] = (uint16_t)rank;
+      if (head) so[rank] = (int32_t)((int64_t)(key[p] >> 32) - 2147483648LL);
+    }
+    base += tot;
+    __syncthreads();
+  }
+  // pair tables: for parity par and entry u, pair l holds the slots of
+  // entries 2l - par and 2l - par + 1 of B row k_u (0xffff: none)
+  uint32_t* p2 = pos2 + (size_t)t * (2 * kTplMaxLen * kTplPairs);
+  __syncthreads();
+  for (int x = tid; x < 2 * m * kTplPairs; x += kBuildT) {
+    const int par = x / (m * kTplPairs), u = (x / kTplPairs) % m, l = x % kTplPairs;
+    const int len = off[u + 1] - off[u];
+    const int v0 = 2 * l - par, v1 = v0 + 1;
+    const uint32_t s0 = (v0 >= 0 && v0 < len) ? rank_of[off[u] + v0] : 0xffffu;
+    const uint32_t s1 = (v1 >= 0 && v1 < len) ? rank_of[off[u] + v1] : 0xffffu;
+    p2[((size_t)par * kTplMaxLen + u) * kTplPairs + l] = s0 | (s1 << 16);
+  }
+  if (tid == 0) info[t] = make_int4(1, m, base, maxb_s);
+}
+
+}  // namespace
+
+void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb,
+                       const int4* bmeta, const int32_t* bc, bool a_is_b, hipStream_t s) {
+  // tables empty (a representative is written by the row that inserts its key)
+  launch_fill_i32(reinterpret_cast<int32_t*>(T.ckey), 2 * kClassSlots, 0, s);
+  launch_fill_i32(reinterpret_cast<int32_t*>(T.tkey), 2 * kTplSlots, 0, s);
+  launch_fill_i32(T.matched, 2, 0, s);
+  auto grid = [](int64_t rows) {
+    int64_t b = (rows * kClsLanes + 255) / 256;
+    return (unsigned)(b < 1 ? 1 : (b > kClsMaxBlocks ? kClsMaxBlocks : b));
+  };
+  const Rows RB{nullptr, bmeta, 0};
+  if (nb > 0) {
+    hipLaunchKernelGGL(k_class_insert, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep, T.bclass);
+    hipLaunchKernelGGL(k_class_verify, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.crep, T.bclass, T.matched);
+  }
+  if (n <= 0) return;
+  if (!a_is_b) {
+    // A-side rows (a row panel, or A next to B = A^T): a table of their own,
+    // whose representatives are A-side rows
+    launch_fill_i32(reinterpret_cast<int32_t*>(T.akey), 2 * kClassSlots, 0, s);
+    const Rows RA{ap, nullptr, T.row0};
+    hipLaunchKernelGGL(k_class_insert, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.akey, T.arep, T.aclass);
+    hipLaunchKernelGGL(k_class_verify, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.arep, T.aclass, T.matched + 1);
+  }
+  hipLaunchKernelGGL(k_tpl_insert, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.aclass, T.bclass, T.tkey, T.trep,
+                     T.tplid, T.matched, nb);
+  hipLaunchKernelGGL(k_tpl_build, dim3(kTplSlots), dim3(kBuildT), 0, s, ap, ac, bmeta, bc, T.row0, T.tkey, T.trep,
+                     T.bclass, T.info, T.posoff, T.pos, T.soff, T.tuple, T.pos2);
+}
+
+// ---------------------------------------------------------------------------
+// Numeric pass of template rows.  G = 32 lanes per row (two rows per
+// wavefront), rows walked XCD-locally by persistent blocks and
+// software-pipelined four rows deep, so that no load waits on another load
+// issued in the same iteration: rows[r] -> (template, row start and length)
+// -> (a_iu, B-row id, template product offsets of entry u) -> B-row start ->
+// products.  For a row, entry u's values are staged in LDS; then every
+// product's gathers (b_kv, and its slot pos[u][v] in the template) are issued
+// at once -- lane v takes entry v of B row k_u, for all u -- and the LDS f64
+// atomics follow in u order, so every slot receives its products in A-row
+// storage order (bit-identical to the row-wise Gustavson loop).  The row is
+// written out in slot order (S is sorted): columns row + S[s].
+// Rows of up to kTplFast A entries; longer template rows take the same
+// product loop kTplFast entries at a time.
+// ---------------------------------------------------------------------------
+#ifndef APPT_TPL_FAST
+#define APPT_TPL_FAST 6
+#endif
+constexpr int kTplFast = APPT_TPL_FAST;  // A entries whose gathers are in flight together
+
+// Gather with a 32-bit byte offset from a uniform base when the array is
+// small enough (one address register per load instead of two).
+template <typename V, bool W64>
+__device__ __forceinline__ V gather(const V* base, uint32_t idx) {
+  if constexpr (W64) return base[idx];
+  else return *(const V*)((const char*)base + (size_t)(idx * (uint32_t)sizeof(V)));
+}
+
+struct TplRow {
+  int i, t, s, len;   // row, template, first entry, entries
+  int k, o, e, bs;    // this lane's entry (lane < len): B row, products [o, e), B-row start
+  double a;           // a_iu of this lane's entry
+};
+
+template <int G, bool W64>
+__global__ void __launch_bounds__(256) k_tpl_num(RowArgs A, TplArgs T, int32_t nsmax,
+                                                 const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
+                                                 double* __restrict__ cval) {
+  // G covers the longest B row of the launch (lane v takes entry v)
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int R = blockDim.x / G;
+  const int gi = threadIdx.x / G, lane = threadIdx.x % G;
+  unsigned char* base = lds_raw + (size_t)gi * (size_t)tpl_group_bytes(nsmax);
+  double* sa = (double*)base;                         // a_iu
+  int4* se = (int4*)(base + kTplMaxLen * 8);          // {B-row start, first product, products, -}
+  double* vals = (double*)(base + kTplMaxLen * 24);
+
+  const RowWalk walk = row_walk(A.nrows, R, gi);
+  const int S = walk.step, r1 = walk.end;
+  auto hop1 = [&](TplRow& q) {
+    q.t = T.tplid[q.i];
+    q.s = A.ap[q.i];
+    q.len = A.ap[q.i + 1] - q.s;
+  };
+  auto hop2 = [&](TplRow& q) {
+    q.k = -1;
+    if (lane < q.len) {
+      const int32_t* po = T.posoff + (size_t)q.t * (kTplMaxLen + 1);
+      q.a = A.av[q.s + lane];
+      q.k = A.ac[q.s + lane];
+      q.o = po[lane];
+      q.e = po[lane + 1];
+    }
+  };
+  auto hop3 = [&](TplRow& q) {
+    if (q.k >= 0) q.bs = A.bmeta[q.k].x;
+  };
+  int r = walk.first;
+  TplRow q0{}, q1{}, q2{}, q3{};
+  q0.len = q1.len = q2.len = q3.len = 0;
+  q0.k = q1.k = q2.k = q3.k = -1;
+  if (r < r1) { q0.i = A.rows[r]; hop1(q0); hop2(q0); hop3(q0); }
+  if (r + S < r1) { q1.i = A.rows[r + S]; hop1(q1); hop2(q1); }
+  if (r + 2 * S < r1) { q2.i = A.rows[r + 2 * S]; hop1(q2); }
+  if (r + 3 * S < r1) q3.i = A.rows[r + 3 * S];
+  for (; r < r1; r += S) {
+    TplRow q4{};
+    q4.len = 0;
+    q4.k = -1;
+    if (r + 4 * S < r1) q4.i = A.rows[r + 4 * S];
+    if (r + 3 * S < r1) hop1(q3);
+    if (r + 2 * S < r1) hop2(q2);
+    if (r + S < r1) hop3(q1);
+    const int i = q0.i, t = q0.t, m = q0.len;
+    const int4 inf = T.info[t];
+    const int nS = inf.z;
+    const int64_t off = rowoff[i];
+    const uint16_t* pp = T.pos + (size_t)t * kTplMaxProducts;
+    for (int q = lane; q < nS; q += G) vals[q] = 0.0;
+    if (lane < m) {
+      sa[lane] = q0.a;
+      se[lane] = make_int4(q0.bs, q0.o, q0.e - q0.o, 0);
+    }
+    for (int u = lane + G; u < m; u += G) {   // rows longer than G entries
+      const int k = A.ac[q0.s + u];
+      const int32_t* po = T.posoff + (size_t)t * (kTplMaxLen + 1);
+      sa[u] = A.av[q0.s + u];
+      se[u] = make_int4(A.bmeta[k].x, po[u], po[u + 1] - po[u], 0);
+    }
+    gsync<G>();
+    for (int u0 = 0; u0 < m; u0 += kTplFast) {
+      double pv[kTplFast];
+      uint32_t slot[kTplFast];
+#pragma unroll
+      for (int d = 0; d < kTplFast; d++) {
+        const int u = u0 + d < m ? u0 + d : u0;
+        const int4 x = se[u];
+        const bool ok = u0 + d < m && lane < x.z;
+        const int v = ok ? lane : 0;
+        slot[d] = ok ? (uint32_t)gather<uint16_t, false>(pp, (uint32_t)(x.y + v)) : 0xffffffffu;
+        pv[d] = sa[u] * gather<double, W64>(A.bv, (uint32_t)(x.x + v));
+      }
+#pragma unroll
+      for (int d = 0; d < kTplFast; d++)
+        if (slot[d] != 0xffffffffu) atomicAdd(&vals[slot[d]], pv[d]);
+    }
+    gsync<G>();
+    const int64_t row = (int64_t)T.row0 + i;
+    const int32_t* so = T.soff + (size_t)t * kTplMaxProducts;
+    for (int q = lane; q < nS; q += G) {
+      ccol[off + q] = (int32_t)(row + so[q]);
+      cval[off + q] = vals[q];
+    }
+    gsync<G>();
+    q0 = q1; q1 = q2; q2 = q3; q3 = q4;
+  }
+}
+
+// Template rows whose B rows have at most kTplPairMax entries: G = 16 lanes
+// per row (four rows per wavefront), each lane gathering an aligned pair of
+// B values with one 16-byte load.  B row k_u starting at position bs is read
+// as pairs from bs & ~1, so lane l holds entries v = 2l - par and 2l - par + 1
+// (par = bs & 1); the template keeps the slots of both for each parity
+// (pos2).  Otherwise as k_tpl_num: entries u in storage order, their gathers
+// issued kTplFast at a time, LDS f64 atomics in u order.
+constexpr int kTplPairG = 16;
+constexpr int kTplPairMax = 2 * kTplPairG - 1;
+
+struct TplRow2 {
+  int i, t, s, len;
+  int k, bs;
+  double a;
+};
+
+template <bool W64>
+__global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t nsmax,
+                                                  const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
+                                                  double* __restrict__ cval) {
+  constexpr int G = kTplPairG;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  const int R = blockDim.x / G;
+  const int gi = threadIdx.x / G, lane = threadIdx.x % G;
+  unsigned char* base = lds_raw + (size_t)gi * (size_t)tpl_pair_group_bytes(nsmax);
+  double* sa = (double*)base;                  // a_iu
+  int2* se = (int2*)(base + kTplMaxLen * 8);   // {B-row start, B-row length}
+  double* vals = (double*)(base + kTplMaxLen * 16);
+  const RowWalk walk = row_walk(A.nrows, R, gi);
+  const int S = walk.step, r1 = walk.end;
+  const uint32_t bnnz = (uint32_t)A.bnnz;
+  auto hop1 = [&](TplRow2& q) {
+    q.t = T.tplid[q.i];
+    q.s = A.ap[q.i];
+    q.len = A.ap[q.i + 1] - q.s;
+  };
+  auto hop2 = [&](TplRow2& q) {
+    q.k = -1;
+    if (lane < q.len) {
+      q.a = A.av[q.s + lane];
+      q.k = A.ac[q.s + lane];
+    }
+  };
+  auto hop3 = [&](TplRow2& q) {
+    if (q.k >= 0) {
+      const int4 b = A.bmeta[q.k];
+      q.bs = b.x;
+      q.k = b.y - b.x;   // from here on: the B row's length
+    }
+  };
+  int r = walk.first;
+  TplRow2 q0{}, q1{}, q2{}, q3{};
+  q0.len = q1.len = q2.len = q3.len = 0;
+  q0.k = q1.k = q2.k = q3.k = -1;
+  if (r < r1) { q0.i = A.rows[r]; hop1(q0); hop2(q0); hop3(q0); }
+  if (r + S < r1) { q1.i = A.rows[r + S]; hop1(q1); hop2(q1); }
+  if (r + 2 * S < r1) { q2.i = A.rows[r + 2 * S]; hop1(q2); }
+  if (r + 3 * S < r1) q3.i = A.rows[r + 3 * S];
+  for (; r < r1; r += S) {
+    TplRow2 q4{};
+    q4.len = 0;
+    q4.k = -1;
+    if (r + 4 * S < r1) q4.i = A.rows[r + 4 * S];
+    if (r + 3 * S < r1) hop1(q3);
+    if (r + 2 * S < r1) hop2(q2);
+    if (r + S < r1) hop3(q1);
+    const int i = q0.i, t = q0.t, m = q0.len;
+    const int nS = T.info[t].z;
+    const int64_t off = rowoff[i];
+    const uint32_t* p2 = T.pos2 + (size_t)t * (2 * kTplMaxLen * kTplPairs);
+    for (int q = lane; q < nS; q += G) vals[q] = 0.0;
+    if (lane < m) {
+      sa[lane] = q0.a;
+      se[lane] = make_int2(q0.bs, q0.k);
+    }
+    for (int u = lane + G; u < m; u += G) {   // rows longer than G entries
+      const int4 b = A.bmeta[A.ac[q0.s + u]];
+      sa[u] = A.av[q0.s + u];
+      se[u] = make_int2(b.x, b.y - b.x);
+    }
+    gsync<G>();
+    for (int u0 = 0; u0 < m; u0 += kTplFast) {
+      double pv0[kTplFast], pv1[kTplFast];
+      uint32_t sl[kTplFast];
+#pragma unroll
+      for (int d = 0; d < kTplFast; d++) {
+        const int u = u0 + d < m ? u0 + d : u0;
+        const int2 x = se[u];
+        const int par = x.x & 1;
+        const bool live = u0 + d < m && 2 * lane < x.y + par;
+        const uint32_t q = (uint32_t)((x.x & ~1) + 2 * lane);   // even: 16-byte aligned pair
+        sl[d] = live ? gather<uint32_t, false>(p2, (uint32_t)(((par * kTplMaxLen) + u) * kTplPairs + lane))
+                     : 0xffffffffu;
+        double2 b = make_double2(0.0, 0.0);
+        if (live) {
+          if (q + 1 < bnnz) b = gather<double2, W64>((const double2*)A.bv, q >> 1);
+          else b.x = gather<double, W64>(A.bv, q);   // last pair of B: its second half is past the end
+        }
+        const double au = sa[u];
+        pv0[d] = au * b.x;
+        pv1[d] = au * b.y;
+      }
+#pragma unroll
+      for (int d = 0; d < kTplFast; d++) {
+        const uint32_t s0 = sl[d] & 0xffffu, s1 = sl[d] >> 16;
+        if (s0 != 0xffffu) atomicAdd(&vals[s0], pv0[d]);
+        if (s1 != 0xffffu) atomicAdd(&vals[s1], pv1[d]);
+      }
+    }
+    gsync<G>();
+    const int64_t row = (int64_t)T.row0 + i;
+    const int32_t* so = T.soff + (size_t)t * kTplMaxProducts;
+    for (int q = lane; q < nS; q += G) {
+      ccol[off + q] = (int32_t)(row + so[q]);
+      cval[off + q] = vals[q];
+    }
+    gsync<G>();
+    q0 = q1; q1 = q2; q2 = q3; q3 = q4;
+  }
+}
+
+void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t nsmax, const int64_t* rowoff,
+                    int32_t* ccol, double* cval, int num_cus, int64_t lds_per_cu, hipStream_t s) {
+  if (a.nrows <= 0) return;
+  if (maxb <= kTplPairMax) {
+    const int64_t gb = tpl_pair_group_bytes(nsmax);
+    int R = 256 / kTplPairG;
+    while (R > 4 && (int64_t)R * gb * 4 > lds_per_cu) R >>= 1;
+    const size_t lds = (size_t)R * (size_t)gb;
+    int64_t blocks = ((int64_t)a.nrows + R - 1) / R;
+    const int64_t cap = (int64_t)num_cus * 8;
+    if (blocks > cap) blocks = cap;
+    if (a.bnnz >= ((int64_t)1 << 29)) {
+      (void)hipFuncSetAttribute((const void*)k_tpl_num2<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(k_tpl_num2<true>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax, rowoff,
+                         ccol, cval);
+    } else {
+      (void)hipFuncSetAttribute((const void*)k_tpl_num2<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      hipLaunchKernelGGL(k_tpl_num2<false>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax, rowoff,
+                         ccol, cval);
+    }
+    return;
+  }
+  const int64_t gb = tpl_group_bytes(nsmax);
+  const int G = maxb <= 32 ? 32 : 64;
+  int R = 256 / G;
+  while (R > 2 && (int64_t)R * gb * 4 > lds_per_cu) R >>= 1;
+  const size_t lds = (size_t)R * (size_t)gb;
+  int64_t blocks = ((int64_t)a.nrows + R - 1) / R;
+  const int64_t cap = (int64_t)num_cus * 8;
+  if (blocks > cap) blocks = cap;
+  const bool w64 = a.bnnz >= ((int64_t)1 << 29);
+  const void* fn = G == 32 ? (w64 ? (const void*)k_tpl_num<32, true> : (const void*)k_tpl_num<32, false>)
+                           : (w64 ? (const void*)k_tpl_num<64, true> : (const void*)k_tpl_num<64, false>);
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  const dim3 grid((unsigned)blocks), block((unsigned)(R * G));
+  if (G == 32) {
+    if (w64) hipLaunchKernelGGL((k_tpl_num<32, true>), grid, block, lds, s, a, T, nsmax, rowoff, ccol, cval);
+    else hipLaunchKernelGGL((k_tpl_num<32, false>), grid, block, lds, s, a, T, nsmax, rowoff, ccol, cval);
+  } else {
+    if (w64) hipLaunchKernelGGL((k_tpl_num<64, true>), grid, block, lds, s, a, T, nsmax, rowoff, ccol, cval);
+    else hipLaunchKernelGGL((k_tpl_num<64, false>), grid, block, lds, s, a, T, nsmax, rowoff, ccol, cval);
+  }
+}
+
+}  // namespace appt

@@ -103,7 +103,10 @@ typedef struct appt_tuning {
                                  a power of two in [16, 2048]; 0 = 2048; < 0 = no register-sort bins) */
   int32_t rowptr64_min_nnz;   /* results with at least this many entries get int64 row pointers
                                  (0 = only when nnz(C) >= 2^31, where int32 cannot hold them) */
-  int32_t reserved[2];
+  int32_t row_templates;      /* rows whose own relative pattern and whose B rows' patterns repeat
+                                 share a template: no symbolic pass, products straight to their slots
+                                 (0 = on, < 0 = off) */
+  int32_t reserved[1];
 } appt_tuning;
 
 typedef struct appt_options {
@@ -163,7 +166,7 @@ typedef struct appt_bin_report {
 } appt_bin_report;
 appt_status appt_engine_bin_report(const appt_engine* eng, appt_bin_report* out);
 /* Static name of a bin id, e.g. "window/c1/G16", "hash/c0/G8", "list/c0/G32",
- * "sort/N256", "wg-window", "wg-hash", "heavy"; "empty" for 0, NULL past the end. */
+ * "sort/N256", "wg-window", "wg-hash", "heavy", "template"; "empty" for 0, NULL past the end. */
 const char* appt_bin_name(int32_t bin);
 /* Release cached device memory held by the engine's pool. */
 appt_status appt_engine_trim(appt_engine* eng);
