@@ -112,7 +112,8 @@ class ServerStats(C.Structure):
                 ("failed", C.c_int64), ("p50_ms", C.c_double), ("p99_ms", C.c_double),
                 ("mean_ms", C.c_double), ("gflops", C.c_double), ("jobs_per_s", C.c_double),
                 ("bytes_reserved", C.c_int64), ("peak_bytes_reserved", C.c_int64),
-                ("groups_run", C.c_int64)]
+                ("groups_run", C.c_int64), ("device_bytes", C.c_int64), ("peak_device_bytes", C.c_int64),
+                ("overruns", C.c_int64)]
 
 
 JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "released"}

@@ -28,7 +28,7 @@ appt_tuning tuning_of(const Config& c) {
   for (int k = 0; k < kNumCls; k++) t.class_lds_bytes[k] = c.sym.cls_bytes[k];
   t.sort_max_products = c.sym.sort_max > 0 ? c.sym.sort_max : -1;
   t.rowptr64_min_nnz = c.rowptr64_min_nnz > INT32_MAX ? 0 : (int32_t)c.rowptr64_min_nnz;
-  t.row_templates = c.templates ? 0 : -1;
+  t.row_templates = !c.templates ? -1 : (c.templates_min_nnz > INT32_MAX ? -1 : (int32_t)c.templates_min_nnz);
   return t;
 }
 
@@ -113,7 +113,8 @@ Batch::~Batch() {
   E_.free_workspace(W_);
   if (ccol_) E_.pool().put(ccol_);
   if (cval_) E_.pool().put(cval_);
-  for (void* p : owned_) E_.pool().put(p);
+  for (void* p : owned_)
+    if (p) E_.pool().put(p);
 }
 
 void Batch::drop_graph() {
@@ -216,7 +217,7 @@ void Batch::run_planning() {
   E_.stage_symbolic(W_, O_, cfg_, plan_);
   E_.stage_offsets(W_, O_, cfg_, rp32_);
   E_.read_num(W_, O_, plan_);
-  if (E_.admission()) E_.admission()(result_bytes(N_, plan_.nnz) + 4 * (int64_t)K_);
+  if (E_.admission()) E_.admission()(result_bytes(N_, plan_.nnz) + 4 * (int64_t)K_, true);
   const size_t cap = (size_t)std::max<int64_t>(plan_.nnz, 1);
   if (ccol_) E_.pool().put(ccol_);
   if (cval_) E_.pool().put(cval_);
@@ -322,6 +323,19 @@ void Batch::autotune(int32_t reps, appt_tuning* chosen) {
   cfg_ = best;
   plan(graph);
   if (chosen) *chosen = tuning_of(cfg_);
+}
+
+void Batch::detach_results(void** crp, void** ccol, void** cval) {
+  *crp = crp_;
+  *ccol = ccol_;
+  *cval = cval_;
+  for (auto& p : owned_)
+    if (p == crp_) p = nullptr;
+  ccol_ = nullptr;
+  cval_ = nullptr;
+  crp_ = nullptr;
+  planned_ = false;
+  drop_graph();
 }
 
 void Batch::result(int32_t k, appt_csr_result* out) const {

@@ -34,6 +34,7 @@ namespace appt {
 Pool::~Pool() {
   trim_locked();
   for (auto& kv : size_of_) (void)hipFree(kv.first);
+  if (tally_) tally_->add(-allocated_.load());
 }
 
 void* Pool::get(size_t bytes) {
@@ -65,7 +66,15 @@ void* Pool::get(size_t bytes) {
   }
   size_of_[p] = bytes;
   allocated_ += (int64_t)bytes;
+  if (tally_) tally_->add((int64_t)bytes);
   return p;
+}
+
+void Pool::set_tally(PoolTally* t) {
+  std::lock_guard<std::mutex> lk(mu_);
+  if (tally_) tally_->add(-allocated_.load());
+  tally_ = t;
+  if (tally_) tally_->add(allocated_.load());
 }
 
 void Pool::put(void* p) {
@@ -85,6 +94,7 @@ void Pool::trim_locked() {
   for (auto& kv : free_) {
     (void)hipFree(kv.second);
     allocated_ -= (int64_t)size_of_[kv.second];
+    if (tally_) tally_->add(-(int64_t)size_of_[kv.second]);
     size_of_.erase(kv.second);
   }
   free_.clear();
@@ -153,6 +163,7 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
     }
     if (t->rowptr64_min_nnz > 0) c.rowptr64_min_nnz = t->rowptr64_min_nnz;
     if (t->row_templates < 0) c.templates = false;
+    if (t->row_templates > 0) c.templates_min_nnz = t->row_templates;
   }
   if (group_budget > lds_max) group_budget = lds_max;
   c.sym.lds_budget = c.num.lds_budget = (int32_t)group_budget;
@@ -264,6 +275,26 @@ void Engine::validate(const appt_csr_view& A) {
 template <typename T>
 static T* as(void* p) { return static_cast<T*>(p); }
 
+int64_t Engine::work_bound(int32_t rows, int32_t nb, int64_t nnz) {
+  // per output row: pre-pass, bins, offsets, word list, template ids and
+  // int32 row pointers; per B row: B-row metadata, transpose pointers and
+  // cursors, template classes; per entry: the transposed column and value,
+  // radix keys / rows / values and histograms, heavy-row cursors.  Buffers
+  // grow by an eighth beyond their request; fixed: bin statistics and the
+  // template tables.
+  const int64_t per_row = 700, per_brow = 64, per_entry = 48;
+  const int64_t fixed = (int64_t)24 << 20;
+  return per_row * ((int64_t)rows + 2) + per_brow * ((int64_t)nb + 2) + per_entry * nnz + fixed;
+}
+
+void Engine::trim_all() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  free_workspace(ws_);
+  plan_key_ = RowPlanKey{};
+  pool_.trim();
+}
+
 int64_t result_bytes(int32_t n, int64_t nnz) {
   return (nnz > INT32_MAX ? 8 : 4) * ((int64_t)n + 1) + 12 * nnz;
 }
@@ -365,7 +396,7 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
   launch_fill3_i32(dstats, (int64_t)stats_ints(), rowcnt, n, rowwords, n, s);
   // row templates: not in the static scratch (no-allocation calls) nor where
   // a batch's columns are local to its blocks (cfg.templates off)
-  W.tpl_on = cfg.templates && !arena_ && n > 0;
+  W.tpl_on = cfg.templates && !arena_ && n > 0 && O.annz >= cfg.templates_min_nnz;
   if (W.tpl_on) {
     TplArgs& T = W.tpl;
     const size_t nb = (size_t)std::max(O.nb, 1);
@@ -463,7 +494,9 @@ void Engine::stage_symbolic(Workspace& W, const Operands& O, const Config& cfg, 
   {
     const int64_t cnt = P.sym[kBinWgBitmap * kStatFields], wmax = P.sym[kBinWgBitmap * kStatFields + 2];
     if (!arena_ && cnt > 0 && wmax > 0 && cnt * wmax * 4 <= kWgBitsBudget) {
-      (void)ensure(W.wgbits, sizeof(uint32_t) * (size_t)(cnt * wmax));
+      const size_t need = sizeof(uint32_t) * (size_t)(cnt * wmax);
+      if (admit_ && W.wgbits.cap < need) admit_((int64_t)(need + need / 8 + 256), false);
+      (void)ensure(W.wgbits, need);
       int32_t* slot = as<int32_t>(ensure(W.wgslot, sizeof(int32_t) * (size_t)std::max(O.n, 1)));
       launch_fill_i32(slot, O.n, -1, s);
       W.wg_stride = (int32_t)wmax;
@@ -647,6 +680,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   std::memset(C, 0, sizeof(*C));
   times_ = appt_phase_times{};
   const int32_t n = r1 - r0;
+  if (admit_) admit_(work_bound(n, A.n, A.nnz), false);
   hipStream_t s = stream_;
   Workspace& W = ws_;
   Operands O;
@@ -692,7 +726,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
     P.direct_sym = O.upper && P.a_strict && P.num[kBinWgBitmap * kStatFields] == 0 &&
                    P.num[kBinWgHash * kStatFields] == 0 && P.num[kBinHeavy * kStatFields] == 0;
     record(4);
-    if (admit_) admit_(result_bytes(n, P.nnz));
+    if (admit_) admit_(result_bytes(n, P.nnz), true);
     // output buffers (the symbolic pass sized C exactly)
     ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(P.nnz, 1)));
     cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(P.nnz, 1)));

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <atomic>
 #include <functional>
 #include <map>
 #include <mutex>
@@ -32,6 +33,17 @@ struct Error : std::runtime_error {
 // Device memory pool: exact-capacity buffers kept on a free list and reused
 // when a request fits in [bytes, 2*bytes].  Accounts every byte against the
 // engine budget.  Thread-safe (a server releases results from other threads).
+// Device bytes held by a set of pools (a server's slots), now and at peak.
+struct PoolTally {
+  std::atomic<int64_t> now{0}, peak{0};
+  void add(int64_t b) {
+    const int64_t v = now.fetch_add(b) + b;
+    int64_t p = peak.load();
+    while (v > p && !peak.compare_exchange_weak(p, v)) {
+    }
+  }
+};
+
 class Pool {
  public:
   explicit Pool(int64_t budget) : budget_(budget) {}
@@ -42,14 +54,17 @@ class Pool {
   int64_t allocated() const { return allocated_; }
   int64_t budget() const { return budget_; }
   void set_budget(int64_t b) { budget_ = b; }
+  // count this pool's hipMalloc'd bytes (already held ones included) in t
+  void set_tally(PoolTally* t);
 
  private:
   void trim_locked();
   std::mutex mu_;
   std::multimap<size_t, void*> free_;
   std::map<void*, size_t> size_of_;
-  int64_t allocated_ = 0;
+  std::atomic<int64_t> allocated_{0};
   int64_t budget_;
+  PoolTally* tally_ = nullptr;
 };
 
 // A pooled buffer that grows on demand and is kept across calls.
@@ -66,7 +81,11 @@ struct Config {
   int64_t radix_min_nnz = 1 << 18;  // asymmetric transposes of at least this many entries use the radix sort
   int64_t radix_mirror_min_nnz = 1 << 20;  // AAtS mirrors of results this large use the radix sort
   int64_t rowptr64_min_nnz = (int64_t)INT32_MAX + 1;  // results this large get int64 row pointers
-  bool templates = true;  // row templates (templates.hip) for rows sharing a relative pattern
+  // row templates (templates.hip) for rows sharing a relative pattern, looked
+  // for when A has at least this many entries (small calls would only pay
+  // for the detection)
+  bool templates = true;
+  int64_t templates_min_nnz = (int64_t)1 << 18;
   bool rowptr64(int64_t nnz) const { return nnz > INT32_MAX || nnz >= rowptr64_min_nnz; }
 };
 
@@ -198,11 +217,22 @@ class Engine {
   // compute_noalloc, carved from the static scratch array.
   void* ensure(Scratch& s, size_t bytes);
   void free_workspace(Workspace& W);
-  // Called with the exact bytes of C once the symbolic phase has sized it,
-  // before C is allocated; may block, or throw to refuse the call.
-  using Admission = std::function<void(int64_t)>;
+  // Called with device bytes a call is about to allocate, before it does:
+  // its working set (result = false: before the operands are staged, bounded
+  // by work_bound; again before the symbolic pass keeps workgroup bitmaps),
+  // then C itself (result = true, the exact bytes once the symbolic phase has
+  // sized it).  May block, or throw to refuse the call.
+  using Admission = std::function<void(int64_t bytes, bool result)>;
   void set_admission(Admission f) { admit_ = std::move(f); }
   const Admission& admission() const { return admit_; }
+  // Upper bound of the device memory one call on `rows` output rows of an
+  // operand with nb rows and nnz entries may newly take from the pool, C
+  // excluded (the workgroup bitmaps are admitted separately): every scratch
+  // buffer at its growth size, the transpose and its radix scratch, the row
+  // templates' tables, the int32 row pointers.
+  static int64_t work_bound(int32_t rows, int32_t nb, int64_t nnz);
+  // Return the workspace and the pool's free blocks to HIP.
+  void trim_all();
 
   // ---- pipeline stages: kernel launches only.  No host synchronisation and,
   // once the workspace has grown to the problem, no allocation, so a replay
@@ -292,6 +322,9 @@ class Batch {
   void plan(bool use_graph);
   void execute();
   void result(int32_t k, appt_csr_result* out) const;
+  // Hand the C buffers (row pointers, columns, values) to the caller, who
+  // returns them to the engine's pool; the batch must be planned again.
+  void detach_results(void** crp, void** ccol, void** cval);
   void autotune(int32_t reps, appt_tuning* chosen);
   void info(appt_batch_info* out) const;
   int64_t products(int32_t k) const { return blkprod_h_.at((size_t)k); }

@@ -450,8 +450,10 @@ def _tuning_struct(tuning: dict | None) -> Tuning:
 
 
 class Server:
-    """Request loop over device-resident jobs (``appt_server_*``): memory
-    admission by exact nnz(C) after the symbolic phase, several jobs in flight
+    """Request loop over device-resident jobs (``appt_server_*``): every byte
+    the slots' pools hold is charged against the HBM budget; a job is admitted
+    for a bound of its working set before it stages its operands and for its
+    exact C once the symbolic phase has fixed nnz(C); several jobs in flight
     on their own streams, small waiting jobs run together as one batch."""
 
     def __init__(self, device: int = 0, slots: int = 4, hbm_budget_bytes: int = 200 * 10 ** 9,

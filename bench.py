@@ -103,7 +103,8 @@ def cpu_baseline(m, op: str, target_s: float) -> dict:
     """Oracle (CPU Gustavson port, OpenMP over rows) on all host cores: the
     full product, repeated until about target_s seconds of CPU work."""
     import oracle
-    threads = os.cpu_count() or 1
+    host = host_cpu()
+    threads = host["usable_cores"]
     b = None
     if op != "AA":
         b = oracle.transpose(m.n, m.rowptr, m.col, m.val)
@@ -126,20 +127,52 @@ def cpu_baseline(m, op: str, target_s: float) -> dict:
     d1 = max(time.perf_counter() - t1, 1e-6)
     return {"value": round(2.0 * prods * reps / dt / 1e9, 4), "unit": "GFLOP/s", "cores": threads,
             "kind": "port", "ms_per_call": round(dt * 1e3 / reps, 3),
-            "value_1thread": round(2.0 * p1 / d1 / 1e9, 4), "cpu_model": cpu_model(),
+            "value_1thread": round(2.0 * p1 / d1 / 1e9, 4), "cpu_model": host.get("model", "unknown"),
+            "lscpu": host,
             "sample": f"the whole {op} product ({m.n} rows, {prods} products) x {reps} calls, {dt:.2f} s, "
                       f"oracle/oracle.c Gustavson (symbolic + numeric passes), OpenMP dynamic rows; "
                       f"1 thread: rows [0, {r1}) ({p1} products), {d1:.2f} s"}
 
 
 def cpu_model() -> str:
+    """CPU model name as lscpu reports it (/proc/cpuinfo if lscpu is absent)."""
+    info = host_cpu()
+    return info.get("model", "unknown")
+
+
+def host_cpu() -> dict:
+    """lscpu's model name, CPU and socket counts, and the cores this process
+    may run on (its affinity mask: a container's share of the host)."""
+    import subprocess
+    out = {}
     try:
-        for line in open("/proc/cpuinfo"):
-            if line.startswith("model name"):
-                return line.split(":", 1)[1].strip()
-    except OSError:
+        r = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10)
+        for line in r.stdout.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k == "Model name":
+                out["model"] = v
+            elif k == "CPU(s)":
+                out["lscpu_cpus"] = int(v)
+            elif k == "Socket(s)":
+                out["sockets"] = int(v)
+            elif k == "Core(s) per socket":
+                out["cores_per_socket"] = int(v)
+    except (OSError, ValueError, subprocess.TimeoutExpired):
         pass
-    return "unknown"
+    if "model" not in out:
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    out["model"] = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
+    try:
+        out["usable_cores"] = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        out["usable_cores"] = os.cpu_count() or 1
+    return out
 
 
 def copy_bandwidth(mib: int = 1024, reps: int = 5) -> float:
@@ -233,7 +266,7 @@ def bench_batch(args) -> dict:
     cpu = None
     if not args.no_cpu:
         import oracle
-        threads = os.cpu_count() or 1
+        threads = host_cpu()["usable_cores"]
         sample = list(range(0, count, max(1, count // 256)))
         sp = sum(products_of(mats[k], ops[k]) for k in sample)
         t0 = time.perf_counter()
@@ -259,6 +292,32 @@ def bench_batch(args) -> dict:
                      "traffic": None},
         "cpu_baseline": cpu,
     }
+
+
+def extras(args):
+    """The batched and served forms beside the headline: the 4096-matrix batch
+    (plain per-matrix calls against the graph-replayed batch) and the served
+    stream at two fixed offered rates, 1000 jobs each."""
+    import copy
+    a = copy.copy(args)
+    a.steps, a.warmup, a.no_cpu = 10, 2, True
+    t0 = time.perf_counter()
+    b = bench_batch(a)
+    bc = b["config"]
+    batched = {"matrices": args.batch_count, "batched_GFLOPs": b["value"], "batched_ms_per_pass": b["ms_per_step"],
+               "plain_GFLOPs": bc["plain_GFLOPs"], "plain_ms_per_pass": bc["plain_ms_per_pass"],
+               "batched_over_plain": bc["batched_over_plain"], "products": bc["products"], "nnz_c": bc["nnz_c"],
+               "roofline_frac": b["roofline"]["frac"], "seconds": round(time.perf_counter() - t0, 1)}
+    print(f"# batched: {json.dumps(batched)}", file=sys.stderr, flush=True)
+    s = copy.copy(args)
+    s.serve_jobs, s.serve_qps = 1000, "10,20"
+    t0 = time.perf_counter()
+    r = bench_serve(s)
+    served = {"slots": r["config"]["slots"], "hbm_budget_GB": r["config"]["hbm_budget_GB"],
+              "pool": r["config"]["workload"], "rates": r["config"]["rates"],
+              "seconds": round(time.perf_counter() - t0, 1)}
+    print(f"# served: {json.dumps(served)}", file=sys.stderr, flush=True)
+    return batched, served
 
 
 def bench_panels(args) -> dict:
@@ -347,41 +406,68 @@ def bench_serve(args) -> dict:
     # so the burst and the Poisson runs serve the same mix
     picks = rng.integers(0, pool_n, args.serve_jobs)
     gaps = rng.exponential(1.0, args.serve_jobs)
+    ops = [jobs[k][1] for k in picks]
 
     def run(qps: float | None, count: int) -> dict:
+        import queue
+        import threading
         with ap.Server(0, slots=args.slots, hbm_budget_bytes=200 * 10 ** 9) as srv:
-            ids = []
+            # a client releases each result as soon as its job is done, so
+            # results do not pile up while later jobs arrive
+            done_q: queue.Queue = queue.Queue()
+
+            def releaser():
+                n_done = 0
+                while True:
+                    j = done_q.get()
+                    if j is None:
+                        return
+                    srv.wait(j)
+                    srv.release(j)
+                    n_done += 1
+                    if n_done % 250 == 0:
+                        print(f"# serve {qps or 'burst'}: {n_done}/{count} jobs done", file=sys.stderr, flush=True)
+
+            th = threading.Thread(target=releaser)
+            th.start()
             t0 = time.perf_counter()
             t_next = t0
-            for n_k, k in enumerate(picks[:count]):
-                if qps:
-                    t_next += gaps[n_k] / qps
-                    dt = t_next - time.perf_counter()
-                    if dt > 0:
-                        time.sleep(dt)
-                ids.append(srv.submit(dev[k], jobs[k][1]))
-            for j in ids:
-                srv.wait(j)
-                srv.release(j)
+            try:
+                for n_k, k in enumerate(picks[:count]):
+                    if qps:
+                        t_next += gaps[n_k] / qps
+                        dt = t_next - time.perf_counter()
+                        if dt > 0:
+                            time.sleep(dt)
+                    done_q.put(srv.submit(dev[k], ops[n_k]))
+            finally:
+                done_q.put(None)
+                th.join()
             st = srv.stats()
         return {"offered_qps": qps, "jobs": count, "p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
                 "mean_ms": round(st["mean_ms"], 3), "jobs_per_s": round(st["jobs_per_s"], 3),
                 "GFLOPs": round(st["gflops"], 3), "refused": st["refused"], "failed": st["failed"],
-                "groups_run": st["groups_run"], "peak_reserved_GB": round(st["peak_bytes_reserved"] / 1e9, 3)}
+                "groups_run": st["groups_run"], "peak_reserved_GB": round(st["peak_bytes_reserved"] / 1e9, 3),
+                "peak_device_GB": round(st["peak_device_bytes"] / 1e9, 3)}
 
-    sat = run(None, args.serve_jobs)
-    rates = [run(f * sat["jobs_per_s"], args.serve_jobs) for f in (0.5, 0.8)]
+    if args.serve_qps:
+        # fixed offered rates (jobs/s)
+        sat = None
+        rates = [run(float(q), args.serve_jobs) for q in args.serve_qps.split(",")]
+    else:
+        sat = run(None, args.serve_jobs)
+        rates = [run(f * sat["jobs_per_s"], args.serve_jobs) for f in (0.5, 0.8)]
     for d in dev:
         d.free()
     eng.close()
     return {
         "metric": "served SpGEMM p50/p99 latency at fixed offered QPS, with throughput",
-        "value": rates[1]["p99_ms"], "unit": "ms (p99 at 80 % of saturation QPS)", "higher_is_better": False,
+        "value": rates[-1]["p99_ms"], "unit": f"ms (p99 at {rates[-1]['offered_qps']:.1f} jobs/s offered)",
+        "higher_is_better": False,
         "dtype": "f64",
         "config": {"workload": f"pool of {pool_n} resident jobs, Laplacian / 27-pt stencil / R-MAT, n log-uniform "
                                f"[1e3, {args.serve_nmax:g}], A*A or A*A^T, seed {args.seed}",
-                   "slots": args.slots, "hbm_budget_GB": 200, "saturation": sat, "poisson_50pct": rates[0],
-                   "poisson_80pct": rates[1]},
+                   "slots": args.slots, "hbm_budget_GB": 200, "saturation": sat, "rates": rates},
         "roofline": None,
         "cpu_baseline": None,
     }
@@ -404,6 +490,11 @@ def main() -> int:
     ap_.add_argument("--serve-pool", type=int, default=24)
     ap_.add_argument("--serve-jobs", type=int, default=200)
     ap_.add_argument("--serve-nmax", type=float, default=2e6)
+    ap_.add_argument("--serve-qps", default="", help="fixed offered rates, e.g. 50,100 (default: 50 %% and 80 %% "
+                                                    "of the measured saturation)")
+    ap_.add_argument("--extras", type=int, default=1,
+                     help="default workload: also measure the batched (4096 matrices) and served (2 fixed rates, "
+                          "1000 jobs each) forms and report them under config")
     ap_.add_argument("--copy-probe", type=int, default=0, help="MiB; run only a device copy (counter calibration)")
     ap_.add_argument("--rocsparse", type=int, default=1, help="also time rocSPARSE on the same matrix (context)")
     ap_.add_argument("--panel-gb", type=float, default=60.0, help="C bytes per row panel (rmatN_panels)")
@@ -478,6 +569,11 @@ def main() -> int:
         roc["nnz_c_matches"] = roc["nnz_c"] == nnz_c
     cpu = None if args.no_cpu else cpu_baseline(m, args.op, args.cpu_seconds)
     copy_gbs = copy_bandwidth()
+    batched = served = None
+    n_rows, nnz_a = m.n, m.nnz
+    if args.extras and args.workload == "stencil27_96":
+        del m
+        batched, served = extras(args)
     out = {
         "metric": "SpGEMM GFLOP/s (FP64, structural C=A*A) with HBM GB/s vs roofline",
         "value": round(gflops, 3),
@@ -491,11 +587,11 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": desc, "op": args.op, "n": m.n, "nnz_a": m.nnz, "nnz_c": nnz_c,
+        "config": {"workload": desc, "op": args.op, "n": n_rows, "nnz_a": nnz_a, "nnz_c": nnz_c,
                    "median_ms_per_step": round(float(np.median(step_ms)), 4),
                    "products": products, "flops": 2 * products, "bytes_alg": b_alg,
                    "call_GBps": round(call_gbs, 2), "call_roofline_frac": round(call_gbs / HBM_PEAK_GBS, 4),
-                   "phase_ms": phases, "rocsparse": roc},
+                   "phase_ms": phases, "rocsparse": roc, "batched": batched, "served": served},
         "roofline": {"bound": "hbm", "kernel": dom.replace("_ms", ""), "achieved": round(achieved, 2),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": pmc_traffic(args.workload, args.op) if dom == "numeric_ms" else None,

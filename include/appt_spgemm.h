@@ -104,8 +104,9 @@ typedef struct appt_tuning {
   int32_t rowptr64_min_nnz;   /* results with at least this many entries get int64 row pointers
                                  (0 = only when nnz(C) >= 2^31, where int32 cannot hold them) */
   int32_t row_templates;      /* rows whose own relative pattern and whose B rows' patterns repeat
-                                 share a template: no symbolic pass, products straight to their slots
-                                 (0 = on, < 0 = off) */
+                                 share a template: no symbolic pass, products straight to their slots.
+                                 Detected for operands of at least this many entries (0 = 262,144;
+                                 < 0 = never) */
   int32_t reserved[1];
 } appt_tuning;
 
@@ -339,8 +340,13 @@ typedef struct appt_server_stats {
   double p50_ms, p99_ms, mean_ms;  /* latency of completed jobs */
   double gflops;                   /* 2 * sum(products) of completed jobs / (last done - first submit) */
   double jobs_per_s;
+  /* budget bytes charged now and at peak: every byte the slots' pools hold
+   * (cached working sets, free blocks, live results) plus what running jobs
+   * were admitted for and have not allocated yet */
   int64_t bytes_reserved, peak_bytes_reserved;
   int64_t groups_run;              /* launches that ran more than one job */
+  int64_t device_bytes, peak_device_bytes;  /* device memory the slots' pools hold, now and at peak */
+  int64_t overruns;                /* runs that allocated more than they were admitted for (0 expected) */
 } appt_server_stats;
 
 appt_status appt_server_create(const appt_server_options* opt, appt_server** out);

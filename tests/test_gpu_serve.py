@@ -1,7 +1,8 @@
 """Serving: jobs submitted while others run, small ones grouped into one
-batch, admission by exact result bytes against the HBM budget.  Each result
-must be the same as the matrix computed alone (structure exact, values within
-1e-12 * sum|a*b|)."""
+batch, admission against the HBM budget of every byte the slots hold (working
+set bound before staging, exact C after the symbolic phase).  Each result must
+be the same as the matrix computed alone (structure exact, values within
+1e-12 * sum|a*b|), and the slots' device memory must stay within the budget."""
 import sys
 import threading
 import time
@@ -78,7 +79,9 @@ def test_serve_mix_equals_alone(eng):
             assert st["completed"] == len(mats) and st["refused"] == 0 and st["failed"] == 0
             assert st["groups_run"] >= 1
             assert st["p99_ms"] >= st["p50_ms"] > 0
-            assert st["bytes_reserved"] == 0
+            # every job released: what stays charged is what the slots' pools cache
+            assert st["bytes_reserved"] == st["device_bytes"]
+            assert st["overruns"] == 0
     finally:
         for d in dev:
             d.free()
@@ -105,16 +108,28 @@ def test_serve_admission_refuses_oversized(eng):
         ds.free()
 
 
+def admitted_bytes(dev, op, slots):
+    """Budget bytes one job is admitted for on a fresh server of `slots`
+    slots (its working-set bound, bitmaps and C), and its result bytes."""
+    with ap.Server(0, slots=slots, batch_max_jobs=1) as srv:
+        j = srv.submit(dev, op)
+        info = srv.wait(j)
+        assert info["status_name"] == "APPT_OK", info
+        st = srv.stats()
+        assert st["overruns"] == 0 and st["peak_device_bytes"] <= st["peak_bytes_reserved"]
+        srv.release(j)
+        return st["peak_bytes_reserved"], info["reserved_bytes"]
+
+
 def test_serve_waits_for_memory(eng):
     m = ap.stencil27(30, 0.0, 4)        # C ~ 27000 x 125 entries ~ 40 MB
     d = eng.upload(m)
     try:
-        need = 4 * (m.n + 1) + 12 * int(ref(m, "AA")[0][-1])
-        work = 64 * m.n + 28 * m.nnz + (8 << 20)   # the server's working-set estimate
-        # room for one job (result + working set) beside nothing, not beside a held result
-        with ap.Server(0, slots=2, hbm_budget_bytes=need + work + need // 2, batch_max_jobs=1,
+        need, cbytes = admitted_bytes(d, "AA", 2)
+        # room for one job, not for a second one beside a held result
+        with ap.Server(0, slots=2, hbm_budget_bytes=need + cbytes // 2, batch_max_jobs=1,
                        admission_wait_ms=20000) as srv:
-            say("case serve memory wait")
+            say("case serve memory wait, budget", need + cbytes // 2)
             j1 = srv.submit(d, "AA")
             assert srv.wait(j1)["status_name"] == "APPT_OK"
             j2 = srv.submit(d, "AA")
@@ -127,5 +142,53 @@ def test_serve_waits_for_memory(eng):
             assert info["status_name"] == "APPT_OK" and info["latency_ms"] >= 300
             assert_product(srv.download(j2), ref(m, "AA"), exact=True)
             srv.release(j2)
+            st = srv.stats()
+            assert st["peak_device_bytes"] <= need + cbytes // 2 and st["overruns"] == 0
     finally:
         d.free()
+
+
+def test_serve_tight_budget_four_slots(eng):
+    # four slots under a budget of twice the largest single job's admission:
+    # jobs (and groups) wait for one another's memory, none is refused, every
+    # result equals the job alone, and the pools never hold more than the budget
+    mats, ops = job_mix(7)
+    dev = [eng.upload(m) for m in mats]
+    try:
+        alone = []
+        for d, op in zip(dev, ops):
+            r = eng.compute(d, op)
+            try:
+                alone.append(r.download())
+            finally:
+                r.release()
+        budget = 2 * max(admitted_bytes(d, op, 1)[0] for d, op in zip(dev, ops))
+        with ap.Server(0, slots=4, hbm_budget_bytes=budget, batch_row_limit=20000,
+                       admission_wait_ms=60000) as srv:
+            say("case serve tight budget: 40 jobs, 4 slots, budget", budget)
+            ids = [srv.submit(d, op) for d, op in zip(dev, ops)]
+            pending = dict(enumerate(ids))
+            deadline = time.time() + 120
+            while pending and time.time() < deadline:
+                for k, j in list(pending.items()):
+                    info = srv.poll(j)
+                    if info["state"] != "done":
+                        continue
+                    assert info["status_name"] == "APPT_OK", (k, info)
+                    got = srv.download(j)
+                    np.testing.assert_array_equal(got.rowptr, alone[k].rowptr)
+                    np.testing.assert_array_equal(got.col, alone[k].col)
+                    assert np.all(np.abs(got.val - alone[k].val) <= TOL * ref(mats[k], ops[k])[3])
+                    srv.release(j)
+                    del pending[k]
+                time.sleep(0.001)
+            assert not pending, "jobs still outstanding"
+            st = srv.stats()
+            say("stats", st)
+            assert st["completed"] == len(mats) and st["refused"] == 0 and st["failed"] == 0
+            assert st["overruns"] == 0
+            assert st["peak_device_bytes"] <= budget
+            assert st["peak_bytes_reserved"] <= budget
+    finally:
+        for d in dev:
+            d.free()

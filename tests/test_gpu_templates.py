@@ -20,7 +20,8 @@ def say(*a):
 
 @pytest.fixture(scope="module")
 def engs():
-    on, off = ap.Engine(0), ap.Engine(0, tuning={"row_templates": -1})
+    # templates looked for at any size (the default skips operands under 2^18 entries)
+    on, off = ap.Engine(0, tuning={"row_templates": 1}), ap.Engine(0, tuning={"row_templates": -1})
     yield on, off
     on.close()
     off.close()
