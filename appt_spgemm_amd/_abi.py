@@ -47,7 +47,7 @@ class Tuning(C.Structure):
                 ("sort_span_factor", C.c_int32), ("sort_span_factor_symbolic", C.c_int32),
                 ("transpose_radix_min_nnz", C.c_int32), ("mirror_radix_min_nnz", C.c_int32),
                 ("class_lds_bytes", C.c_int32 * MAX_CLASSES), ("sort_max_products", C.c_int32),
-                ("rowptr64_min_nnz", C.c_int32), ("row_templates", C.c_int32), ("reserved", C.c_int32 * 1)]
+                ("rowptr64_min_nnz", C.c_int32), ("row_templates", C.c_int32), ("gram", C.c_int32)]
 
 
 class Options(C.Structure):
@@ -76,7 +76,7 @@ class Descriptors(C.Structure):
                 ("deg_cv", C.c_double), ("deg_max", C.c_int32), ("fill_aa", C.c_double),
                 ("fill_aat", C.c_double), ("psym", C.c_double), ("products_aa", C.c_int64),
                 ("products_aat", C.c_int64), ("max_row_products_aa", C.c_int32),
-                ("bandwidth", C.c_int32), ("pick_ours", C.c_int32), ("plan_bins", C.c_int32)]
+                ("bandwidth", C.c_int32), ("pick_ours", C.c_int32)]
 
 
 class HostCsr(C.Structure):

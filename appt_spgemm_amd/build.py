@@ -36,7 +36,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__",
              f"-I{ROCM / 'include'}"]
 
-HIP_SOURCES = ["kernels.hip", "templates.hip", "descriptors.hip", "batch.hip", "noalloc.hip"]
+HIP_SOURCES = ["kernels.hip", "templates.hip", "descriptors.hip", "batch.hip", "noalloc.hip", "gram.hip"]
 CXX_SOURCES = ["engine.cpp", "batch.cpp", "server.cpp", "capi.cpp", "gen.cpp"]
 
 

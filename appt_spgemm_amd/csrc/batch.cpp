@@ -29,6 +29,7 @@ appt_tuning tuning_of(const Config& c) {
   t.sort_max_products = c.sym.sort_max > 0 ? c.sym.sort_max : -1;
   t.rowptr64_min_nnz = c.rowptr64_min_nnz > INT32_MAX ? 0 : (int32_t)c.rowptr64_min_nnz;
   t.row_templates = !c.templates ? -1 : (c.templates_min_nnz > INT32_MAX ? -1 : (int32_t)c.templates_min_nnz);
+  t.gram = c.gram;
   return t;
 }
 

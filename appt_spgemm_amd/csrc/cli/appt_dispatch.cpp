@@ -36,9 +36,11 @@ int main(int argc, char** argv) {
   appt_eng_result r{};
   CLI_CHECK(appt_engine_run_timed(eng, task, &A, reps, &r));
   const double fill = op == APPT_OP_AA ? d.fill_aa : d.fill_aat;
+  appt_phase_times pt{};
+  CLI_CHECK(appt_engine_phase_times(eng, &pt));
   std::printf("DISPATCH,%s,%s,ours,dcv=%.3f,fill=%.1f,C_nnz=%lld,desc_ms=%.4f,run_ms=%.4f,total_ms=%.4f,rule=%s,bins=%d\n",
               tag, task, d.deg_cv, fill, (long long)r.nnz, desc_ms, r.ms, desc_ms + r.ms,
-              d.pick_ours ? "ours" : "cusparse", d.plan_bins);
+              d.pick_ours ? "ours" : "cusparse", pt.n_bins_used);
   appt_device_free_view(eng, &A);
   appt_engine_destroy(eng);
   appt_host_csr_free(&h);

@@ -164,6 +164,7 @@ Config make_config(const appt_tuning* t, int64_t lds_max) {
     if (t->rowptr64_min_nnz > 0) c.rowptr64_min_nnz = t->rowptr64_min_nnz;
     if (t->row_templates < 0) c.templates = false;
     if (t->row_templates > 0) c.templates_min_nnz = t->row_templates;
+    c.gram = t->gram;
   }
   if (group_budget > lds_max) group_budget = lds_max;
   c.sym.lds_budget = c.num.lds_budget = (int32_t)group_budget;
@@ -396,7 +397,8 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
   launch_fill3_i32(dstats, (int64_t)stats_ints(), rowcnt, n, rowwords, n, s);
   // row templates: not in the static scratch (no-allocation calls) nor where
   // a batch's columns are local to its blocks (cfg.templates off)
-  W.tpl_on = cfg.templates && !arena_ && n > 0 && O.annz >= cfg.templates_min_nnz;
+  // nor for a Gram, whose B rows are per-entry suffixes, not matrix rows
+  W.tpl_on = cfg.templates && !arena_ && !O.gram && n > 0 && O.annz >= cfg.templates_min_nnz;
   if (W.tpl_on) {
     TplArgs& T = W.tpl;
     const size_t nb = (size_t)std::max(O.nb, 1);
@@ -592,6 +594,83 @@ void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, c
   }
 }
 
+int Engine::gram_mode(const appt_csr_view& A) const {
+  // the static scratch holds no per-entry descriptors or triangle.  By
+  // default AAtS takes full rows: the Gram halves the products, but the
+  // transposition of the mirrored half (a radix sort of nnz(C)/2 entries)
+  // costs more than they do on every measured input, banded to power-law
+  if (arena_ || A.n == 0 || cfg_.gram <= 0) return kGramNone;
+  return cfg_.gram == 1 ? kGramUpper : kGramLower;
+}
+
+void Engine::stage_gram_operand(Workspace& W, Operands& O, bool lower) {
+  hipStream_t s = stream_;
+  const int64_t na = std::max<int64_t>(O.annz, 1);
+  int4* vmeta = as<int4>(ensure(W.vmeta, sizeof(int4) * (size_t)na));
+  int32_t* vac = as<int32_t>(ensure(W.vac, sizeof(int32_t) * (size_t)na));
+  auto* full = reinterpret_cast<unsigned long long*>(as<int32_t>(W.ctl.p) + kCtlGramProducts);
+  launch_gram_operand(O.n, O.ap, O.ac, O.nb, as<const int4>(W.bmeta.p), O.bc, lower ? 1 : 0, vmeta, vac, full, s);
+  // the per-entry descriptors take the B-row metadata's place
+  std::swap(W.bmeta, W.vmeta);
+  O.ac = vac;
+  O.nb = (int32_t)O.annz;
+  O.upper = 0;  // every product formed is one of the triangle
+  O.a_is_b = false;
+  O.gram = true;
+  O.gram_lower = lower;
+}
+
+void Engine::stage_gram_expand(Workspace& W, const Operands& O, const Plan& P, const int32_t* ucol,
+                               const double* uval, int64_t nc, int32_t* rp32, int32_t* ccol, double* cval) {
+  hipStream_t s = stream_;
+  const int32_t n = O.n;
+  if (n <= 0) return;
+  const size_t nn = (size_t)n;
+  const int64_t nu = P.nnz;
+  // strictly-upper entries: U's less each non-empty row's diagonal
+  const int64_t ns = (nc - ((int64_t)n - P.num[0])) / 2;
+  const int64_t* uoff = as<int64_t>(W.rowoff.p);
+  const int32_t* ucnt = as<int32_t>(W.rowcnt.p);
+  int32_t* lower = as<int32_t>(ensure(W.mlow, sizeof(int32_t) * nn));
+  int32_t* strict = as<int32_t>(ensure(W.mup, sizeof(int32_t) * nn));
+  int32_t* total = as<int32_t>(ensure(W.gtot, sizeof(int32_t) * nn));
+  int64_t* sup_off = as<int64_t>(ensure(W.mup_off, sizeof(int64_t) * (nn + 1)));
+  int64_t* lo_off = as<int64_t>(ensure(W.mlo_off, sizeof(int64_t) * (nn + 1)));
+  int64_t* coff = as<int64_t>(ensure(W.goff, sizeof(int64_t) * (nn + 1)));
+  int64_t* tmp = as<int64_t>(ensure(W.scan_tmp, sizeof(int64_t) * (size_t)scan_tmp_elems(n)));
+  // radix buffers sized by U (an upper bound of the strict entries)
+  const size_t nz = (size_t)std::max<int64_t>(nu, 1);
+  RadixScratch rx{};
+  const int64_t hn = ((int64_t)1 << radix_bits(n)) * radix_tiles((int64_t)nz);
+  rx.keys = as<int32_t>(ensure(W.rkeys, sizeof(int32_t) * nz));
+  rx.rows = as<int32_t>(ensure(W.rrows, sizeof(int32_t) * nz));
+  rx.vals = as<double>(ensure(W.rvals, sizeof(double) * nz));
+  rx.keys2 = as<int32_t>(ensure(W.rkeys2, sizeof(int32_t) * nz));
+  rx.hist = as<int32_t>(ensure(W.rhist, sizeof(int32_t) * (size_t)std::max<int64_t>(hn, 1)));
+  rx.hoff = as<int64_t>(ensure(W.rhoff, sizeof(int64_t) * (size_t)(hn + 1)));
+  rx.htmp = as<int64_t>(ensure(W.rtmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max<int64_t>(hn, 1))));
+  int32_t* rows2 = as<int32_t>(ensure(W.rrows2, sizeof(int32_t) * nz));
+  double* vals2 = as<double>(ensure(W.rvals2, sizeof(double) * nz));
+
+  // row j of C: lower[j] mirrored entries, then U row j
+  launch_fill_i32(lower, n, 0, s);
+  launch_gram_mirrored(n, nu, uoff, ucol, lower, s);
+  launch_gram_counts(n, ucnt, lower, strict, total, s);
+  launch_scan(strict, n, sup_off, nullptr, tmp, s);
+  launch_scan(lower, n, lo_off, nullptr, tmp, s);
+  launch_scan(total, n, coff, nc <= INT32_MAX ? rp32 : nullptr, tmp, s);
+  // the strictly-upper entries in row-major order, stably sorted by column:
+  // column j's entries come out with their rows ascending, row j's lower part
+  const bool odd = radix_passes(n) % 2 == 1;
+  const int lt = O.gram_lower ? 1 : 0;
+  launch_gram_extract(n, nu, uoff, ucol, uval, sup_off, lt, odd ? rx.keys : rx.keys2, odd ? rx.rows : rows2,
+                      odd ? rx.vals : vals2, s);
+  launch_radix_pairs(n, ns, rx, rows2, vals2, s);
+  launch_gram_assemble(n, nc, coff, lt, lower, lo_off, rows2, vals2, uoff, ucol, uval, ccol, cval, s);
+  // the last offset is nnz(C): checked against nc once the call has synchronised
+  APPT_HIP(hipMemcpyAsync(&hs_->total, coff + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+}
+
 void Engine::stage_mirror(Workspace& W, const Operands& O, const Plan& P, const int32_t* ccol, double* cval) {
   // AAtS: lower entries take the value of their upper mirror image
   if (!O.upper || P.direct_sym) return;
@@ -641,6 +720,8 @@ void Engine::symbolic_rowptr(const appt_csr_view& A, appt_op op, int64_t* rowptr
   validate(A);
   if (!rowptr_host) throw Error(APPT_ERR_INVALID, "rowptr pointer is NULL");
   if (op != APPT_OP_AA && op != APPT_OP_AAT && op != APPT_OP_AATS) throw Error(APPT_ERR_INVALID, "bad op");
+  // the symmetric A*A^T has A*A^T's rows (its panels run as A*A^T)
+  if (op == APPT_OP_AATS) op = APPT_OP_AAT;
   APPT_HIP(hipSetDevice(device_));
   Workspace& W = ws_;
   Operands O;
@@ -674,8 +755,9 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   if (!C) throw Error(APPT_ERR_INVALID, "result pointer is NULL");
   if (op != APPT_OP_AA && op != APPT_OP_AAT && op != APPT_OP_AATS) throw Error(APPT_ERR_INVALID, "bad op");
   if (r0 < 0 || r1 < r0 || r1 > A.n) throw Error(APPT_ERR_INVALID, "row range outside [0, n]");
-  if (op == APPT_OP_AATS && (r0 != 0 || r1 != A.n))
-    throw Error(APPT_ERR_UNSUPPORTED, "AAtS mirrors across all rows; row panels take AA or AAt");
+  // a row panel of the symmetric A*A^T holds complete rows, which A*A^T's
+  // own rows are: no upper triangle to mirror from
+  if (op == APPT_OP_AATS && (r0 != 0 || r1 != A.n)) op = APPT_OP_AAT;
   APPT_HIP(hipSetDevice(device_));
   std::memset(C, 0, sizeof(*C));
   times_ = appt_phase_times{};
@@ -691,9 +773,16 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   O.ap = A.rowptr + r0;
   O.n = n;
   O.row0 = r0;
+  const int gmode = op == APPT_OP_AATS ? gram_mode(A) : kGramNone;
+  if (gmode == kGramUpper || gmode == kGramLower) stage_gram_operand(W, O, gmode == kGramLower);
   record(1);
   stage_prepass(W, O, cfg_);
-  read_sym(W, P, O.upper != 0);
+  read_sym(W, P, O.upper != 0 || O.gram);
+  if (O.gram) {
+    unsigned long long full;
+    std::memcpy(&full, hs_->ctl + kCtlGramProducts, sizeof(full));
+    P.gram_products = (int64_t)full;
+  }
   record(2);
   {
     // heavy rows reuse the row-pointer plan of this view and op, if any
@@ -715,6 +804,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
   int32_t* rp32 = as<int32_t>(pool_.get(sizeof(int32_t) * ((size_t)n + 1)));
   int32_t* ccol = nullptr;
   double* cval = nullptr;
+  int64_t cnnz = 0;
   void* rowptr_out = rp32;
   try {
     stage_offsets(W, O, cfg_, rp32);
@@ -726,17 +816,28 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
     P.direct_sym = O.upper && P.a_strict && P.num[kBinWgBitmap * kStatFields] == 0 &&
                    P.num[kBinWgHash * kStatFields] == 0 && P.num[kBinHeavy * kStatFields] == 0;
     record(4);
-    if (admit_) admit_(result_bytes(n, P.nnz), true);
+    // a Gram's C: U's off-diagonal entries twice, each non-empty row's
+    // diagonal once
+    cnnz = O.gram ? 2 * P.nnz - ((int64_t)n - P.num[0]) : P.nnz;
+    if (admit_) admit_(result_bytes(n, cnnz), true);
     // output buffers (the symbolic pass sized C exactly)
-    ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(P.nnz, 1)));
-    cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(P.nnz, 1)));
-    stage_numeric(W, O, cfg_, P, ccol, cval);
-    record(5);
-    stage_mirror(W, O, P, ccol, cval);
-    if (cfg_.rowptr64(P.nnz)) {
+    ccol = as<int32_t>(pool_.get(sizeof(int32_t) * (size_t)std::max<int64_t>(cnnz, 1)));
+    cval = as<double>(pool_.get(sizeof(double) * (size_t)std::max<int64_t>(cnnz, 1)));
+    if (O.gram) {
+      int32_t* ucol = as<int32_t>(ensure(W.ucol, sizeof(int32_t) * (size_t)std::max<int64_t>(P.nnz, 1)));
+      double* uval = as<double>(ensure(W.uval, sizeof(double) * (size_t)std::max<int64_t>(P.nnz, 1)));
+      stage_numeric(W, O, cfg_, P, ucol, uval);
+      record(5);
+      stage_gram_expand(W, O, P, ucol, uval, cnnz, rp32, ccol, cval);
+    } else {
+      stage_numeric(W, O, cfg_, P, ccol, cval);
+      record(5);
+      stage_mirror(W, O, P, ccol, cval);
+    }
+    if (cfg_.rowptr64(cnnz)) {
       int64_t* rp64 = as<int64_t>(pool_.get(sizeof(int64_t) * ((size_t)n + 1)));
       rowptr_out = rp64;
-      launch_copy_i64(as<int64_t>(W.rowoff.p), (int64_t)n + 1, rp64, s);
+      launch_copy_i64(as<int64_t>(O.gram ? W.goff.p : W.rowoff.p), (int64_t)n + 1, rp64, s);
     }
     record(6);
     APPT_HIP(hipGetLastError());
@@ -745,6 +846,9 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
     wait_stream();
     if (heavy && hs_->ctl[kCtlHeavyFault] != 0)
       throw Error(APPT_ERR_HIP, "heavy-row window could not advance (window shape below one word of outputs)");
+    if (O.gram && hs_->total != cnnz)
+      throw Error(APPT_ERR_HIP, "Gram expansion produced " + std::to_string(hs_->total) + " entries, expected " +
+                                    std::to_string(cnnz));
   } catch (...) {
     // nothing enqueued may still write these buffers once they are reused
     (void)hipStreamSynchronize(s);
@@ -758,7 +862,7 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
 
   Result* R = new Result{rowptr_out, ccol, cval};
   C->n = n;
-  C->nnz = P.nnz;
+  C->nnz = cnnz;
   C->rowptr_is_64 = rowptr_out != rp32 ? 1 : 0;
   C->rowptr = rowptr_out;
   C->col = ccol;
@@ -773,8 +877,8 @@ void Engine::compute_rows(const appt_csr_view& A, appt_op op, int32_t r0, int32_
     bins_.sym_metric_max[b] = P.sym[b * kStatFields + 1];
     bins_.num_metric_max[b] = P.num[b * kStatFields + 1];
   }
-  times_.products = P.products;
-  times_.nnz_c = P.nnz;
+  times_.products = O.gram ? P.gram_products : P.products;
+  times_.nnz_c = cnnz;
   times_.n_bins_used = P.bins_used;
   times_.sync_points = 3;
   if (timing_) {

@@ -86,6 +86,9 @@ struct Config {
   // for the detection)
   bool templates = true;
   int64_t templates_min_nnz = (int64_t)1 << 18;
+  // AAtS: 1 / 2 = Gram over the upper / lower triangle (gram.hip), otherwise
+  // full rows (direct, or a mirror where rows accumulate with atomics)
+  int32_t gram = 0;
   bool rowptr64(int64_t nnz) const { return nnz > INT32_MAX || nnz >= rowptr64_min_nnz; }
 };
 
@@ -95,7 +98,8 @@ struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
       numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
       rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot, plancnt, planwords, digest,
-      ckey, crep, akey, arep, bclass, aclass, tkey, trep, tplid, tinfo, tposoff, tpos, tsoff, ttuple, tpos2, tmatched;
+      ckey, crep, akey, arep, bclass, aclass, tkey, trep, tplid, tinfo, tposoff, tpos, tsoff, ttuple, tpos2, tmatched,
+      vmeta, vac, ucol, uval, rrows2, goff, gtot;
   // words per row of the stored workgroup-bin bitmaps of the current call
   // (0: none stored)
   int32_t wg_stride = 0;
@@ -109,7 +113,8 @@ struct Workspace {
                       &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp,
                       &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot, &plancnt, &planwords,
                       &digest, &ckey,    &crep,  &akey,   &arep,   &bclass,  &aclass, &tkey,  &trep,   &tplid,
-                      &tinfo,  &tposoff, &tpos,  &tsoff,  &ttuple, &tpos2, &tmatched};
+                      &tinfo,  &tposoff, &tpos,  &tsoff,  &ttuple, &tpos2, &tmatched, &vmeta, &vac,
+                      &ucol,   &uval,    &rrows2, &goff,  &gtot};
     for (Scratch* s : all) f(*s);
   }
 };
@@ -131,6 +136,8 @@ struct Operands {
   int32_t ncols = 0;  // bound on output column indices (a batch's are local to each block)
   int32_t row0 = 0;   // global row index of A-side row 0 (row panels)
   bool a_is_b = false;  // the A-side rows are B's rows (A*A over all rows)
+  bool gram = false;    // AAtS as a triangle-only Gram: B rows are per-entry column ranges (gram.hip)
+  bool gram_lower = false;  // the Gram's triangle: j <= i (else j >= i)
 };
 
 // Bin statistics read back at the two host synchronisations; they fix the
@@ -143,6 +150,7 @@ struct Plan {
   int bins_used = 0;
   bool a_strict = false;    // A's rows strictly increasing (read for AAtS)
   bool direct_sym = false;  // AAtS: lower entries computed directly, no mirror
+  int64_t gram_products = 0;  // AAtS as a Gram: products of the full A*A^T (products counts U's)
 };
 
 struct Result {  // what appt_csr_result::_priv points to
@@ -155,9 +163,11 @@ struct Result {  // what appt_csr_result::_priv points to
 // [0..1] heavy-row claim counters (symbolic, numeric); [2] B rows unsorted;
 // [3] the symmetric-pattern transpose gather failed; [4] A rows not strictly
 // increasing; [5..7] structure check failures (ends, row pointer order,
-// columns); [8] a heavy numeric window could not advance.
+// columns); [8] a heavy numeric window could not advance; [12..13] AAtS as
+// an upper-triangle Gram: the full A*A^T product count (u64).
 constexpr int kCtlWords = 16;
 constexpr int kCtlHeavyFault = 8;
+constexpr int kCtlGramProducts = 12;
 
 struct HostStats {  // pinned staging for the small device->host reads; the
                     // fields up to ctl mirror the device stats layout
@@ -252,6 +262,16 @@ class Engine {
   void stage_numeric(Workspace& W, const Operands& O, const Config& cfg, const Plan& P, int32_t* ccol,
                      double* cval);
   void stage_mirror(Workspace& W, const Operands& O, const Plan& P, const int32_t* ccol, double* cval);
+  // AAtS as an upper-triangle Gram (gram.hip): B rows become per-entry
+  // suffixes (j >= i) of the transposed operand ...
+  void stage_gram_operand(Workspace& W, Operands& O, bool lower);
+  // AAtS: full rows, or a Gram over the upper / lower triangle
+  enum { kGramNone = 0, kGramUpper = 1, kGramLower = 2 };
+  int gram_mode(const appt_csr_view& A) const;
+  // ... and U (ucol / uval, offsets in W.rowoff) expands into the full C
+  // (nc entries; final offsets in W.goff and rp32)
+  void stage_gram_expand(Workspace& W, const Operands& O, const Plan& P, const int32_t* ucol, const double* uval,
+                         int64_t nc, int32_t* rp32, int32_t* ccol, double* cval);
   // host synchronisations: bin statistics of the symbolic / numeric binning
   void read_sym(Workspace& W, Plan& P, bool want_ctl = false);
   void read_num(Workspace& W, const Operands& O, Plan& P);

@@ -578,6 +578,17 @@ static void radix_passes_run(int32_t n, int64_t count, const int32_t* k0, const 
   }
 }
 
+void launch_radix_pairs(int32_t n, int64_t count, const RadixScratch& rx, int32_t* rows2, double* vals2,
+                        hipStream_t s) {
+  if (n <= 0 || count <= 0) return;
+  const bool odd = radix_passes(n) % 2 == 1;
+  const int32_t* k0 = odd ? rx.keys : rx.keys2;
+  const int32_t* r0 = odd ? rx.rows : rows2;
+  const double* v0 = odd ? rx.vals : vals2;
+  radix_passes_run(n, count, k0, r0, v0, rx.keys, rx.rows, rx.vals, rx.keys2, rows2, vals2, rx.hist, rx.hoff,
+                   rx.htmp, nullptr, s);
+}
+
 void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
                       int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
                       int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s) {

@@ -191,6 +191,37 @@ struct HeavyArgs {
   int32_t* fault;
 };
 
+// ---- triangle-only Gram, op AAtS (gram.hip) ----
+// Per A entry p = (i, k): vmeta[p] = the suffix of B row k (rows ascending)
+// from its first row >= i (lower: the prefix up to its last row <= i), as a
+// B-row descriptor; vac[p] = p.  Adds the full A*A^T product count (sum over
+// B rows of nnz^2) to *full_products.
+void launch_gram_operand(int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb, const int4* bmeta,
+                         const int32_t* bc, int lower, int4* vmeta, int32_t* vac,
+                         unsigned long long* full_products, hipStream_t s);
+// Expand T (one triangle of C's rows, diagonal included; offsets uoff, nu
+// entries) into C: mir[j] += off-diagonal entries of column j (zeroed first) ...
+void launch_gram_mirrored(int32_t n, int64_t nu, const int64_t* uoff, const int32_t* ucol, int32_t* mir,
+                          hipStream_t s);
+// ... strict[i] = T row i less its diagonal, total[i] = mir[i] + T row i ...
+void launch_gram_counts(int32_t n, const int32_t* ucnt, const int32_t* mir, int32_t* strict, int32_t* total,
+                        hipStream_t s);
+// ... the off-diagonal entries in row-major order at sup_off (scan of strict) ...
+void launch_gram_extract(int32_t n, int64_t nu, const int64_t* uoff, const int32_t* ucol, const double* uval,
+                         const int64_t* sup_off, int lower, int32_t* keys, int32_t* rows, double* vals,
+                         hipStream_t s);
+// ... (stably sorted by column: launch_radix_pairs) placed with T's rows into
+// C (mir_off: scan of mir).
+void launch_gram_assemble(int32_t n, int64_t nc, const int64_t* coff, int lower_t, const int32_t* mir,
+                          const int64_t* mir_off, const int32_t* srow, const double* sval, const int64_t* uoff,
+                          const int32_t* ucol, const double* uval, int32_t* ccol, double* cval, hipStream_t s);
+// Stable LSD radix sort of `count` (key < n, row, value) triples.  The input
+// must sit in the buffers the first pass does not write: (rx.keys, rx.rows,
+// rx.vals) when radix_passes(n) is odd, else (rx.keys2, rows2, vals2); the
+// result is in (rx.keys2, rows2, vals2).
+void launch_radix_pairs(int32_t n, int64_t count, const RadixScratch& rx, int32_t* rows2, double* vals2,
+                        hipStream_t s);
+
 // ---- row templates (templates.hip) ----
 constexpr int kClassSlots = 4096;      // relative-pattern classes per call
 constexpr int kTplSlots = 256;         // row templates per call

@@ -335,7 +335,7 @@ def _tuning_dict(t: Tuning) -> dict:
             "mirror_radix_min_nnz": int(t.mirror_radix_min_nnz),
             "class_lds_bytes": [int(x) for x in t.class_lds_bytes if x > 0],
             "sort_max_products": int(t.sort_max_products), "rowptr64_min_nnz": int(t.rowptr64_min_nnz),
-            "row_templates": int(t.row_templates)}
+            "row_templates": int(t.row_templates), "gram": int(t.gram)}
 
 
 def panels(rowptr: np.ndarray, max_bytes: int) -> list:

@@ -42,7 +42,7 @@ typedef enum appt_status {
 typedef enum appt_op {
   APPT_OP_AA = 0,    /* C = A*A */
   APPT_OP_AAT = 1,   /* C = A*A^T, B = A^T materialised */
-  APPT_OP_AATS = 2   /* C = A*A^T, upper triangle computed then mirrored */
+  APPT_OP_AATS = 2   /* C = A*A^T, symmetric: full rows, or (tuning.gram) one triangle's products, mirrored */
 } appt_op;
 
 /* Device CSR operand (src/spgemm_engine.cuh:132 dOff/dCol/dVal, n, nnz).
@@ -107,7 +107,10 @@ typedef struct appt_tuning {
                                  share a template: no symbolic pass, products straight to their slots.
                                  Detected for operands of at least this many entries (0 = 262,144;
                                  < 0 = never) */
-  int32_t reserved[1];
+  int32_t gram;               /* AAtS: 1 / 2 = Gram over the upper / lower triangle (only j >= i /
+                                 j <= i products are formed, then mirrored); 0 or -1 = full rows
+                                 (the default: the mirror's transposition outweighs the saved
+                                 products on the measured inputs) */
 } appt_tuning;
 
 typedef struct appt_options {
@@ -130,11 +133,14 @@ typedef struct appt_phase_times {
 } appt_phase_times;
 
 /* Structural descriptors of src/fetch_and_prep.py:37-48 and bench/dispatch.cu:26-37
- * plus the products / span summary the internal dispatcher bins rows by. */
+ * plus products and bandwidth.  Degrees are the stored row lengths (as
+ * bench/dispatch.cu:27-30 reads them); psym is taken over the numeric pattern
+ * after merging duplicates (src/fetch_and_prep.py:39-46): positions whose
+ * stored values, summed in storage order, are not zero. */
 typedef struct appt_descriptors {
   int32_t n;
   int64_t nnz;
-  double deg_mean, deg_cv;     /* row-degree mean and std/mean (ddof=0) */
+  double deg_mean, deg_cv;     /* row-degree mean and std/mean (ddof=0, variance exact) */
   int32_t deg_max;
   double fill_aa;              /* sum_p deg(col_p) / nnz       (bench/dispatch.cu:31-33) */
   double fill_aat;             /* sum_k colcount_k^2 / nnz     (bench/dispatch.cu:34-36) */
@@ -143,7 +149,6 @@ typedef struct appt_descriptors {
   int32_t max_row_products_aa; /* clamped to int32 range */
   int32_t bandwidth;           /* max |i - j| over stored entries */
   int32_t pick_ours;           /* decision rule of bench/dispatch.cu:14-17 on (deg_cv, fill, nnz) */
-  int32_t plan_bins;           /* number of non-empty row bins the engine would launch */
 } appt_descriptors;
 
 /* ---- engine lifetime ------------------------------------------------------ */
@@ -208,8 +213,10 @@ appt_status appt_spgemm_compute_noalloc(appt_engine* eng, const appt_csr_view* A
  * replaced by the compacted result. */
 appt_status appt_csr_eliminate_zeros(appt_engine* eng, appt_csr_result* C);
 
-/* Row panel: rows [row_begin, row_end) of op(A) (op AA or AAT) as a CSR with
- * row_end - row_begin rows and global column indices.  For products whose C
+/* Row panel: rows [row_begin, row_end) of op(A) as a CSR with row_end -
+ * row_begin rows and global column indices.  A panel of AATS holds complete
+ * rows of the symmetric product, computed as A*A^T's rows (no triangle to
+ * mirror from when only some rows are formed).  For products whose C
  * does not fit the device at once; no counterpart in the reference, whose
  * int32 arena overflows first (src/spgemm_engine.cuh:156-162). */
 appt_status appt_spgemm_compute_rows(appt_engine* eng, const appt_csr_view* A, appt_op op, int32_t row_begin,
@@ -236,7 +243,8 @@ appt_status appt_spgemm_run_timed(const char* task, const int32_t* dOff, const i
 appt_status appt_engine_run_timed(appt_engine* eng, const char* task, const appt_csr_view* A,
                                   int32_t reps, appt_eng_result* out);
 
-/* Descriptors + plan (bench/dispatch.cu:26-38), computed on the GPU. */
+/* Descriptors and the decision rule (bench/dispatch.cu:26-38), computed on
+ * the GPU from A as it lies (nothing staged), with one host synchronisation. */
 appt_status appt_dispatch_descriptors(appt_engine* eng, const appt_csr_view* A, appt_op op,
                                       appt_descriptors* out);
 

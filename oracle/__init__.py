@@ -48,7 +48,7 @@ def lib() -> C.CDLL:
         L.oracle_checksum.restype = None
         L.oracle_checksum.argtypes = [f64p, C.c_int64, f64p]
         L.oracle_descriptors.restype = C.c_int
-        L.oracle_descriptors.argtypes = [C.c_int32, i32p, i32p, f64p]
+        L.oracle_descriptors.argtypes = [C.c_int32, i32p, i32p, f64p, f64p]
         L.oracle_pick_ours.restype = C.c_int
         L.oracle_pick_ours.argtypes = [C.c_double, C.c_double, C.c_int64]
         _lib = L
@@ -151,10 +151,11 @@ def checksum(vals):
     return tuple(float(x) for x in out)
 
 
-def descriptors(n, rowptr, col):
+def descriptors(n, rowptr, col, val):
     rp, cl = np.ascontiguousarray(rowptr, np.int32), np.ascontiguousarray(col, np.int32)
+    vl = np.ascontiguousarray(val, np.float64)
     out = np.empty(9, np.float64)
-    if lib().oracle_descriptors(n, _p(rp, C.c_int32), _p(cl, C.c_int32), _p(out, C.c_double)):
+    if lib().oracle_descriptors(n, _p(rp, C.c_int32), _p(cl, C.c_int32), _p(vl, C.c_double), _p(out, C.c_double)):
         raise MemoryError("oracle_descriptors failed")
     keys = ["deg_mean", "deg_cv", "deg_max", "fill_aa", "fill_aat", "psym", "products_aa",
             "products_aat", "bandwidth"]

@@ -267,8 +267,12 @@ int oracle_pick_ours(double dcv, double fill, int64_t nnz) {
 /* Descriptors (bench/dispatch.cu:26-37; src/fetch_and_prep.py:37-48).
  * out[0]=deg_mean out[1]=deg_cv out[2]=deg_max out[3]=fill_aa out[4]=fill_aat
  * out[5]=psym out[6]=products_aa out[7]=products_aat out[8]=bandwidth
- * psym counts distinct stored positions (after merging duplicates). */
-int oracle_descriptors(int32_t n, const int32_t* ap, const int32_t* ac, double* out) {
+ * Degrees are the stored row lengths (bench/dispatch.cu:27-30, two-pass
+ * variance, ddof=0).  psym follows src/fetch_and_prep.py:39-46: P is the
+ * numeric pattern after merging duplicates -- the positions whose stored
+ * values, summed in storage order, are not zero (NaN counts as nonzero) --
+ * and psym = |P & P^T| / max(1, |P|). */
+int oracle_descriptors(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, double* out) {
   int64_t nnz = ap[n];
   double dmean = n > 0 ? (double)nnz / n : 0.0, var = 0;
   int32_t dmax = 0;
@@ -286,28 +290,44 @@ int oracle_descriptors(int32_t n, const int32_t* ap, const int32_t* ac, double* 
   int64_t acc_aat = 0;
   for (int32_t k = 0; k < n; k++) acc_aat += cc[k] * cc[k];
   free(cc);
-  /* psym: distinct (i,j) with (j,i) also stored, over distinct stored (i,j) */
-  int32_t* tp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
-  int32_t* tc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+  /* P, row by row: merge each row's duplicates into a dense accumulator */
+  int32_t* prp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+  int32_t* pcol = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+  int32_t* order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nnz > 0 ? nnz : 1));
+  double* accv = (double*)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
   int32_t* mark = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-  int32_t* mark2 = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
-  if (!tp || !tc || !mark || !mark2) return 1;
-  oracle_transpose(n, ap, ac, NULL, tp, tc, NULL);
-  for (int32_t j = 0; j < n; j++) { mark[j] = -1; mark2[j] = -1; }
-  int64_t distinct = 0, inter = 0;
+  if (!prp || !pcol || !order || !accv || !mark) return 1;
+  for (int32_t j = 0; j < n; j++) mark[j] = -1;
   int32_t bw = 0;
+  int64_t np = 0;
+  prp[0] = 0;
   for (int32_t i = 0; i < n; i++) {
-    for (int32_t q = tp[i]; q < tp[i + 1]; q++) mark[tc[q]] = i;      /* row i of A^T */
+    int64_t nd = 0;
     for (int32_t p = ap[i]; p < ap[i + 1]; p++) {
       int32_t j = ac[p];
       int32_t d = i > j ? i - j : j - i;
       if (d > bw) bw = d;
-      if (mark2[j] == i) continue;                                       /* duplicate */
-      mark2[j] = i; distinct++;
-      if (mark[j] == i) inter++;
+      if (mark[j] != i) { mark[j] = i; accv[j] = 0.0; order[nd++] = j; }
+      accv[j] += av[p];
     }
+    for (int64_t t = 0; t < nd; t++)
+      if (accv[order[t]] != 0.0) pcol[np++] = order[t];
+    prp[i + 1] = (int32_t)np;
   }
-  free(tp); free(tc); free(mark); free(mark2);
+  /* |P & P^T|: row i of P against row i of P^T */
+  int32_t* tp = (int32_t*)malloc(sizeof(int32_t) * ((size_t)n + 1));
+  int32_t* tc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(np > 0 ? np : 1));
+  if (!tp || !tc) return 1;
+  if (oracle_transpose(n, prp, pcol, NULL, tp, tc, NULL)) return 1;
+  for (int32_t j = 0; j < n; j++) mark[j] = -1;
+  int64_t inter = 0;
+  for (int32_t i = 0; i < n; i++) {
+    for (int32_t q = tp[i]; q < tp[i + 1]; q++) mark[tc[q]] = i;
+    for (int32_t p = prp[i]; p < prp[i + 1]; p++)
+      if (mark[pcol[p]] == i) inter++;
+  }
+  free(prp); free(pcol); free(order); free(accv); free(mark); free(tp); free(tc);
+  int64_t distinct = np;
   out[0] = dmean; out[1] = dcv; out[2] = dmax;
   out[3] = nnz > 0 ? (double)acc_aa / nnz : 0.0;
   out[4] = nnz > 0 ? (double)acc_aat / nnz : 0.0;

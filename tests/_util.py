@@ -46,3 +46,39 @@ def rows_sorted(rp, col):
     starts = np.zeros(len(col), bool)
     starts[rp[1:-1][rp[1:-1] < len(col)]] = True
     return bool(np.all((d > 0) | starts[1:]))
+
+
+def messy(n: int, seed: int, mean_len: int = 6):
+    """A square CSR matrix the way files and assemblers hand them over: rows
+    in random column order, duplicate positions (half of them cancelling to
+    zero), explicit zeros, and a large symmetric part (40 % of the entries
+    mirrored).  Returns an engine HostMatrix."""
+    import appt_spgemm_amd as ap
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, 2 * mean_len + 1, n)
+    r = np.repeat(np.arange(n), k)
+    c = rng.integers(0, n, r.size)
+    v = rng.uniform(-1, 1, r.size)
+    mir = rng.random(r.size) < 0.4
+    dup = rng.random(r.size) < 0.1
+    cancel = rng.random(int(dup.sum())) < 0.5
+    vd = np.where(cancel, -v[dup], rng.uniform(-1, 1, int(dup.sum())))
+    nz = max(1, n // 5)
+    zr, zc = rng.integers(0, n, nz), rng.integers(0, n, nz)
+    R = np.concatenate([r, c[mir], r[dup], zr])
+    Cc = np.concatenate([c, r[mir], c[dup], zc])
+    V = np.concatenate([v, rng.uniform(-1, 1, int(mir.sum())), vd, np.zeros(nz)])
+    order = np.lexsort((rng.random(R.size), R))     # by row, random within a row
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, R + 1, 1)
+    return ap.HostMatrix(n, np.cumsum(rp).astype(np.int32), Cc[order].astype(np.int32), V[order].astype(np.float64))
+
+
+def scipy_psym(m):
+    """psym as src/fetch_and_prep.py:39-46 computes it: merge duplicates,
+    keep nonzero values, |P & P^T| / |P|."""
+    import scipy.sparse as sp
+    A = sp.csr_matrix((m.val.copy(), m.col.copy(), m.rowptr.copy()), shape=(m.n, m.n))
+    A.sum_duplicates()
+    P = (A != 0).astype(np.int8)
+    return P.multiply(P.T).nnz / max(1, P.nnz)

@@ -387,21 +387,25 @@ def test_run_ours_free_function(eng):
 
 
 def test_descriptors_match_oracle(eng):
-    for m in (ap.rmat(11, 16, seed=3), ap.laplace2d(30), ap.stencil27(8, 0.1, 1)):
+    # includes inputs with unsorted rows, duplicate positions, cancelling
+    # duplicates and explicit zeros (psym over the merged numeric pattern)
+    from tests._util import messy
+    for m in (ap.rmat(11, 16, seed=3), ap.laplace2d(30), ap.stencil27(8, 0.1, 1), messy(2000, 1), messy(500, 2, 40)):
         A = eng.upload(m)
         try:
             for op in ("AA", "AAt"):
+                say("case descriptors", m.n, m.nnz, op)
                 d = eng.descriptors(A, op)
-                o = oracle.descriptors(m.n, m.rowptr, m.col)
-                for k in ("deg_mean", "deg_cv", "fill_aa", "fill_aat", "psym"):
-                    assert d[k] == pytest.approx(o[k], rel=1e-12, abs=1e-12), k
+                o = oracle.descriptors(m.n, m.rowptr, m.col, m.val)
+                for k in ("deg_mean", "deg_cv", "fill_aa", "fill_aat"):
+                    assert d[k] == pytest.approx(o[k], rel=1e-13, abs=1e-15), k
+                assert d["psym"] == o["psym"]      # same two integer counts
                 assert d["products_aa"] == o["products_aa"]
                 assert d["products_aat"] == o["products_aat"]
                 assert d["bandwidth"] == o["bandwidth"]
                 assert d["deg_max"] == o["deg_max"]
                 fill = o["fill_aa"] if op == "AA" else o["fill_aat"]
                 assert d["pick_ours"] == oracle.pick_ours(o["deg_cv"], fill, m.nnz)
-                assert d["plan_bins"] >= 1
         finally:
             A.free()
 
@@ -546,13 +550,3 @@ def test_row_panels_planned_heavy_rows(eng_heavy, op):
         A.free()
         B.free()
         fresh.close()
-
-
-def test_row_panels_reject_aats(eng):
-    m = ap.laplace2d(16)
-    A = eng.upload(m)
-    try:
-        with pytest.raises(ap.ApptError):
-            eng.compute_rows(A, "AAtS", 0, 10)
-    finally:
-        A.free()

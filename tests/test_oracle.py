@@ -171,7 +171,7 @@ def test_checksum_sequential():
 def test_descriptors_against_numpy():
     sp = pytest.importorskip("scipy.sparse")
     for m in (ap.rmat(10, 8, seed=2), ap.laplace2d(9), ap.erdos_renyi(700, 4.0, 9)):
-        d = oracle.descriptors(m.n, m.rowptr, m.col)
+        d = oracle.descriptors(m.n, m.rowptr, m.col, m.val)
         deg = np.diff(m.rowptr).astype(np.float64)
         assert d["deg_mean"] == pytest.approx(deg.mean())
         assert d["deg_cv"] == pytest.approx(deg.std() / deg.mean())
@@ -182,6 +182,26 @@ def test_descriptors_against_numpy():
         assert d["psym"] == pytest.approx(P.multiply(P.T).nnz / P.nnz)
         rows = np.repeat(np.arange(m.n), np.diff(m.rowptr))
         assert d["bandwidth"] == np.abs(rows - m.col).max()
+
+
+def test_descriptors_psym_numeric_pattern():
+    # src/fetch_and_prep.py:39-46 merges duplicates and drops zeros before the
+    # symmetry count; stored row lengths still give the degree statistics
+    from tests._util import messy, scipy_psym
+    for seed in range(4):
+        m = messy(300, seed)
+        d = oracle.descriptors(m.n, m.rowptr, m.col, m.val)
+        assert d["psym"] == pytest.approx(scipy_psym(m), rel=0, abs=1e-15)
+        deg = np.diff(m.rowptr).astype(np.float64)
+        assert d["deg_cv"] == pytest.approx(deg.std() / deg.mean(), rel=1e-13)
+    # hand-made: (0,1) stored twice summing to zero, (1,0) stored -> not
+    # symmetric; (1,2)/(2,1) symmetric; an explicit zero at (2,2) is no entry
+    m = ap.HostMatrix(3, np.array([0, 2, 4, 7], np.int32), np.array([1, 1, 0, 2, 1, 2, 0], np.int32),
+                      np.array([0.5, -0.5, 2.0, 3.0, 4.0, 0.0, 1.0]))
+    d = oracle.descriptors(m.n, m.rowptr, m.col, m.val)
+    # P = {(1,0), (1,2), (2,1), (2,0)}; symmetric pairs: (1,2) and (2,1)
+    assert d["psym"] == 2 / 4
+    assert d["psym"] == scipy_psym(m)
 
 
 def test_pick_ours_rule():
