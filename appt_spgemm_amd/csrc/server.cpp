@@ -533,6 +533,7 @@ void Server::release(uint64_t id) {
   std::shared_ptr<SharedC> last;  // the group's C, when this is its last job
   int slot = -1;
   bool waiting = false;
+  int64_t live = 0;
   {
     std::lock_guard<std::mutex> lk(mu_);
     auto it = jobs_.find(id);
@@ -551,6 +552,7 @@ void Server::release(uint64_t id) {
     j.reserved = 0;
     j.state = APPT_JOB_RELEASED;
     waiting = waiters_ > 0;
+    if (slot >= 0) live = slots_[slot]->live;
   }
   if (slot >= 0 && (C._priv || last)) {
     Engine& eng = *slots_[slot]->eng;
@@ -562,8 +564,8 @@ void Server::release(uint64_t id) {
       eng.pool().put(last->val);
     }
     // the bytes stay in the slot's pool (and charged to it) for reuse, unless
-    // a job is waiting for memory
-    if (waiting) {
+    // a job is waiting for memory or the slot's idle cache outgrows its share
+    if (waiting || eng.pool().allocated() - live > cache_cap_) {
       const int64_t before = eng.pool().allocated();
       eng.pool().trim();
       credit(slot, before - eng.pool().allocated());

@@ -221,8 +221,10 @@ def test_c5_served_poisson_stream(eng):
             f"{st['jobs_per_s']:.1f} jobs/s, peak reserved {st['peak_bytes_reserved'] / 1e9:.1f} GB")
         assert ok + refused == jobs_n and st["completed"] == ok and st["refused"] == refused
         assert refused >= 4
-        assert st["peak_bytes_reserved"] <= budget
-        assert st["bytes_reserved"] == 0
+        assert st["peak_bytes_reserved"] <= budget and st["peak_device_bytes"] <= budget
+        assert st["overruns"] == 0
+        # every job released: what stays charged is what the slots' pools cache
+        assert st["bytes_reserved"] == st["device_bytes"]
     finally:
         for d in dev:
             d.free()
