@@ -5,8 +5,9 @@
 // fill_aat = sum_k colcount_k^2/nnz), the psym estimate of
 // src/fetch_and_prep.py:39-46 (the numeric pattern after merging duplicates),
 // the decision rule pick_ours of bench/dispatch.cu:14-17, and adds bandwidth
-// and products.  Reads A once or twice with no staging, in four kernels and
-// one host synchronisation.
+// and products.  Reads A as it lies (no staging): per-entry products, order
+// and bandwidth, per-row statistics, the column histogram, psym; one host
+// synchronisation.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -33,86 +34,90 @@ enum : int {
   kDescWords
 };
 
-// One G-lane group per row: degree, products, bandwidth, order; writes the
-// row's strictly-ascending flag for the psym pass.
-__global__ void __launch_bounds__(256) k_desc_rows(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
-                            uint8_t* __restrict__ ascending, unsigned long long* __restrict__ acc) {
-  constexpr int G = 16;
-  const int lane = threadIdx.x % G;
-  const int64_t gpb = blockDim.x / G;
-  unsigned long long sd2 = 0, mx = 0, bw = 0, uns = 0, prod = 0, pmax = 0;
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb) {
-    const int s = ap[i], e = ap[i + 1];
-    unsigned long long f = 0, b = 0, u = 0;
-    for (int p = s + lane; p < e; p += G) {
-      const int c = ac[p];
-      f += (unsigned long long)(ap[c + 1] - ap[c]);
-      const long long d = (long long)c - i;
-      const unsigned long long ad = (unsigned long long)(d < 0 ? -d : d);
-      b = ad > b ? ad : b;
-      if (p + 1 < e && ac[p + 1] <= c) u = 1;
-    }
-    f = gsum<G, unsigned long long>(f, nullptr);
-    b = gmax<G>(b);
-    u = gmax<G>(u);
-    if (lane == 0) {
-      ascending[i] = u ? 0 : 1;
-      const unsigned long long d = (unsigned long long)(e - s);
-      sd2 += d * d;
-      mx = d > mx ? d : mx;
-      bw = b > bw ? b : bw;
-      uns += u;
-      prod += f;
-      pmax = f > pmax ? f : pmax;
-    }
+// Entry-parallel kernels: each thread takes kDescRun consecutive entries of
+// A, finds the row of its first one by binary search in the row pointers,
+// and walks rows forward from there, so a power-law hub row is spread over
+// many threads and blocks instead of one lane group.
+constexpr int kDescRun = 8;
+
+__device__ __forceinline__ int32_t desc_row_of(const int32_t* __restrict__ ap, int32_t n, int64_t e0) {
+  int l = 0, h = n - 1;  // the last row r with ap[r] <= e0
+  while (l < h) {
+    const int mid = (int)(((int64_t)l + h + 1) >> 1);
+    if ((int64_t)ap[mid] <= e0) l = mid;
+    else h = mid - 1;
   }
-  // the block's groups, then one atomic per accumulator and block
-  __shared__ unsigned long long red[6][16];
-  const int g = threadIdx.x / G;
-  if (lane == 0) {
-    red[0][g] = sd2;
-    red[1][g] = mx;
-    red[2][g] = bw;
-    red[3][g] = uns;
-    red[4][g] = prod;
-    red[5][g] = pmax;
+  return l;
+}
+
+// Zero the accumulators, the column histogram, the per-row products and the
+// per-row order flags.
+__global__ void k_desc_clear(int32_t n, unsigned long long* __restrict__ acc, int32_t* __restrict__ cnt,
+                             unsigned long long* __restrict__ rowf, uint8_t* __restrict__ notasc) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    cnt[i] = 0;
+    rowf[i] = 0ull;
+    notasc[i] = 0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < kDescWords) acc[threadIdx.x] = 0ull;
+}
+
+// Per entry: products deg(col) (summed per row into rowf, and over A),
+// bandwidth, the column histogram, and each row's order (notasc[i] = 1 on a
+// descent or a repeated column).
+__global__ void __launch_bounds__(256) k_desc_entries(int32_t n, int64_t nnz, const int32_t* __restrict__ ap,
+                                                      const int32_t* __restrict__ ac, int32_t* __restrict__ cnt,
+                                                      unsigned long long* __restrict__ rowf,
+                                                      uint8_t* __restrict__ notasc,
+                                                      unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long red[2][4];
+  const int64_t e0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDescRun;
+  unsigned long long prod = 0, bw = 0;
+  if (e0 < nnz) {
+    const int64_t e1 = e0 + kDescRun < nnz ? e0 + kDescRun : nnz;
+    int32_t r = desc_row_of(ap, n, e0);
+    int64_t rend = ap[r + 1];
+    unsigned long long run = 0;  // products of row r among this thread's entries
+    int prev = -1;
+    for (int64_t e = e0; e < e1; e++) {
+      if (e >= rend) {
+        if (run) atomicAdd(&rowf[r], run);
+        run = 0;
+        prev = -1;
+        while (e >= rend) rend = ap[++r + 1];
+      }
+      const int c = ac[e];
+      const unsigned long long f = (unsigned long long)(ap[c + 1] - ap[c]);
+      run += f;
+      prod += f;
+      const long long d = (long long)c - r;
+      const unsigned long long ad = (unsigned long long)(d < 0 ? -d : d);
+      bw = ad > bw ? ad : bw;
+      // order across this thread's entries, and with the next thread's first
+      if ((prev >= 0 && c <= prev) || (e + 1 == e1 && e + 1 < rend && ac[e + 1] <= c)) notasc[r] = 1;
+      prev = c;
+      atomicAdd(&cnt[c], 1);
+    }
+    if (run) atomicAdd(&rowf[r], run);
+  }
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    prod += __shfl_xor(prod, m, 64);
+    const unsigned long long o = __shfl_xor(bw, m, 64);
+    bw = o > bw ? o : bw;
+  }
+  if (threadIdx.x % 64 == 0) {
+    red[0][threadIdx.x / 64] = prod;
+    red[1][threadIdx.x / 64] = bw;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int ng = blockDim.x / G;
-    for (int t = 1; t < ng; t++) {
-      sd2 += red[0][t];
-      mx = red[1][t] > mx ? red[1][t] : mx;
-      bw = red[2][t] > bw ? red[2][t] : bw;
-      uns += red[3][t];
-      prod += red[4][t];
-      pmax = red[5][t] > pmax ? red[5][t] : pmax;
-    }
-    if (sd2) {
-      const unsigned long long old = atomicAdd(&acc[kDescSumD2], sd2);
-      if (old + sd2 < old) atomicAdd(&acc[kDescSumD2Hi], 1ull);
-    }
-    if (mx) atomicMax(&acc[kDescMaxD], mx);
-    if (bw) atomicMax(&acc[kDescBandwidth], bw);
-    if (uns) atomicAdd(&acc[kDescUnsorted], uns);
+    prod = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    bw = red[1][0];
+    for (int w = 1; w < 4; w++) bw = red[1][w] > bw ? red[1][w] : bw;
     if (prod) atomicAdd(&acc[kDescProducts], prod);
-    if (pmax) atomicMax(&acc[kDescMaxRowProd], pmax);
+    if (bw) atomicMax(&acc[kDescBandwidth], bw);
   }
-}
-
-__global__ void __launch_bounds__(256) k_sumsq(int32_t n, const int32_t* __restrict__ cnt,
-                                               unsigned long long* __restrict__ out) {
-  __shared__ unsigned long long red[4];
-  unsigned long long s = 0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const unsigned long long c = (unsigned long long)cnt[i];
-    s += c * c;
-  }
-#pragma unroll
-  for (int m = 32; m > 0; m >>= 1) s += __shfl_xor(s, m, 64);
-  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = s;
-  __syncthreads();
-  if (threadIdx.x == 0 && red[0] + red[1] + red[2] + red[3]) atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
 }
 
 // Merged value of position (r, c): the stored values of row r at column c,
@@ -147,59 +152,87 @@ __device__ __forceinline__ double merged_value(const int32_t* __restrict__ ap, c
   return v;
 }
 
-// psym: each position (i, j) of P -- the first stored occurrence of column j
-// in row i, kept when the row's merged value there is not zero -- counts in
-// |P|, and in |P & P^T| when (j, i) is in P too.
-__global__ void __launch_bounds__(256) k_psym(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
-                       const double* __restrict__ av, const uint8_t* __restrict__ ascending,
-                       unsigned long long* __restrict__ acc) {
-  constexpr int G = 16;
-  const int lane = threadIdx.x % G;
-  const int64_t gpb = blockDim.x / G;
-  unsigned long long hits = 0, pat = 0;
-  for (int64_t i = (int64_t)blockIdx.x * gpb + threadIdx.x / G; i < n; i += (int64_t)gridDim.x * gpb) {
-    const int s = ap[i], e = ap[i + 1];
-    const bool asc = ascending[i] != 0;
-    for (int p = s + lane; p < e; p += G) {
-      const int32_t j = ac[p];
+// Per row (threads below n): sum of d^2 (with its carry word), longest row,
+// rows out of order, largest row products, sum of squared column counts.
+// Per entry (kDescRun a thread): psym -- each position (i, j) of P, the first
+// stored occurrence of column j in row i kept when the row's merged value
+// there is not zero, counts in |P|, and in |P & P^T| when (j, i) is in P too.
+// One atomic per accumulator and block.
+__global__ void __launch_bounds__(256) k_desc_finish(int32_t n, int64_t nnz, const int32_t* __restrict__ ap,
+                                                     const int32_t* __restrict__ ac, const double* __restrict__ av,
+                                                     const int32_t* __restrict__ cnt,
+                                                     const unsigned long long* __restrict__ rowf,
+                                                     const uint8_t* __restrict__ notasc,
+                                                     unsigned long long* __restrict__ acc) {
+  __shared__ unsigned long long red[7][4];
+  unsigned long long sd2 = 0, mx = 0, uns = 0, pmax = 0, csq = 0, hits = 0, pat = 0;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t i = gid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const unsigned long long d = (unsigned long long)(ap[i + 1] - ap[i]);
+    sd2 += d * d;
+    mx = d > mx ? d : mx;
+    uns += notasc[i];
+    pmax = rowf[i] > pmax ? rowf[i] : pmax;
+    const unsigned long long c = (unsigned long long)cnt[i];
+    csq += c * c;
+  }
+  const int64_t e0 = gid * kDescRun;
+  if (e0 < nnz) {
+    const int64_t e1 = e0 + kDescRun < nnz ? e0 + kDescRun : nnz;
+    int32_t r = desc_row_of(ap, n, e0);
+    int64_t rend = ap[r + 1];
+    for (int64_t e = e0; e < e1; e++) {
+      while (e >= rend) rend = ap[++r + 1];
+      const int32_t j = ac[e];
       double v;
-      if (asc) {
-        v = av[p];
+      if (notasc[r] == 0) {
+        v = av[e];
       } else {
+        const int s = ap[r];
         bool first = true;
-        for (int q = s; q < p; q++)
+        for (int q = s; q < (int)e; q++)
           if (ac[q] == j) {
             first = false;
             break;
           }
         if (!first) continue;
         v = 0.0;
-        for (int q = p; q < e; q++)
+        for (int q = (int)e; q < (int)rend; q++)
           if (ac[q] == j) v += av[q];
       }
       if (!(v != 0.0)) continue;  // NaN stays in the pattern, as (A != 0) keeps it
       pat++;
       bool found;
-      const double w = merged_value(ap, ac, av, ascending[j] != 0, j, (int32_t)i, &found);
+      const double w = merged_value(ap, ac, av, notasc[j] == 0, j, r, &found);
       if (found && w != 0.0) hits++;
     }
   }
+  unsigned long long v[7] = {sd2, mx, uns, pmax, csq, hits, pat};
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) {
-    hits += __shfl_xor(hits, m, 64);
-    pat += __shfl_xor(pat, m, 64);
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+      const unsigned long long o = __shfl_xor(v[k], m, 64);
+      v[k] = (k == 1 || k == 3) ? (o > v[k] ? o : v[k]) : v[k] + o;
+    }
   }
-  __shared__ unsigned long long red[2][4];
-  if (threadIdx.x % 64 == 0) {
-    red[0][threadIdx.x / 64] = hits;
-    red[1][threadIdx.x / 64] = pat;
-  }
+  if (threadIdx.x % 64 == 0)
+    for (int k = 0; k < 7; k++) red[k][threadIdx.x / 64] = v[k];
   __syncthreads();
   if (threadIdx.x == 0) {
-    hits = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    pat = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-    if (hits) atomicAdd(&acc[kDescInter], hits);
-    if (pat) atomicAdd(&acc[kDescPattern], pat);
+    for (int k = 0; k < 7; k++)
+      for (int w = 1; w < 4; w++)
+        v[k] = (k == 1 || k == 3) ? (red[k][w] > v[k] ? red[k][w] : v[k]) : v[k] + red[k][w];
+    if (v[0]) {
+      const unsigned long long old = atomicAdd(&acc[kDescSumD2], v[0]);
+      if (old + v[0] < old) atomicAdd(&acc[kDescSumD2Hi], 1ull);
+    }
+    if (v[1]) atomicMax(&acc[kDescMaxD], v[1]);
+    if (v[2]) atomicAdd(&acc[kDescUnsorted], v[2]);
+    if (v[3]) atomicMax(&acc[kDescMaxRowProd], v[3]);
+    if (v[4]) atomicAdd(&acc[kDescColSq], v[4]);
+    if (v[5]) atomicAdd(&acc[kDescInter], v[5]);
+    if (v[6]) atomicAdd(&acc[kDescPattern], v[6]);
   }
 }
 
@@ -220,19 +253,24 @@ void Engine::descriptors(const appt_csr_view& A, appt_op op, appt_descriptors* o
   if (n == 0) return;
   unsigned long long* acc = as_ull(ensure(ws_.digest, sizeof(unsigned long long) * kDescWords));
   int32_t* cnt = (int32_t*)ensure(ws_.cursor, sizeof(int32_t) * ((size_t)n + 1));
-  uint8_t* asc = (uint8_t*)ensure(ws_.symbin, (size_t)n);
-  launch_fill_i32((int32_t*)acc, 2 * kDescWords, 0, s);
-  launch_fill_i32(cnt, n, 0, s);
-  const int64_t blocks = std::min<int64_t>(((int64_t)n * 16 + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_desc_rows, dim3((unsigned)blocks), dim3(256), 0, s, n, A.rowptr, A.col, asc, acc);
-  launch_colcount(nnz, A.col, cnt, s);
-  const int64_t b2 = std::min<int64_t>(((int64_t)n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_sumsq, dim3((unsigned)b2), dim3(256), 0, s, n, cnt, acc + kDescColSq);
-  hipLaunchKernelGGL(k_psym, dim3((unsigned)blocks), dim3(256), 0, s, n, A.rowptr, A.col, A.val, asc, acc);
+  uint8_t* notasc = (uint8_t*)ensure(ws_.symbin, (size_t)n);
+  auto* rowf = as_ull(ensure(ws_.rowf, sizeof(unsigned long long) * (size_t)n));
+  const int64_t rb = std::min<int64_t>(((int64_t)n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_desc_clear, dim3((unsigned)rb), dim3(256), 0, s, n, acc, cnt, rowf, notasc);
+  // one thread per kDescRun entries, and at least one per row for the row statistics
+  const int64_t eb = (nnz + 256 * kDescRun - 1) / (256 * kDescRun);
+  if (nnz > 0)
+    hipLaunchKernelGGL(k_desc_entries, dim3((unsigned)eb), dim3(256), 0, s, n, nnz, A.rowptr, A.col, cnt, rowf,
+                       notasc, acc);
+  hipLaunchKernelGGL(k_desc_finish, dim3((unsigned)std::max(eb, rb)), dim3(256), 0, s, n, nnz, A.rowptr, A.col,
+                     A.val, cnt, rowf, notasc, acc);
   APPT_HIP(hipGetLastError());
+  // one copy into the engine's pinned staging, one synchronisation
+  static_assert(sizeof(HostStats::sym) >= sizeof(unsigned long long) * kDescWords, "staging too small");
+  APPT_HIP(hipMemcpyAsync(hs_->sym, acc, sizeof(unsigned long long) * kDescWords, hipMemcpyDeviceToHost, s));
+  wait_stream();
   unsigned long long h[kDescWords];
-  APPT_HIP(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
-  APPT_HIP(hipStreamSynchronize(s));
+  std::memcpy(h, hs_->sym, sizeof(h));
 
   // variance exactly: n^2 var = n * sum d^2 - nnz^2 (the two-pass
   // sum (d - mean)^2 of bench/dispatch.cu:28-30, without its rounding)
