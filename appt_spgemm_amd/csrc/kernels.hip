@@ -933,7 +933,23 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     if (tpl >= 0 && (T.info[tpl].x == 0 || T.aclass[i] != T.aclass[T.trep[tpl]])) tpl = -1;
     const int32_t* tup = tpl >= 0 ? T.tuple + (size_t)tpl * kTplMaxLen - s : nullptr;
     int same = 1;
-    if (live && !lng) {
+    if (tpl >= 0) {
+      // a template candidate is checked on its entries' B classes alone; its
+      // products are the template's, its span unused
+      for (int p0 = s + lane; p0 < e; p0 += 4 * G) {
+        int k[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) k[u] = p0 + u * G < e ? ac[p0 + u * G] : -1;
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+          if (k[u] >= 0) same &= T.bclass[k[u]] == tup[p0 + u * G] ? 1 : 0;
+      }
+    }
+    same = gmin<G>(same);
+    if (!same) tpl = -1;
+    if (tpl >= 0) {
+      if (lane == 0) f = T.posoff[(size_t)tpl * (kTplMaxLen + 1) + T.info[tpl].y];
+    } else if (live && !lng) {
       // 4 entries per lane per step: their column and B-row loads in flight together
       for (int p0 = s + lane; p0 < e; p0 += 4 * G) {
         int k[4];
@@ -942,11 +958,6 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
         int4 bm[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) bm[u] = k[u] >= 0 ? bmeta[k[u]] : make_int4(0, 0, INT32_MAX, -1);
-        if (tpl >= 0) {
-#pragma unroll
-          for (int u = 0; u < 4; u++)
-            if (k[u] >= 0) same &= T.bclass[k[u]] == tup[p0 + u * G] ? 1 : 0;
-        }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
           const int len = bm[u].y - bm[u].x;
@@ -961,8 +972,6 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     f = gsum<G, int64_t>(f, nullptr);
     lo = gmin<G>(lo);
     hi = gmax<G>(hi);
-    same = gmin<G>(same);
-    if (!same) tpl = -1;
     const bool leader = live && !lng && lane == 0;
     const auto bs = finish(i, f, lo, hi, e - s, leader, tpl);
     // template rows: metric = products (their slot-table size), words = longest B row
