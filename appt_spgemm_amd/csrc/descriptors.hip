@@ -34,20 +34,59 @@ enum : int {
   kDescWords
 };
 
-// Entry-parallel kernels: each thread takes kDescRun consecutive entries of
-// A, finds the row of its first one by binary search in the row pointers,
-// and walks rows forward from there, so a power-law hub row is spread over
-// many threads and blocks instead of one lane group.
-constexpr int kDescRun = 8;
+// Entry-parallel kernels: a block takes kDescChunk consecutive entries of A
+// (lane-strided, so loads coalesce); the rows of its chunk come from one
+// search per chunk (k_desc_chunks) and are expanded into a row id per entry
+// in LDS, so a power-law hub row is spread over many blocks instead of one
+// lane group.
+constexpr int kDescChunk = 2048;
 
-__device__ __forceinline__ int32_t desc_row_of(const int32_t* __restrict__ ap, int32_t n, int64_t e0) {
-  int l = 0, h = n - 1;  // the last row r with ap[r] <= e0
-  while (l < h) {
-    const int mid = (int)(((int64_t)l + h + 1) >> 1);
-    if ((int64_t)ap[mid] <= e0) l = mid;
-    else h = mid - 1;
+__device__ __forceinline__ int32_t desc_row_of(const int32_t* __restrict__ ap, int32_t lo, int32_t hi, int64_t e) {
+  while (lo < hi) {  // the last row r in [lo, hi] with ap[r] <= e
+    const int mid = (int)(((int64_t)lo + hi + 1) >> 1);
+    if ((int64_t)ap[mid] <= e) lo = mid;
+    else hi = mid - 1;
   }
-  return l;
+  return lo;
+}
+
+// crow[c] = the row of entry min(c * kDescChunk, nnz - 1), c = 0 .. nchunks:
+// block c's entries lie in rows [crow[c], crow[c + 1]].  One thread per
+// chunk, so no block starts with a search over all rows.
+__global__ void k_desc_chunks(int32_t n, int64_t nnz, const int32_t* __restrict__ ap, int64_t nchunks,
+                              int32_t* __restrict__ crow) {
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= nchunks; c += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t e = c * kDescChunk < nnz ? c * kDescChunk : nnz - 1;
+    crow[c] = desc_row_of(ap, 0, n - 1, e);
+  }
+}
+
+// Row of every entry of the block's chunk [e0, e1), in LDS (rows r0 .. r1
+// expanded by the block's threads, a row at a time each).
+__device__ __forceinline__ void desc_rowids(const int32_t* __restrict__ ap, int32_t r0, int32_t r1, int64_t e0,
+                                            int64_t e1, int32_t* rowid) {
+  for (int32_t r = r0 + (int32_t)threadIdx.x; r <= r1; r += (int32_t)blockDim.x) {
+    const int64_t a = (int64_t)ap[r] > e0 ? (int64_t)ap[r] : e0;
+    const int64_t b = (int64_t)ap[r + 1] < e1 ? (int64_t)ap[r + 1] : e1;
+    for (int64_t e = a; e < b; e++) rowid[e - e0] = r;
+  }
+  __syncthreads();
+}
+
+// Sum of v over the lanes of a wave that share key (equal keys on
+// consecutive lanes); the segment's last lane gets the total and *last.
+__device__ __forceinline__ unsigned long long seg_sum(unsigned long long v, int key, bool live, bool* last) {
+  const int lane = threadIdx.x % kWave;
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    const unsigned long long o = __shfl_up(v, d, kWave);
+    const int ko = __shfl_up(key, d, kWave);
+    if (lane >= d && ko == key) v += o;
+  }
+  const int kn = __shfl_down(key, 1, kWave);
+  const bool ln = __shfl_down(live ? 1 : 0, 1, kWave) != 0;
+  *last = live && (lane == kWave - 1 || kn != key || !ln);
+  return v;
 }
 
 // Zero the accumulators, the column histogram, the per-row products and the
@@ -66,39 +105,37 @@ __global__ void k_desc_clear(int32_t n, unsigned long long* __restrict__ acc, in
 // bandwidth, the column histogram, and each row's order (notasc[i] = 1 on a
 // descent or a repeated column).
 __global__ void __launch_bounds__(256) k_desc_entries(int32_t n, int64_t nnz, const int32_t* __restrict__ ap,
-                                                      const int32_t* __restrict__ ac, int32_t* __restrict__ cnt,
+                                                      const int32_t* __restrict__ ac,
+                                                      const int32_t* __restrict__ crow, int32_t* __restrict__ cnt,
                                                       unsigned long long* __restrict__ rowf,
                                                       uint8_t* __restrict__ notasc,
                                                       unsigned long long* __restrict__ acc) {
   __shared__ unsigned long long red[2][4];
-  const int64_t e0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kDescRun;
+  __shared__ int32_t rowid[kDescChunk];
+  const int64_t e0 = (int64_t)blockIdx.x * kDescChunk;
+  const int64_t e1 = e0 + kDescChunk < nnz ? e0 + kDescChunk : nnz;
+  desc_rowids(ap, crow[blockIdx.x], crow[blockIdx.x + 1], e0, e1, rowid);
   unsigned long long prod = 0, bw = 0;
-  if (e0 < nnz) {
-    const int64_t e1 = e0 + kDescRun < nnz ? e0 + kDescRun : nnz;
-    int32_t r = desc_row_of(ap, n, e0);
-    int64_t rend = ap[r + 1];
-    unsigned long long run = 0;  // products of row r among this thread's entries
-    int prev = -1;
-    for (int64_t e = e0; e < e1; e++) {
-      if (e >= rend) {
-        if (run) atomicAdd(&rowf[r], run);
-        run = 0;
-        prev = -1;
-        while (e >= rend) rend = ap[++r + 1];
-      }
+  // a fixed trip count per wave keeps the segmented sums converged
+  for (int64_t b = e0; b < e1; b += blockDim.x) {
+    const int64_t e = b + threadIdx.x;
+    const bool live = e < e1;
+    unsigned long long f = 0;
+    int32_t r = -1;
+    if (live) {
+      r = rowid[e - e0];
       const int c = ac[e];
-      const unsigned long long f = (unsigned long long)(ap[c + 1] - ap[c]);
-      run += f;
-      prod += f;
+      f = (unsigned long long)(ap[c + 1] - ap[c]);
       const long long d = (long long)c - r;
       const unsigned long long ad = (unsigned long long)(d < 0 ? -d : d);
       bw = ad > bw ? ad : bw;
-      // order across this thread's entries, and with the next thread's first
-      if ((prev >= 0 && c <= prev) || (e + 1 == e1 && e + 1 < rend && ac[e + 1] <= c)) notasc[r] = 1;
-      prev = c;
+      if (e + 1 < (int64_t)ap[r + 1] && ac[e + 1] <= c) notasc[r] = 1;
       atomicAdd(&cnt[c], 1);
+      prod += f;
     }
-    if (run) atomicAdd(&rowf[r], run);
+    bool last;
+    const unsigned long long seg = seg_sum(f, r, live, &last);
+    if (last) atomicAdd(&rowf[r], seg);
   }
 #pragma unroll
   for (int m = 32; m > 0; m >>= 1) {
@@ -123,10 +160,16 @@ __global__ void __launch_bounds__(256) k_desc_entries(int32_t n, int64_t nnz, co
 // Merged value of position (r, c): the stored values of row r at column c,
 // summed in storage order; *found says whether any is stored.  A strictly
 // ascending row holds at most one, found by binary search.
+// guess: a position to try first (stencil-like patterns put (r, c) at the
+// mirror of (c, r)'s offset in its row).
 __device__ __forceinline__ double merged_value(const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
                                                const double* __restrict__ av, bool asc, int32_t r, int32_t c,
-                                               bool* found) {
+                                               int64_t guess, bool* found) {
   int lo = ap[r], hi = ap[r + 1];
+  if (asc && guess >= lo && guess < hi && ac[guess] == c) {
+    *found = true;
+    return av[guess];
+  }
   if (asc) {
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
@@ -152,22 +195,22 @@ __device__ __forceinline__ double merged_value(const int32_t* __restrict__ ap, c
   return v;
 }
 
-// Per row (threads below n): sum of d^2 (with its carry word), longest row,
-// rows out of order, largest row products, sum of squared column counts.
-// Per entry (kDescRun a thread): psym -- each position (i, j) of P, the first
-// stored occurrence of column j in row i kept when the row's merged value
-// there is not zero, counts in |P|, and in |P & P^T| when (j, i) is in P too.
-// One atomic per accumulator and block.
+// Per row (grid-stride): sum of d^2 (with its carry word), longest row, rows
+// out of order, largest row products, sum of squared column counts.  Per
+// entry (block chunks as above): psym -- each position (i, j) of P, the
+// first stored occurrence of column j in row i kept when the row's merged
+// value there is not zero, counts in |P|, and in |P & P^T| when (j, i) is in
+// P too.  One atomic per accumulator and block.
 __global__ void __launch_bounds__(256) k_desc_finish(int32_t n, int64_t nnz, const int32_t* __restrict__ ap,
                                                      const int32_t* __restrict__ ac, const double* __restrict__ av,
-                                                     const int32_t* __restrict__ cnt,
+                                                     const int32_t* __restrict__ crow, const int32_t* __restrict__ cnt,
                                                      const unsigned long long* __restrict__ rowf,
                                                      const uint8_t* __restrict__ notasc,
                                                      unsigned long long* __restrict__ acc) {
   __shared__ unsigned long long red[7][4];
+  __shared__ int32_t rowid[kDescChunk];
   unsigned long long sd2 = 0, mx = 0, uns = 0, pmax = 0, csq = 0, hits = 0, pat = 0;
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  for (int64_t i = gid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     const unsigned long long d = (unsigned long long)(ap[i + 1] - ap[i]);
     sd2 += d * d;
     mx = d > mx ? d : mx;
@@ -176,19 +219,18 @@ __global__ void __launch_bounds__(256) k_desc_finish(int32_t n, int64_t nnz, con
     const unsigned long long c = (unsigned long long)cnt[i];
     csq += c * c;
   }
-  const int64_t e0 = gid * kDescRun;
-  if (e0 < nnz) {
-    const int64_t e1 = e0 + kDescRun < nnz ? e0 + kDescRun : nnz;
-    int32_t r = desc_row_of(ap, n, e0);
-    int64_t rend = ap[r + 1];
-    for (int64_t e = e0; e < e1; e++) {
-      while (e >= rend) rend = ap[++r + 1];
+  const int64_t e0 = (int64_t)blockIdx.x * kDescChunk;
+  const int64_t e1 = e0 + kDescChunk < nnz ? e0 + kDescChunk : nnz;
+  if (e0 < e1) {
+    desc_rowids(ap, crow[blockIdx.x], crow[blockIdx.x + 1], e0, e1, rowid);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+      const int32_t r = rowid[e - e0];
       const int32_t j = ac[e];
       double v;
       if (notasc[r] == 0) {
         v = av[e];
       } else {
-        const int s = ap[r];
+        const int s = ap[r], t = ap[r + 1];
         bool first = true;
         for (int q = s; q < (int)e; q++)
           if (ac[q] == j) {
@@ -197,13 +239,14 @@ __global__ void __launch_bounds__(256) k_desc_finish(int32_t n, int64_t nnz, con
           }
         if (!first) continue;
         v = 0.0;
-        for (int q = (int)e; q < (int)rend; q++)
+        for (int q = (int)e; q < t; q++)
           if (ac[q] == j) v += av[q];
       }
       if (!(v != 0.0)) continue;  // NaN stays in the pattern, as (A != 0) keeps it
       pat++;
       bool found;
-      const double w = merged_value(ap, ac, av, notasc[j] == 0, j, r, &found);
+      const int64_t mirror = (int64_t)ap[j] + ((int64_t)ap[r + 1] - 1 - e);
+      const double w = merged_value(ap, ac, av, notasc[j] == 0, j, r, mirror, &found);
       if (found && w != 0.0) hits++;
     }
   }
@@ -257,13 +300,17 @@ void Engine::descriptors(const appt_csr_view& A, appt_op op, appt_descriptors* o
   auto* rowf = as_ull(ensure(ws_.rowf, sizeof(unsigned long long) * (size_t)n));
   const int64_t rb = std::min<int64_t>(((int64_t)n + 255) / 256, 4096);
   hipLaunchKernelGGL(k_desc_clear, dim3((unsigned)rb), dim3(256), 0, s, n, acc, cnt, rowf, notasc);
-  // one thread per kDescRun entries, and at least one per row for the row statistics
-  const int64_t eb = (nnz + 256 * kDescRun - 1) / (256 * kDescRun);
-  if (nnz > 0)
-    hipLaunchKernelGGL(k_desc_entries, dim3((unsigned)eb), dim3(256), 0, s, n, nnz, A.rowptr, A.col, cnt, rowf,
-                       notasc, acc);
+  // a block per kDescChunk entries, and enough threads for the row statistics
+  const int64_t eb = (nnz + kDescChunk - 1) / kDescChunk;
+  int32_t* crow = (int32_t*)ensure(ws_.cur, sizeof(int32_t) * (size_t)(eb + 1));
+  if (nnz > 0) {
+    hipLaunchKernelGGL(k_desc_chunks, dim3((unsigned)std::min<int64_t>((eb + 256) / 256, 1024)), dim3(256), 0, s, n,
+                       nnz, A.rowptr, eb, crow);
+    hipLaunchKernelGGL(k_desc_entries, dim3((unsigned)eb), dim3(256), 0, s, n, nnz, A.rowptr, A.col, crow, cnt,
+                       rowf, notasc, acc);
+  }
   hipLaunchKernelGGL(k_desc_finish, dim3((unsigned)std::max(eb, rb)), dim3(256), 0, s, n, nnz, A.rowptr, A.col,
-                     A.val, cnt, rowf, notasc, acc);
+                     A.val, crow, cnt, rowf, notasc, acc);
   APPT_HIP(hipGetLastError());
   // one copy into the engine's pinned staging, one synchronisation
   static_assert(sizeof(HostStats::sym) >= sizeof(unsigned long long) * kDescWords, "staging too small");
