@@ -39,12 +39,6 @@ def generate(specs, workers: int = 16) -> list:
         return list(ex.map(lambda f: f(), specs))
 
 
-def batch_mix(count: int = 4096, seed: int = 1, nmin: int = 64, nmax: int = 16384):
-    """The uneven batch (batch_specs) generated: returns (matrices, ops)."""
-    specs, ops = batch_specs(count, seed, nmin, nmax)
-    return generate(specs), ops
-
-
 def serve_job(rng: np.random.Generator, nmin: int = 1000, nmax: int = 2_000_000, seed: int = 0):
     """One job of the served stream: n log-uniform in [nmin, nmax] drawn
     from the Laplacian / 27-point stencil / R-MAT families (grid sides from
