@@ -360,7 +360,12 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
     bptr = tp;
   }
   if (op == APPT_OP_AATS) launch_rows_strict(n, A.rowptr, A.col, ctl + 4, s);
-  stage_bmeta(W, O, bptr, nullptr, nullptr, 0);
+  O.bptr = bptr;
+  // when row templates will be looked for, their class pass over B's rows
+  // also writes the B-row metadata (a Gram needs it at once)
+  O.bmeta_pending = cfg_.templates && !arena_ && n > 0 && nnz >= cfg_.templates_min_nnz &&
+                    !(op == APPT_OP_AATS && gram_mode(A) != kGramNone);
+  if (!O.bmeta_pending) stage_bmeta(W, O, bptr, nullptr, nullptr, 0);
 }
 
 // control words: see kCtlWords
@@ -381,7 +386,16 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
   const int32_t n = O.n;
   hipStream_t s = stream_;
   const size_t nn = (size_t)std::max(n, 1);
-  const int4* bmeta = as<int4>(W.bmeta.p);
+  // row templates: not in the static scratch (no-allocation calls) nor where
+  // a batch's columns are local to its blocks (cfg.templates off), nor for a
+  // Gram, whose B rows are per-entry suffixes, not matrix rows
+  W.tpl_on = cfg.templates && !arena_ && !O.gram && n > 0 && O.annz >= cfg.templates_min_nnz;
+  // B-row metadata left by the operand stage: the class pass writes it, or
+  // it is built here
+  const bool meta_in_classes = O.bmeta_pending && W.tpl_on;
+  if (O.bmeta_pending && !W.tpl_on) stage_bmeta(W, O, O.bptr, nullptr, nullptr, 0);
+  if (meta_in_classes) (void)ensure(W.bmeta, sizeof(int4) * (size_t)std::max(O.nb, 1));
+  int4* bmeta = as<int4>(W.bmeta.p);
   int64_t* rowf = as<int64_t>(ensure(W.rowf, sizeof(int64_t) * nn));
   int32_t* rowlo = as<int32_t>(ensure(W.rowlo, sizeof(int32_t) * nn));
   int32_t* rowhi = as<int32_t>(ensure(W.rowhi, sizeof(int32_t) * nn));
@@ -395,10 +409,6 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
   int32_t* part = as<int32_t>(ensure(W.statpart, sizeof(int32_t) * (size_t)stats_part_elems() +
                                                       sizeof(unsigned long long) * kStatsMaxBlocks));
   launch_fill3_i32(dstats, (int64_t)stats_ints(), rowcnt, n, rowwords, n, s);
-  // row templates: not in the static scratch (no-allocation calls) nor where
-  // a batch's columns are local to its blocks (cfg.templates off)
-  // nor for a Gram, whose B rows are per-entry suffixes, not matrix rows
-  W.tpl_on = cfg.templates && !arena_ && !O.gram && n > 0 && O.annz >= cfg.templates_min_nnz;
   if (W.tpl_on) {
     TplArgs& T = W.tpl;
     const size_t nb = (size_t)std::max(O.nb, 1);
@@ -420,7 +430,8 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
     T.pos2 = as<uint32_t>(ensure(W.tpos2, sizeof(uint32_t) * (size_t)kTplSlots * 2 * kTplMaxLen * kTplPairs));
     T.matched = as<int32_t>(ensure(W.tmatched, sizeof(int32_t) * 2));
     T.row0 = O.row0;
-    launch_tpl_detect(T, n, O.ap, O.ac, O.nb, bmeta, O.bc, a_is_b, s);
+    launch_tpl_detect(T, n, O.ap, O.ac, O.nb, bmeta, O.bc, a_is_b, meta_in_classes ? O.bptr : nullptr,
+                      as<int32_t>(W.ctl.p) + 2, s);
   }
   launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part,
                  W.tpl_on ? &W.tpl : nullptr, rowcnt, rowwords, s);

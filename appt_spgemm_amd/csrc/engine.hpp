@@ -138,6 +138,10 @@ struct Operands {
   bool a_is_b = false;  // the A-side rows are B's rows (A*A over all rows)
   bool gram = false;    // AAtS as a triangle-only Gram: B rows are per-entry column ranges (gram.hip)
   bool gram_lower = false;  // the Gram's triangle: j <= i (else j >= i)
+  // B's row pointers; bmeta_pending: the B-row metadata is left to the
+  // pre-pass (the template class pass writes it from the same loads)
+  const int32_t* bptr = nullptr;
+  bool bmeta_pending = false;
 };
 
 // Bin statistics read back at the two host synchronisations; they fix the

@@ -151,9 +151,14 @@ __device__ __forceinline__ unsigned long long entry_hash(int u, int64_t rel) {
 // Pattern class of each row: hash of its relative pattern, inserted into the
 // class table; the row that inserts a key is the class representative (any
 // row of the class would do: every other row is compared with it exactly).
+// META (B rows through their row pointers, R.ap): also each row's metadata
+// {start, end, min column, max column} for rows of at most kTplMaxLen
+// entries, and *flag = 1 on a descending row, from the same loads.
+template <bool META>
 __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, const int32_t* __restrict__ col,
                                                       unsigned long long* __restrict__ keys,
-                                                      int32_t* __restrict__ rep, int32_t* __restrict__ cls) {
+                                                      int32_t* __restrict__ rep, int32_t* __restrict__ cls,
+                                                      int4* __restrict__ meta, int32_t* __restrict__ flag) {
   constexpr int G = kClsLanes;
   __shared__ KeyCache kc;
   cache_init(kc);
@@ -170,15 +175,34 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
     const bool ok = r < nrows && m > 0 && m <= kTplMaxLen;
     const int64_t b = ok ? R.base((int)r) : 0;
     unsigned long long h = 0;
+    int mn = INT32_MAX, mx = -1;
+    bool desc = false;
     if (ok)
       for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
-        int c[kClsUnroll];
+        int c[kClsUnroll], nx[kClsUnroll];
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++) c[x] = u0 + x * G < m ? col[se.x + u0 + x * G] : 0;
+        if (META) {
+#pragma unroll
+          for (int x = 0; x < kClsUnroll; x++) nx[x] = u0 + x * G + 1 < m ? col[se.x + u0 + x * G + 1] : INT32_MAX;
+        }
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++)
-          if (u0 + x * G < m) h += entry_hash(u0 + x * G, (int64_t)c[x] - b);
+          if (u0 + x * G < m) {
+            h += entry_hash(u0 + x * G, (int64_t)c[x] - b);
+            if (META) {
+              mn = c[x] < mn ? c[x] : mn;
+              mx = c[x] > mx ? c[x] : mx;
+              desc |= nx[x] < c[x];
+            }
+          }
       }
+    if (META) {
+      mn = gmin<G>(mn);
+      mx = gmax<G>(mx);
+      if (lane == 0 && r < nrows && m <= kTplMaxLen) meta[r] = make_int4(se.x, se.y, mn, mx);
+      if (__ballot(desc) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
+    }
     h = gsum<G, unsigned long long>(h, nullptr);
     h = (h + tmix((unsigned long long)m)) | 1ull;
     bool mine;
@@ -419,7 +443,8 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
 }  // namespace
 
 void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb,
-                       const int4* bmeta, const int32_t* bc, bool a_is_b, hipStream_t s) {
+                       int4* bmeta, const int32_t* bc, bool a_is_b, const int32_t* bptr, int32_t* bflag,
+                       hipStream_t s) {
   // tables empty (a representative is written by the row that inserts its key)
   launch_fill_i32(reinterpret_cast<int32_t*>(T.ckey), 2 * kClassSlots, 0, s);
   launch_fill_i32(reinterpret_cast<int32_t*>(T.tkey), 2 * kTplSlots, 0, s);
@@ -430,7 +455,17 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
   };
   const Rows RB{nullptr, bmeta, 0};
   if (nb > 0) {
-    hipLaunchKernelGGL(k_class_insert, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep, T.bclass);
+    if (bptr) {
+      // B's row metadata from the class pass (rows of at most kTplMaxLen
+      // entries) and a pass of its own for the longer rows
+      const Rows RP{bptr, nullptr, 0};
+      hipLaunchKernelGGL(k_class_insert<true>, dim3(grid(nb)), dim3(256), 0, s, nb, RP, bc, T.ckey, T.crep,
+                         T.bclass, bmeta, bflag);
+      launch_rowstats(nb, bptr, nullptr, nullptr, 0, bc, bmeta, bflag, s, kTplMaxLen);
+    } else {
+      hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep,
+                         T.bclass, nullptr, nullptr);
+    }
     hipLaunchKernelGGL(k_class_verify, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.crep, T.bclass, T.matched);
   }
   if (n <= 0) return;
@@ -439,7 +474,8 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
     // whose representatives are A-side rows
     launch_fill_i32(reinterpret_cast<int32_t*>(T.akey), 2 * kClassSlots, 0, s);
     const Rows RA{ap, nullptr, T.row0};
-    hipLaunchKernelGGL(k_class_insert, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.akey, T.arep, T.aclass);
+    hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.akey, T.arep, T.aclass,
+                       nullptr, nullptr);
     hipLaunchKernelGGL(k_class_verify, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.arep, T.aclass, T.matched + 1);
   }
   hipLaunchKernelGGL(k_tpl_insert, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.aclass, T.bclass, T.tkey, T.trep,
