@@ -1049,7 +1049,7 @@ __global__ void k_numbin(int32_t n, const int32_t* __restrict__ ap, const int64_
       const int t = tplid ? tplid[i] : -1;
       if (t >= 0) {
         b = kBinTemplate;
-        stat = tinfo[t].w;  // longest B row: lanes per row
+        stat = tinfo[t].w;  // widest A or B row of the template: lanes per row
       } else {
         b = num_bin_of(P, c, words, rw & kWordsMask, rowf[i], nA, listed, &stat);
       }

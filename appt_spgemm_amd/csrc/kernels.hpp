@@ -238,7 +238,7 @@ struct TplArgs {
   unsigned long long* tkey;  // template table: kTplSlots keys
   int32_t* trep;             // template representative (A-side row)
   int32_t* tplid;            // per A-side row: template slot, or -1
-  int4* info;                // per template: {valid, A entries m, slots nS, longest B row}
+  int4* info;                // per template: {valid, A entries m, slots nS, widest of the A row and its B rows}
   int32_t* posoff;           // per template: kTplMaxLen + 1 product offsets of the A entries
   uint16_t* pos;             // per template: kTplMaxProducts output slots of the products
   int32_t* soff;             // per template: kTplMaxProducts output offsets (column - row), ascending
@@ -253,9 +253,10 @@ constexpr int kTplPairs = kTplMaxLen / 2 + 1;  // B-value pairs of one B row (ei
 __host__ __device__ inline int64_t tpl_group_bytes(int64_t nsmax) {
   return (int64_t)kTplMaxLen * 24 + ((nsmax * 8 + 15) & ~(int64_t)15);
 }
-// The same for the pair-load kernel: values and {B-row start, length}.
+// The same for the pair-load kernel (rows of at most 31 entries): values and
+// {B-row start, length} of 32 entries, then nS f64 slots.
 __host__ __device__ inline int64_t tpl_pair_group_bytes(int64_t nsmax) {
-  return (int64_t)kTplMaxLen * 16 + ((nsmax * 8 + 15) & ~(int64_t)15);
+  return (int64_t)32 * 16 + ((nsmax * 8 + 15) & ~(int64_t)15);
 }
 // Classes of B's rows (and of A's rows unless a_is_b), template candidates
 // of A's rows (tplid, pending the pre-pass's check against the template's

@@ -437,7 +437,9 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
     const uint32_t s1 = (v1 >= 0 && v1 < len) ? rank_of[off[u] + v1] : 0xffffu;
     p2[((size_t)par * kTplMaxLen + u) * kTplPairs + l] = s0 | (s1 << 16);
   }
-  if (tid == 0) info[t] = make_int4(1, m, base, maxb_s);
+  // .w: the widest of the A row and its B rows (sizes the numeric kernel's
+  // lanes per row and its staged entries)
+  if (tid == 0) info[t] = make_int4(1, m, base, maxb_s > m ? maxb_s : m);
 }
 
 }  // namespace
@@ -612,20 +614,46 @@ __global__ void __launch_bounds__(256) k_tpl_num(RowArgs A, TplArgs T, int32_t n
   }
 }
 
-// Template rows whose B rows have at most kTplPairMax entries: G = 16 lanes
-// per row (four rows per wavefront), each lane gathering an aligned pair of
-// B values with one 16-byte load.  B row k_u starting at position bs is read
-// as pairs from bs & ~1, so lane l holds entries v = 2l - par and 2l - par + 1
-// (par = bs & 1); the template keeps the slots of both for each parity
-// (pos2).  Otherwise as k_tpl_num: entries u in storage order, their gathers
-// issued kTplFast at a time, LDS f64 atomics in u order.
+// Template rows no wider than kTplPairMax (A row and every B row): G = 16
+// lanes per row (four rows per wavefront), each lane gathering an aligned
+// pair of B values with one 16-byte load.  B row k_u starting at position bs
+// is read as pairs from bs & ~1, so lane l holds entries v = 2l - par and
+// 2l - par + 1 (par = bs & 1); the template keeps the slots of both for each
+// parity (pos2).  Entries u are taken in storage order, their gathers issued
+// kTplFast at a time with no branch around them (dead lanes load entry 0),
+// and the LDS f64 atomics follow in u order.  A lane holds two of the row's
+// A entries (lane, lane + G) through the four-deep row pipeline, which also
+// brings the row's output offset and slot count.  The write-out takes four
+// slots per lane (16-byte stores) and clears them for the next row.
 constexpr int kTplPairG = 16;
 constexpr int kTplPairMax = 2 * kTplPairG - 1;
+constexpr int kTplPairLen = 2 * kTplPairG;   // A entries a group stages (<= kTplPairMax)
+
+// Two consecutive B values from entry idx on (8-byte aligned).
+typedef double dpair __attribute__((ext_vector_type(2), aligned(8)));
+template <bool W64>
+__device__ __forceinline__ dpair load_pair(const double* base, uint32_t idx) {
+  if constexpr (W64) return *(const dpair*)(base + idx);
+  else return *(const dpair*)((const char*)base + (size_t)(idx * 8u));
+}
+
+// 16-byte access to int32 data that is only 4-byte aligned.
+typedef int32_t i32x4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ int4 load_i4(const int32_t* p) {
+  const i32x4u v = *(const i32x4u*)p;
+  return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void store_i4(int32_t* p, int4 v) {
+  *(i32x4u*)p = i32x4u{v.x, v.y, v.z, v.w};
+}
 
 struct TplRow2 {
   int i, t, s, len;
-  int k, bs;
-  double a;
+  int k0, k1;        // B rows of entries lane and lane + G (-1: none); after hop3 their lengths
+  int bs0, bs1;      // B-row starts
+  double a0, a1;     // a_iu of those entries
+  int64_t off;       // first output position of the row
+  int nS;            // the template's slot count
 };
 
 template <bool W64>
@@ -637,35 +665,50 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   const int R = blockDim.x / G;
   const int gi = threadIdx.x / G, lane = threadIdx.x % G;
   unsigned char* base = lds_raw + (size_t)gi * (size_t)tpl_pair_group_bytes(nsmax);
-  double* sa = (double*)base;                  // a_iu
-  int2* se = (int2*)(base + kTplMaxLen * 8);   // {B-row start, B-row length}
-  double* vals = (double*)(base + kTplMaxLen * 16);
+  double* sa = (double*)base;                      // a_iu
+  int2* se = (int2*)(base + kTplPairLen * 8);      // {B-row start, B-row length}
+  double* vals = (double*)(base + kTplPairLen * 16);
   const RowWalk walk = row_walk(A.nrows, R, gi);
   const int S = walk.step, r1 = walk.end;
   const uint32_t bnnz = (uint32_t)A.bnnz;
+  for (int q = lane; q < nsmax; q += G) vals[q] = 0.0;
+  gsync<G>();
+
   auto hop1 = [&](TplRow2& q) {
     q.t = T.tplid[q.i];
     q.s = A.ap[q.i];
     q.len = A.ap[q.i + 1] - q.s;
   };
   auto hop2 = [&](TplRow2& q) {
-    q.k = -1;
+    q.k0 = q.k1 = -1;
     if (lane < q.len) {
-      q.a = A.av[q.s + lane];
-      q.k = A.ac[q.s + lane];
+      q.a0 = A.av[q.s + lane];
+      q.k0 = A.ac[q.s + lane];
     }
+    if (lane + G < q.len) {
+      q.a1 = A.av[q.s + lane + G];
+      q.k1 = A.ac[q.s + lane + G];
+    }
+    q.off = rowoff[q.i];
+    q.nS = T.info[q.t].z;
   };
   auto hop3 = [&](TplRow2& q) {
-    if (q.k >= 0) {
-      const int4 b = A.bmeta[q.k];
-      q.bs = b.x;
-      q.k = b.y - b.x;   // from here on: the B row's length
+    if (q.k0 >= 0) {
+      const int4 b = A.bmeta[q.k0];
+      q.bs0 = b.x;
+      q.k0 = b.y - b.x;   // from here on: the B row's length
+    }
+    if (q.k1 >= 0) {
+      const int4 b = A.bmeta[q.k1];
+      q.bs1 = b.x;
+      q.k1 = b.y - b.x;
     }
   };
   int r = walk.first;
   TplRow2 q0{}, q1{}, q2{}, q3{};
   q0.len = q1.len = q2.len = q3.len = 0;
-  q0.k = q1.k = q2.k = q3.k = -1;
+  q0.k0 = q1.k0 = q2.k0 = q3.k0 = -1;
+  q0.k1 = q1.k1 = q2.k1 = q3.k1 = -1;
   if (r < r1) { q0.i = A.rows[r]; hop1(q0); hop2(q0); hop3(q0); }
   if (r + S < r1) { q1.i = A.rows[r + S]; hop1(q1); hop2(q1); }
   if (r + 2 * S < r1) { q2.i = A.rows[r + 2 * S]; hop1(q2); }
@@ -673,46 +716,45 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   for (; r < r1; r += S) {
     TplRow2 q4{};
     q4.len = 0;
-    q4.k = -1;
+    q4.k0 = q4.k1 = -1;
     if (r + 4 * S < r1) q4.i = A.rows[r + 4 * S];
     if (r + 3 * S < r1) hop1(q3);
     if (r + 2 * S < r1) hop2(q2);
     if (r + S < r1) hop3(q1);
-    const int i = q0.i, t = q0.t, m = q0.len;
-    const int nS = T.info[t].z;
-    const int64_t off = rowoff[i];
-    const uint32_t* p2 = T.pos2 + (size_t)t * (2 * kTplMaxLen * kTplPairs);
-    for (int q = lane; q < nS; q += G) vals[q] = 0.0;
+    const int t = q0.t, m = q0.len, nS = q0.nS;
     if (lane < m) {
-      sa[lane] = q0.a;
-      se[lane] = make_int2(q0.bs, q0.k);
+      sa[lane] = q0.a0;
+      se[lane] = make_int2(q0.bs0, q0.k0);
     }
-    for (int u = lane + G; u < m; u += G) {   // rows longer than G entries
-      const int4 b = A.bmeta[A.ac[q0.s + u]];
-      sa[u] = A.av[q0.s + u];
-      se[u] = make_int2(b.x, b.y - b.x);
+    if (lane + G < m) {
+      sa[lane + G] = q0.a1;
+      se[lane + G] = make_int2(q0.bs1, q0.k1);
     }
     gsync<G>();
+    const uint32_t* p2 = T.pos2 + (size_t)t * (2 * kTplMaxLen * kTplPairs);
+    const int64_t row = (int64_t)T.row0 + q0.i;
+    const int64_t off = q0.off;
+    const int32_t* so = T.soff + (size_t)t * kTplMaxProducts;
     for (int u0 = 0; u0 < m; u0 += kTplFast) {
       double pv0[kTplFast], pv1[kTplFast];
       uint32_t sl[kTplFast];
 #pragma unroll
       for (int d = 0; d < kTplFast; d++) {
-        const int u = u0 + d < m ? u0 + d : u0;
+        const bool in = u0 + d < m;
+        const int u = in ? u0 + d : u0;
         const int2 x = se[u];
         const int par = x.x & 1;
-        const bool live = u0 + d < m && 2 * lane < x.y + par;
-        const uint32_t q = (uint32_t)((x.x & ~1) + 2 * lane);   // even: 16-byte aligned pair
-        sl[d] = live ? gather<uint32_t, false>(p2, (uint32_t)(((par * kTplMaxLen) + u) * kTplPairs + lane))
-                     : 0xffffffffu;
-        double2 b = make_double2(0.0, 0.0);
-        if (live) {
-          if (q + 1 < bnnz) b = gather<double2, W64>((const double2*)A.bv, q >> 1);
-          else b.x = gather<double, W64>(A.bv, q);   // last pair of B: its second half is past the end
-        }
+        const bool live = in && 2 * lane < x.y + par;
+        const uint32_t q = live ? (uint32_t)((x.x & ~1) + 2 * lane) : 0u;   // even: the pair (q, q + 1)
+        // the last entry of an odd-length B: load the pair ending there
+        const bool tail = q + 1 >= bnnz;
+        const uint32_t v = gather<uint32_t, false>(p2, (uint32_t)(((par * kTplMaxLen) + u) * kTplPairs + lane));
+        sl[d] = live ? v : 0xffffffffu;
+        const dpair raw = load_pair<W64>(A.bv, tail ? q - 1 : q);
+        const double bx = tail ? raw.y : raw.x, by = tail ? 0.0 : raw.y;
         const double au = sa[u];
-        pv0[d] = au * b.x;
-        pv1[d] = au * b.y;
+        pv0[d] = au * bx;
+        pv1[d] = au * by;
       }
 #pragma unroll
       for (int d = 0; d < kTplFast; d++) {
@@ -722,11 +764,27 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
       }
     }
     gsync<G>();
-    const int64_t row = (int64_t)T.row0 + i;
-    const int32_t* so = T.soff + (size_t)t * kTplMaxProducts;
-    for (int q = lane; q < nS; q += G) {
-      ccol[off + q] = (int32_t)(row + so[q]);
-      cval[off + q] = vals[q];
+    // write-out: four consecutive slots per lane (16-byte stores, so a
+    // row takes a third of the store instructions), clearing them
+    const int32_t rowc = (int32_t)row;
+    for (int q = 4 * lane; q < nS; q += 4 * G) {
+      if (q + 4 <= nS) {
+        const int4 o = load_i4(so + q);
+        const dpair v0 = *(const dpair*)&vals[q], v1 = *(const dpair*)&vals[q + 2];
+        *(dpair*)&vals[q] = dpair{0.0, 0.0};
+        *(dpair*)&vals[q + 2] = dpair{0.0, 0.0};
+        store_i4(ccol + off + q, make_int4(rowc + o.x, rowc + o.y, rowc + o.z, rowc + o.w));
+        *(dpair*)(cval + off + q) = v0;
+        *(dpair*)(cval + off + q + 2) = v1;
+      } else {
+        for (int e = q; e < nS; e++) {
+          const int32_t o = so[e];
+          const double v = vals[e];
+          vals[e] = 0.0;
+          ccol[off + e] = rowc + o;
+          cval[off + e] = v;
+        }
+      }
     }
     gsync<G>();
     q0 = q1; q1 = q2; q2 = q3; q3 = q4;
@@ -736,23 +794,30 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
 void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t nsmax, const int64_t* rowoff,
                     int32_t* ccol, double* cval, int num_cus, int64_t lds_per_cu, hipStream_t s) {
   if (a.nrows <= 0) return;
-  if (maxb <= kTplPairMax) {
+  if (maxb <= kTplPairMax && a.bnnz >= 2) {
+    // maxb: the widest A or B row of any template row (pairs of B values
+    // need at least two of them)
     const int64_t gb = tpl_pair_group_bytes(nsmax);
     int R = 256 / kTplPairG;
     while (R > 4 && (int64_t)R * gb * 4 > lds_per_cu) R >>= 1;
-    const size_t lds = (size_t)R * (size_t)gb;
+    const size_t lds = (size_t)((int64_t)R * gb);
+    const bool w64 = a.bnnz >= ((int64_t)1 << 29);
+    const void* fn = w64 ? (const void*)k_tpl_num2<true> : (const void*)k_tpl_num2<false>;
+    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    // persistent blocks: as many as are resident at once (rows are split
+    // among the blocks up front, so a block that waits for a slot is a tail)
+    int fit = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, fn, R * kTplPairG, lds) != hipSuccess || fit < 1) fit = 4;
+    if (fit > 8) fit = 8;
     int64_t blocks = ((int64_t)a.nrows + R - 1) / R;
-    const int64_t cap = (int64_t)num_cus * 8;
+    const int64_t cap = (int64_t)num_cus * fit;
     if (blocks > cap) blocks = cap;
-    if (a.bnnz >= ((int64_t)1 << 29)) {
-      (void)hipFuncSetAttribute((const void*)k_tpl_num2<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (w64)
       hipLaunchKernelGGL(k_tpl_num2<true>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax, rowoff,
                          ccol, cval);
-    } else {
-      (void)hipFuncSetAttribute((const void*)k_tpl_num2<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    else
       hipLaunchKernelGGL(k_tpl_num2<false>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax, rowoff,
                          ccol, cval);
-    }
     return;
   }
   const int64_t gb = tpl_group_bytes(nsmax);
