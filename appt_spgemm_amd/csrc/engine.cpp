@@ -428,7 +428,7 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
     T.soff = as<int32_t>(ensure(W.tsoff, sizeof(int32_t) * (size_t)kTplSlots * kTplMaxProducts));
     T.tuple = as<int32_t>(ensure(W.ttuple, sizeof(int32_t) * (size_t)kTplSlots * kTplMaxLen));
     T.pos2 = as<uint32_t>(ensure(W.tpos2, sizeof(uint32_t) * (size_t)kTplSlots * 2 * kTplMaxLen * kTplPairs));
-    T.matched = as<int32_t>(ensure(W.tmatched, sizeof(int32_t) * 2));
+    T.matched = as<int32_t>(ensure(W.tmatched, sizeof(int32_t) * 4));
     T.row0 = O.row0;
     launch_tpl_detect(T, n, O.ap, O.ac, O.nb, bmeta, O.bc, a_is_b, meta_in_classes ? O.bptr : nullptr,
                       as<int32_t>(W.ctl.p) + 2, s);

@@ -472,11 +472,24 @@ __global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, i
   const int64_t base = tile * kRadixTile;
   const int64_t wbase = base + (int64_t)w * kWave * kRadixChunks;
   for (int d = threadIdx.x; d < 4 * NB; d += blockDim.x) (&wcnt[0][0])[d] = 0;
-  __syncthreads();
-  // (A) per-wave counts
+  // every entry of the wave's chunks is loaded once, up front (all loads in
+  // flight together), and kept in registers through the ranking
+  int32_t kr[kRadixChunks], rr[kRadixChunks];
+  double vr[kRadixChunks];
+#pragma unroll
   for (int c = 0; c < kRadixChunks; c++) {
     const int64_t e = wbase + (int64_t)c * kWave + lane;
-    if (e < nnz) atomicAdd(&wcnt[w][((uint32_t)kin[e] >> shift) & dm], 1);
+    const bool valid = e < nnz;
+    kr[c] = valid ? kin[e] : 0;
+    rr[c] = valid && rin ? rin[e] : 0;
+    vr[c] = valid ? vin[e] : 0.0;
+  }
+  __syncthreads();
+  // (A) per-wave counts
+#pragma unroll
+  for (int c = 0; c < kRadixChunks; c++) {
+    const int64_t e = wbase + (int64_t)c * kWave + lane;
+    if (e < nnz) atomicAdd(&wcnt[w][((uint32_t)kr[c] >> shift) & dm], 1);
   }
   __syncthreads();
   // digit starts in the tile, and each wave's first slot per digit
@@ -501,10 +514,11 @@ __global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, i
   __syncthreads();
   // (B) stable ranks inside the wave, placed in LDS
   const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
   for (int c = 0; c < kRadixChunks; c++) {
     const int64_t e = wbase + (int64_t)c * kWave + lane;
     const bool valid = e < nnz;
-    const int32_t key = valid ? kin[e] : 0;
+    const int32_t key = kr[c];
     const uint32_t digit = ((uint32_t)key >> shift) & dm;
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < bits; b++) {
@@ -519,8 +533,8 @@ __global__ void __launch_bounds__(256) k_radix_scatter(int64_t nnz, int shift, i
     if (valid) {
       const int ps = slot + (slot >> 5);
       sk[ps] = key;
-      sr[ps] = rin ? rin[e] : 0;
-      sv[ps] = vin[e];
+      sr[ps] = rr[c];
+      sv[ps] = vr[c];
     }
     gsync<kWave>();
   }
@@ -3298,20 +3312,39 @@ __global__ void k_mirror_counts(int32_t n, const int64_t* __restrict__ off, cons
     upper[i] = (int32_t)(e - lo - diag);
   }
 }
-__global__ void k_mirror_extract(int32_t n, const int64_t* __restrict__ off, const int32_t* __restrict__ ccol,
-                                 const double* __restrict__ cval, const int32_t* __restrict__ upper,
-                                 const int64_t* __restrict__ up_off, int32_t* __restrict__ keys,
-                                 double* __restrict__ vals) {
-  constexpr int G = 16;
-  const int lane = threadIdx.x % G;
-  const int64_t gpb = blockDim.x / G;
-  for (int64_t j = (int64_t)blockIdx.x * gpb + threadIdx.x / G; j < n; j += (int64_t)gridDim.x * gpb) {
-    const int u = upper[j];
-    const int64_t src = off[j + 1] - u, dst = up_off[j];
-    for (int t = lane; t < u; t += G) {
-      keys[dst + t] = ccol[src + t];
-      vals[dst + t] = cval[src + t];
-    }
+// The strictly-upper entries, compacted in row-major order: entry-parallel
+// over the compacted positions (a block per chunk of them), so that long
+// rows spread over many blocks.  Position q belongs to the last row j with
+// up_off[j] <= q; a block finds the rows of its chunk's ends, and each
+// thread searches between them.
+constexpr int64_t kMirrorExtractChunk = 4096;
+__device__ __forceinline__ int32_t last_row_at_most(const int64_t* __restrict__ pre, int32_t lo, int32_t hi,
+                                                    int64_t q) {
+  // largest j in [lo, hi] with pre[j] <= q (pre[lo] <= q holds)
+  while (lo < hi) {
+    const int32_t mid = lo + (hi - lo + 1) / 2;
+    if (pre[mid] <= q) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+__global__ void __launch_bounds__(256) k_mirror_extract(int32_t n, int64_t count, const int64_t* __restrict__ off,
+                                                        const int32_t* __restrict__ ccol,
+                                                        const double* __restrict__ cval,
+                                                        const int32_t* __restrict__ upper,
+                                                        const int64_t* __restrict__ up_off,
+                                                        int32_t* __restrict__ keys, double* __restrict__ vals) {
+  __shared__ int32_t ends[2];
+  const int64_t c0 = (int64_t)blockIdx.x * kMirrorExtractChunk;
+  const int64_t c1 = c0 + kMirrorExtractChunk < count ? c0 + kMirrorExtractChunk : count;
+  if (threadIdx.x < 2) ends[threadIdx.x] = last_row_at_most(up_off, 0, n - 1, threadIdx.x == 0 ? c0 : c1 - 1);
+  __syncthreads();
+  const int32_t rlo = ends[0], rhi = ends[1];
+  for (int64_t q = c0 + threadIdx.x; q < c1; q += blockDim.x) {
+    const int32_t j = last_row_at_most(up_off, rlo, rhi, q);
+    const int64_t src = off[j + 1] - upper[j] + (q - up_off[j]);
+    keys[q] = ccol[src];
+    vals[q] = cval[src];
   }
 }
 __global__ void k_mirror_place(int64_t count, const int32_t* __restrict__ keys, const double* __restrict__ vals,
@@ -3337,10 +3370,9 @@ void launch_mirror_radix(int32_t n, int64_t count, const int64_t* off, const int
   const bool odd = radix_passes(n) % 2 == 1;
   int32_t* ek = odd ? rx.keys : rx.keys2;
   double* ev = odd ? rx.vals : vals2;
-  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
-  if (blocks > 16384) blocks = 16384;
-  hipLaunchKernelGGL(k_mirror_extract, dim3((unsigned)blocks), dim3(256), 0, s, n, off, ccol, (const double*)cval,
-                     upper, up_off, ek, ev);
+  const int64_t blocks = (count + kMirrorExtractChunk - 1) / kMirrorExtractChunk;
+  hipLaunchKernelGGL(k_mirror_extract, dim3((unsigned)blocks), dim3(256), 0, s, n, count, off, ccol,
+                     (const double*)cval, upper, up_off, ek, ev);
   radix_passes_run(n, count, ek, nullptr, ev, rx.keys, nullptr, rx.vals, rx.keys2, nullptr, vals2, rx.hist, rx.hoff,
                    rx.htmp, nullptr, s);
   int64_t b2 = (count + 255) / 256;

@@ -244,7 +244,8 @@ struct TplArgs {
   int32_t* soff;             // per template: kTplMaxProducts output offsets (column - row), ascending
   int32_t* tuple;            // per template: kTplMaxLen B classes of its representative's entries
   uint32_t* pos2;            // per template: [parity][entry u][pair l] slots of B entries 2l - par, 2l - par + 1
-  int32_t* matched;          // [0]: B rows whose pattern repeats another's; [1]: the same for A-side rows
+  int32_t* matched;          // [0]: B rows whose pattern repeats another's; [1]: the same for A-side rows;
+                             // [2], [3]: distinct patterns among sampled B rows, and the samples
   int32_t row0;              // global row index of A-side row 0 (row panels)
 };
 constexpr int kTplPairs = kTplMaxLen / 2 + 1;  // B-value pairs of one B row (either parity)

@@ -148,6 +148,71 @@ __device__ __forceinline__ unsigned long long entry_hash(int u, int64_t rel) {
   return tmix(((unsigned long long)(u + 1) << 32) ^ (unsigned long long)(uint32_t)(int32_t)rel);
 }
 
+// Whether B's row patterns repeat at all, from a sample: kSampleBlocks blocks
+// hash the relative patterns of kSampleRows rows each (spread over B; one
+// row per four lanes, so a block takes its rows in one step) into an LDS set and count the distinct ones into matched[2] (samples into
+// matched[3]).  When more than three quarters of the sampled rows are
+// distinct, the class and template passes skip their hashing and table
+// probes: no template can pay (a quarter of B's rows must repeat another's
+// pattern), and a table that fills with distinct keys costs more than
+// anything else in the call.
+constexpr int kSampleBlocks = 64, kSampleRows = 64, kSampleSlots = 256;
+__global__ void __launch_bounds__(256) k_class_sample(int32_t nrows, Rows R, const int32_t* __restrict__ col,
+                                                      int32_t* __restrict__ matched) {
+  constexpr int G = kClsLanes;
+  __shared__ unsigned long long set[kSampleSlots];
+  __shared__ int32_t distinct, taken;
+  for (int t = threadIdx.x; t < kSampleSlots; t += blockDim.x) set[t] = 0ull;
+  if (threadIdx.x == 0) distinct = taken = 0;
+  __syncthreads();
+  const int lane = threadIdx.x % G;
+  const int64_t total = (int64_t)kSampleBlocks * kSampleRows;
+  for (int j0 = 0; j0 < kSampleRows; j0 += (int)blockDim.x / G) {
+    const int64_t j = (int64_t)blockIdx.x * kSampleRows + j0 + threadIdx.x / G;   // sample index
+    const int64_t r = nrows < total ? (j < nrows ? j : -1) : j * nrows / total;
+    int2 se = make_int2(0, 0);
+    if (r >= 0) se = R.range((int)r);
+    const int m = se.y - se.x;
+    const bool ok = r >= 0 && m > 0 && m <= kTplMaxLen;
+    const int64_t b = ok ? R.base((int)r) : 0;
+    unsigned long long h = 0;
+    if (ok) {
+      // a row has at most kTplMaxLen entries: all of a lane's loads at once
+      constexpr int U = kTplMaxLen / G;
+      int c[U];
+#pragma unroll
+      for (int x = 0; x < U; x++) c[x] = lane + x * G < m ? col[se.x + lane + x * G] : 0;
+#pragma unroll
+      for (int x = 0; x < U; x++)
+        if (lane + x * G < m) h += entry_hash(lane + x * G, (int64_t)c[x] - b);
+    }
+    h = gsum<G, unsigned long long>(h, nullptr);
+    h = (h + tmix((unsigned long long)m)) | 1ull;
+    if (lane == 0 && r >= 0) {
+      atomicAdd(&taken, 1);
+      bool fresh = !ok;   // rows without a class count as distinct
+      if (ok) {
+        int slot = (int)(h & (kSampleSlots - 1));
+        for (int p = 0; p < kSampleSlots; p++) {
+          const unsigned long long old = atomicCAS(&set[slot], 0ull, h);
+          if (old == 0ull) { fresh = true; break; }
+          if (old == h) break;
+          slot = (slot + 1) & (kSampleSlots - 1);
+        }
+      }
+      if (fresh) atomicAdd(&distinct, 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&matched[2], distinct);
+    atomicAdd(&matched[3], taken);
+  }
+}
+__device__ __forceinline__ bool patterns_distinct(const int32_t* matched) {
+  return (int64_t)matched[2] * 4 > (int64_t)matched[3] * 3;
+}
+
 // Pattern class of each row: hash of its relative pattern, inserted into the
 // class table; the row that inserts a key is the class representative (any
 // row of the class would do: every other row is compared with it exactly).
@@ -158,10 +223,13 @@ template <bool META>
 __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, const int32_t* __restrict__ col,
                                                       unsigned long long* __restrict__ keys,
                                                       int32_t* __restrict__ rep, int32_t* __restrict__ cls,
-                                                      int4* __restrict__ meta, int32_t* __restrict__ flag) {
+                                                      int4* __restrict__ meta, int32_t* __restrict__ flag,
+                                                      const int32_t* __restrict__ matched) {
   constexpr int G = kClsLanes;
   __shared__ KeyCache kc;
   cache_init(kc);
+  // sampled patterns mostly distinct: no classes (the metadata still)
+  const bool off = patterns_distinct(matched);
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
   const int64_t stride = (int64_t)gridDim.x * gpb;
@@ -172,12 +240,13 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
     int2 se = make_int2(0, 0);
     if (r < nrows) se = R.range((int)r);
     const int m = se.y - se.x;
-    const bool ok = r < nrows && m > 0 && m <= kTplMaxLen;
+    const bool ok = r < nrows && m > 0 && m <= kTplMaxLen && !off;
+    const bool scan = r < nrows && m > 0 && m <= kTplMaxLen && (META || !off);
     const int64_t b = ok ? R.base((int)r) : 0;
     unsigned long long h = 0;
     int mn = INT32_MAX, mx = -1;
     bool desc = false;
-    if (ok)
+    if (scan)
       for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
         int c[kClsUnroll], nx[kClsUnroll];
 #pragma unroll
@@ -189,7 +258,7 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++)
           if (u0 + x * G < m) {
-            h += entry_hash(u0 + x * G, (int64_t)c[x] - b);
+            if (!off) h += entry_hash(u0 + x * G, (int64_t)c[x] - b);
             if (META) {
               mn = c[x] < mn ? c[x] : mn;
               mx = c[x] > mx ? c[x] : mx;
@@ -202,6 +271,10 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
       mx = gmax<G>(mx);
       if (lane == 0 && r < nrows && m <= kTplMaxLen) meta[r] = make_int4(se.x, se.y, mn, mx);
       if (__ballot(desc) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
+    }
+    if (off) {
+      if (lane == 0 && r < nrows) cls[r] = -1;
+      continue;
     }
     h = gsum<G, unsigned long long>(h, nullptr);
     h = (h + tmix((unsigned long long)m)) | 1ull;
@@ -220,8 +293,9 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
 // counted into *matched (one atomic per block).
 __global__ void __launch_bounds__(256) k_class_verify(int32_t nrows, Rows R, const int32_t* __restrict__ col,
                                                       const int32_t* __restrict__ rep, int32_t* __restrict__ cls,
-                                                      int32_t* __restrict__ matched) {
+                                                      int32_t* __restrict__ matched, const int32_t* __restrict__ gate) {
   constexpr int G = kClsLanes;
+  if (patterns_distinct(gate)) return;   // every row is classless already
   __shared__ int32_t block_matched;
   if (threadIdx.x == 0) block_matched = 0;
   __syncthreads();
@@ -450,25 +524,28 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
   // tables empty (a representative is written by the row that inserts its key)
   launch_fill_i32(reinterpret_cast<int32_t*>(T.ckey), 2 * kClassSlots, 0, s);
   launch_fill_i32(reinterpret_cast<int32_t*>(T.tkey), 2 * kTplSlots, 0, s);
-  launch_fill_i32(T.matched, 2, 0, s);
+  launch_fill_i32(T.matched, 4, 0, s);
   auto grid = [](int64_t rows) {
     int64_t b = (rows * kClsLanes + 255) / 256;
     return (unsigned)(b < 1 ? 1 : (b > kClsMaxBlocks ? kClsMaxBlocks : b));
   };
   const Rows RB{nullptr, bmeta, 0};
   if (nb > 0) {
+    hipLaunchKernelGGL(k_class_sample, dim3(kSampleBlocks), dim3(256), 0, s, nb, bptr ? Rows{bptr, nullptr, 0} : RB,
+                       bc, T.matched);
     if (bptr) {
       // B's row metadata from the class pass (rows of at most kTplMaxLen
       // entries) and a pass of its own for the longer rows
       const Rows RP{bptr, nullptr, 0};
       hipLaunchKernelGGL(k_class_insert<true>, dim3(grid(nb)), dim3(256), 0, s, nb, RP, bc, T.ckey, T.crep,
-                         T.bclass, bmeta, bflag);
+                         T.bclass, bmeta, bflag, (const int32_t*)T.matched);
       launch_rowstats(nb, bptr, nullptr, nullptr, 0, bc, bmeta, bflag, s, kTplMaxLen);
     } else {
       hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep,
-                         T.bclass, nullptr, nullptr);
+                         T.bclass, nullptr, nullptr, (const int32_t*)T.matched);
     }
-    hipLaunchKernelGGL(k_class_verify, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.crep, T.bclass, T.matched);
+    hipLaunchKernelGGL(k_class_verify, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.crep, T.bclass, T.matched,
+                       (const int32_t*)T.matched);
   }
   if (n <= 0) return;
   if (!a_is_b) {
@@ -477,8 +554,9 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
     launch_fill_i32(reinterpret_cast<int32_t*>(T.akey), 2 * kClassSlots, 0, s);
     const Rows RA{ap, nullptr, T.row0};
     hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.akey, T.arep, T.aclass,
-                       nullptr, nullptr);
-    hipLaunchKernelGGL(k_class_verify, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.arep, T.aclass, T.matched + 1);
+                       nullptr, nullptr, (const int32_t*)T.matched);
+    hipLaunchKernelGGL(k_class_verify, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.arep, T.aclass, T.matched + 1,
+                       (const int32_t*)T.matched);
   }
   hipLaunchKernelGGL(k_tpl_insert, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.aclass, T.bclass, T.tkey, T.trep,
                      T.tplid, T.matched, nb);
