@@ -944,9 +944,10 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     const bool lng = live && e - s > kPrepassLong;
     // a template candidate (template rows have at most kTplMaxLen <= kPrepassLong
     // entries): valid template, same A class as its representative, and the
-    // same B class at every entry
+    // same B class at every entry (classes set: a class check may clear a
+    // class after the template pass read it)
     int tpl = (tpl_on && live && !lng) ? T.tplid[i] : -1;
-    if (tpl >= 0 && (T.info[tpl].x == 0 || T.aclass[i] != T.aclass[T.trep[tpl]])) tpl = -1;
+    if (tpl >= 0 && (T.info[tpl].x == 0 || T.aclass[i] < 0 || T.aclass[i] != T.aclass[T.trep[tpl]])) tpl = -1;
     const int32_t* tup = tpl >= 0 ? T.tuple + (size_t)tpl * kTplMaxLen - s : nullptr;
     int same = 1;
     if (tpl >= 0) {
@@ -958,7 +959,10 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
         for (int u = 0; u < 4; u++) k[u] = p0 + u * G < e ? ac[p0 + u * G] : -1;
 #pragma unroll
         for (int u = 0; u < 4; u++)
-          if (k[u] >= 0) same &= T.bclass[k[u]] == tup[p0 + u * G] ? 1 : 0;
+          if (k[u] >= 0) {
+            const int bcl = T.bclass[k[u]];
+            same &= (bcl >= 0 && bcl == tup[p0 + u * G]) ? 1 : 0;
+          }
       }
     }
     same = gmin<G>(same);

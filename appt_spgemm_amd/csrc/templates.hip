@@ -400,6 +400,91 @@ __global__ void __launch_bounds__(256) k_tpl_insert(int32_t n, const int32_t* __
   }
 }
 
+// A*A (A's rows are B's rows): the class check of each row and its template
+// candidate in one pass.  Row r is compared with its class representative
+// (as k_class_verify) and, as an output row, hashed with its own class and
+// its entries' B classes (as k_tpl_insert).  The B classes are read while
+// other rows' checks may still clear theirs: a candidate whose B class is
+// cleared afterwards fails the pre-pass check, which requires every class
+// of the row to be set and equal to the template's.  Templates go ahead
+// unless the sampled patterns were mostly distinct.
+__global__ void __launch_bounds__(256) k_verify_tpl(int32_t n, const int32_t* __restrict__ ap,
+                                                    const int32_t* __restrict__ ac, const int32_t* __restrict__ crep,
+                                                    int32_t* cls, unsigned long long* __restrict__ tkeys,
+                                                    int32_t* __restrict__ trep, int32_t* __restrict__ tplid,
+                                                    const int32_t* __restrict__ gate) {
+  constexpr int G = kClsLanes;
+  if (patterns_distinct(gate)) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+      tplid[i] = -1;
+    return;
+  }
+  __shared__ KeyCache kc;
+  cache_init(kc);
+  const int lane = threadIdx.x % G;
+  const int64_t gpb = blockDim.x / G;
+  const int64_t stride = (int64_t)gridDim.x * gpb;
+  const int64_t wave_first = (int64_t)blockIdx.x * gpb + (threadIdx.x / kWave) * (kWave / G);
+  for (int64_t w0 = wave_first; w0 < n; w0 += stride) {
+    const int64_t r = w0 + (threadIdx.x % kWave) / G;
+    const int c = r < n ? cls[r] : -1;
+    const int q = c >= 0 ? crep[c] : -1;
+    const int s = r < n ? ap[r] : 0, e = r < n ? ap[r + 1] : 0;
+    const int m = e - s;
+    int same = 1;
+    if (c >= 0 && q != (int)r) {
+      const int sq = ap[q];
+      if (m != ap[q + 1] - sq) {
+        same = 0;
+      } else {
+        for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
+          int x0[kClsUnroll], x1[kClsUnroll];
+#pragma unroll
+          for (int x = 0; x < kClsUnroll; x++) {
+            const bool in = u0 + x * G < m;
+            x0[x] = in ? ac[s + u0 + x * G] : 0;
+            x1[x] = in ? ac[sq + u0 + x * G] : 0;
+          }
+#pragma unroll
+          for (int x = 0; x < kClsUnroll; x++)
+            if (u0 + x * G < m && (int64_t)x0[x] - r != (int64_t)x1[x] - q) same = 0;
+        }
+      }
+    }
+    same = gmin<G>(same);
+    if (lane == 0 && c >= 0 && !same) cls[r] = -1;
+    const int a = (c >= 0 && same) ? c : -1;
+    unsigned long long h = 0;
+    int ok = a >= 0 ? 1 : 0;
+    if (a >= 0) {
+      for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
+        int k[kClsUnroll];
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++) k[x] = u0 + x * G < m ? ac[s + u0 + x * G] : -1;
+        int bc[kClsUnroll];
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++) bc[x] = k[x] >= 0 ? __hip_atomic_load(&cls[k[x]], __ATOMIC_RELAXED,
+                                                                                   __HIP_MEMORY_SCOPE_AGENT) : 0;
+#pragma unroll
+        for (int x = 0; x < kClsUnroll; x++)
+          if (k[x] >= 0) {
+            ok &= bc[x] >= 0 ? 1 : 0;
+            h += tmix(((unsigned long long)(u0 + x * G + 1) << 32) ^ (unsigned long long)(uint32_t)bc[x]);
+          }
+      }
+    }
+    h = gsum<G, unsigned long long>(h, nullptr);
+    ok = gmin<G>(ok);
+    h = (h + tmix(0x7a3ull ^ ((unsigned long long)(uint32_t)a << 8))) | 1ull;
+    bool mine;
+    const int slot = wave_insert(kc, tkeys, kTplSlots, h, lane == 0 && ok, &mine);
+    if (lane == 0 && r < n) {
+      if (mine) trep[slot] = (int32_t)r;
+      tplid[r] = ok ? slot : -1;
+    }
+  }
+}
+
 // Template tables, one block per template slot, from its representative row:
 // the products' output offsets (B column - row), sorted; the distinct ones
 // are S, and pos[(u, v)] is each product's rank among them.  The template is
@@ -544,10 +629,19 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
       hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep,
                          T.bclass, nullptr, nullptr, (const int32_t*)T.matched);
     }
-    hipLaunchKernelGGL(k_class_verify, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.crep, T.bclass, T.matched,
-                       (const int32_t*)T.matched);
+    // A*A: the check runs inside the template pass below
+    if (!a_is_b)
+      hipLaunchKernelGGL(k_class_verify, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.crep, T.bclass, T.matched,
+                         (const int32_t*)T.matched);
   }
   if (n <= 0) return;
+  if (a_is_b) {
+    hipLaunchKernelGGL(k_verify_tpl, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.crep, T.bclass, T.tkey, T.trep,
+                       T.tplid, (const int32_t*)T.matched);
+    hipLaunchKernelGGL(k_tpl_build, dim3(kTplSlots), dim3(kBuildT), 0, s, ap, ac, bmeta, bc, T.row0, T.tkey, T.trep,
+                       T.bclass, T.info, T.posoff, T.pos, T.soff, T.tuple, T.pos2);
+    return;
+  }
   if (!a_is_b) {
     // A-side rows (a row panel, or A next to B = A^T): a table of their own,
     // whose representatives are A-side rows
