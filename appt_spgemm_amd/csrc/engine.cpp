@@ -428,6 +428,7 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
     T.soff = as<int32_t>(ensure(W.tsoff, sizeof(int32_t) * (size_t)kTplSlots * kTplMaxProducts));
     T.tuple = as<int32_t>(ensure(W.ttuple, sizeof(int32_t) * (size_t)kTplSlots * kTplMaxLen));
     T.pos2 = as<uint32_t>(ensure(W.tpos2, sizeof(uint32_t) * (size_t)kTplSlots * 2 * kTplMaxLen * kTplPairs));
+    T.grid = as<int2>(ensure(W.tgrid, sizeof(int2) * kTplSlots));
     T.matched = as<int32_t>(ensure(W.tmatched, sizeof(int32_t) * 4));
     T.row0 = O.row0;
     launch_tpl_detect(T, n, O.ap, O.ac, O.nb, bmeta, O.bc, a_is_b, meta_in_classes ? O.bptr : nullptr,
@@ -598,7 +599,8 @@ void Engine::stage_numeric(Workspace& W, const Operands& O, const Config& cfg, c
     // the structure is the full row
     RowArgs ra = row_args(W, O, numrows + off, cnt, P.direct_sym ? 0 : O.upper, cfg.num);
     if (b == kBinTemplate)
-      launch_tpl_num(ra, W.tpl, wmax, cmax, rowoff, ccol, cval, num_cus(), lds_per_cu(), s);
+      launch_tpl_num(ra, W.tpl, wmax, cmax, (O.row0 == 0 && O.n == O.nb) ? O.n : 0, rowoff, ccol, cval,
+                     num_cus(), lds_per_cu(), s);
     else
       launch_numeric_bin(b, ra, wmax, cmax, rowoff, ccol, cval, heavy, s);
     off += cnt;

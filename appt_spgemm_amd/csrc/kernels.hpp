@@ -244,6 +244,7 @@ struct TplArgs {
   int32_t* soff;             // per template: kTplMaxProducts output offsets (column - row), ascending
   int32_t* tuple;            // per template: kTplMaxLen B classes of its representative's entries
   uint32_t* pos2;            // per template: [parity][entry u][pair l] slots of B entries 2l - par, 2l - par + 1
+  int2* grid;                // per template: {line, plane} strides read off its A offsets (0: none)
   int32_t* matched;          // [0]: B rows whose pattern repeats another's; [1]: the same for A-side rows;
                              // [2], [3]: distinct patterns among sampled B rows, and the samples
   int32_t row0;              // global row index of A-side row 0 (row panels)
@@ -270,7 +271,10 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
                        hipStream_t s);
 // Numeric pass of the template bin (maxb: longest B row, nsmax: most slots
 // of a row of the bin).
-void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t nsmax, const int64_t* rowoff,
+// tile_n > 0: the bin holds every template row of an n = tile_n row matrix
+// (no panel), so the pair kernel may walk the rows in grid tiles.
+void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t nsmax, int32_t tile_n,
+                    const int64_t* rowoff,
                     int32_t* ccol, double* cval, int num_cus, int64_t lds_per_cu, hipStream_t s);
 
 // Symbolic: rowcnt[i] = distinct output columns of row i.

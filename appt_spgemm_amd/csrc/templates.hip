@@ -485,6 +485,39 @@ __global__ void __launch_bounds__(256) k_verify_tpl(int32_t n, const int32_t* __
   }
 }
 
+// Line and plane strides of a structured grid read off a row's column
+// offsets (a row-ordering hint only): the positive offsets sorted, runs of
+// consecutive values are clusters; the line stride is the centre of the
+// first cluster past the immediate neighbours (offsets above 2), the plane
+// stride the centre of the next cluster beyond twice that.  A 27-point 96^3
+// stencil row gives {1}, {95, 96, 97}, {9215, 9216, 9217}, ...: 96 and 9216.
+// {0, 0} when the offsets show no such clusters.
+__device__ inline int2 grid_strides(const int32_t* cols, int m, int64_t row, int32_t* d /* kTplMaxLen, LDS */) {
+  int np = 0;
+  for (int u = 0; u < m; u++) {
+    const int64_t x = (int64_t)cols[u] - row;
+    if (x > 0 && x < INT32_MAX) {
+      int k = np++;
+      while (k > 0 && d[k - 1] > (int32_t)x) { d[k] = d[k - 1]; k--; }
+      d[k] = (int32_t)x;
+    }
+  }
+  int line = 0, plane = 0;
+  int k = 0;
+  while (k < np) {
+    int e = k;
+    while (e + 1 < np && d[e + 1] <= d[e] + 1) e++;   // run [k, e] (repeats included)
+    const int a = d[k], b = d[e];
+    if (a > 2) {
+      const int c = a + (b - a) / 2;
+      if (line == 0) line = c;
+      else if (c > 2 * line) { plane = c; break; }
+    }
+    k = e + 1;
+  }
+  return make_int2(line, plane);
+}
+
 // Template tables, one block per template slot, from its representative row:
 // the products' output offsets (B column - row), sorted; the distinct ones
 // are S, and pos[(u, v)] is each product's rank among them.  The template is
@@ -496,16 +529,21 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
                                                        const int32_t* __restrict__ rep, const int32_t* __restrict__ bclass,
                                                        int4* __restrict__ info, int32_t* __restrict__ posoff,
                                                        uint16_t* __restrict__ pos, int32_t* __restrict__ soff,
-                                                       int32_t* __restrict__ tuple, uint32_t* __restrict__ pos2) {
+                                                       int32_t* __restrict__ tuple, uint32_t* __restrict__ pos2,
+                                                       int2* __restrict__ grid) {
   __shared__ unsigned long long key[kTplMaxProducts];
   __shared__ uint16_t rank_of[kTplMaxProducts];
   __shared__ int32_t off[kTplMaxLen + 1];
   __shared__ int32_t bstart[kTplMaxLen];
   __shared__ int32_t scan[kBuildT / kWave + 1];
   __shared__ int32_t maxb_s;
+  __shared__ int32_t dsort[kTplMaxLen];
   const int t = blockIdx.x, tid = threadIdx.x;
   if (keys[t] == 0ull) {
-    if (tid == 0) info[t] = make_int4(0, 0, 0, 0);
+    if (tid == 0) {
+      info[t] = make_int4(0, 0, 0, 0);
+      grid[t] = make_int2(0, 0);
+    }
     return;
   }
   const int r = rep[t];
@@ -522,6 +560,7 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
     }
     off[m] = run;
     maxb_s = maxb;
+    grid[t] = grid_strides(ac + s, m, gi, dsort);
   }
   __syncthreads();
   const int P = off[m];
@@ -639,7 +678,7 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
     hipLaunchKernelGGL(k_verify_tpl, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.crep, T.bclass, T.tkey, T.trep,
                        T.tplid, (const int32_t*)T.matched);
     hipLaunchKernelGGL(k_tpl_build, dim3(kTplSlots), dim3(kBuildT), 0, s, ap, ac, bmeta, bc, T.row0, T.tkey, T.trep,
-                       T.bclass, T.info, T.posoff, T.pos, T.soff, T.tuple, T.pos2);
+                       T.bclass, T.info, T.posoff, T.pos, T.soff, T.tuple, T.pos2, T.grid);
     return;
   }
   if (!a_is_b) {
@@ -655,7 +694,7 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
   hipLaunchKernelGGL(k_tpl_insert, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.aclass, T.bclass, T.tkey, T.trep,
                      T.tplid, T.matched, nb);
   hipLaunchKernelGGL(k_tpl_build, dim3(kTplSlots), dim3(kBuildT), 0, s, ap, ac, bmeta, bc, T.row0, T.tkey, T.trep,
-                     T.bclass, T.info, T.posoff, T.pos, T.soff, T.tuple, T.pos2);
+                     T.bclass, T.info, T.posoff, T.pos, T.soff, T.tuple, T.pos2, T.grid);
 }
 
 // ---------------------------------------------------------------------------
@@ -829,7 +868,7 @@ struct TplRow2 {
 };
 
 template <bool W64>
-__global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t nsmax,
+__global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t nsmax, int32_t tile_n,
                                                   const int64_t* __restrict__ rowoff, int32_t* __restrict__ ccol,
                                                   double* __restrict__ cval) {
   constexpr int G = kTplPairG;
@@ -840,13 +879,55 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   double* sa = (double*)base;                      // a_iu
   int2* se = (int2*)(base + kTplPairLen * 8);      // {B-row start, B-row length}
   double* vals = (double*)(base + kTplPairLen * 16);
-  const RowWalk walk = row_walk(A.nrows, R, gi);
+  // Row order.  When the bin holds every template row of the matrix and the
+  // middle row's template shows grid strides (line TL, plane TP), rows are
+  // walked in tiles: blocks of TY lines through all planes, (tile, plane,
+  // line, column) order, rows of no template skipped.  An XCD's rows in
+  // flight then reach into a few planes of one tile instead of whole planes,
+  // so the B rows they share stay in its L2 (C2s numeric 0.91 -> 0.86 ms).
+  // The walk covers each row once whatever the strides; otherwise rows come
+  // from the bin's list.
+  int TL = 0, TP = 0, TY = 1, Ly = 1, Zs = 1;
+  if (tile_n > 0) {
+    const int th = T.tplid[tile_n / 2];
+    if (th >= 0) {
+      const int2 g = T.grid[th];
+      if (g.x >= 2 && g.y >= 4 * g.x && g.y <= tile_n) {
+        TL = g.x;
+        TP = g.y;
+        Ly = (TP + TL - 1) / TL;
+        TY = 2304 / TL;
+        TY = TY < 1 ? 1 : (TY > Ly ? Ly : TY);
+        Zs = (int)(((int64_t)tile_n + TP - 1) / TP);
+      }
+    }
+  }
+  const int64_t tiles_n = TL > 0 ? (int64_t)((Ly + TY - 1) / TY) * Zs * TY * TL : 0;
+  const bool tiled = TL > 0 && tiles_n < INT32_MAX;
+  const RowWalk walk = row_walk(tiled ? (int)tiles_n : A.nrows, R, gi);
+  auto row_at = [&](int rr) -> int {
+    if (!tiled) return A.rows[rr];
+    const int x = rr % TL;
+    int t = rr / TL;
+    const int yl = t % TY;
+    t /= TY;
+    const int z = t % Zs, y = (t / Zs) * TY + yl;
+    if (y >= Ly || y * TL + x >= TP) return -1;
+    const int64_t i = (int64_t)z * TP + (int64_t)y * TL + x;
+    if (i >= tile_n) return -1;
+    return T.tplid[i] >= 0 ? (int)i : -1;
+  };
   const int S = walk.step, r1 = walk.end;
   const uint32_t bnnz = (uint32_t)A.bnnz;
   for (int q = lane; q < nsmax; q += G) vals[q] = 0.0;
   gsync<G>();
 
   auto hop1 = [&](TplRow2& q) {
+    if (q.i < 0) {   // a tile position with no template row
+      q.t = 0;
+      q.s = q.len = 0;
+      return;
+    }
     q.t = T.tplid[q.i];
     q.s = A.ap[q.i];
     q.len = A.ap[q.i + 1] - q.s;
@@ -860,6 +941,11 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
     if (lane + G < q.len) {
       q.a1 = A.av[q.s + lane + G];
       q.k1 = A.ac[q.s + lane + G];
+    }
+    if (q.i < 0) {
+      q.off = 0;
+      q.nS = 0;
+      return;
     }
     q.off = rowoff[q.i];
     q.nS = T.info[q.t].z;
@@ -881,15 +967,15 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   q0.len = q1.len = q2.len = q3.len = 0;
   q0.k0 = q1.k0 = q2.k0 = q3.k0 = -1;
   q0.k1 = q1.k1 = q2.k1 = q3.k1 = -1;
-  if (r < r1) { q0.i = A.rows[r]; hop1(q0); hop2(q0); hop3(q0); }
-  if (r + S < r1) { q1.i = A.rows[r + S]; hop1(q1); hop2(q1); }
-  if (r + 2 * S < r1) { q2.i = A.rows[r + 2 * S]; hop1(q2); }
-  if (r + 3 * S < r1) q3.i = A.rows[r + 3 * S];
+  if (r < r1) { q0.i = row_at(r); hop1(q0); hop2(q0); hop3(q0); }
+  if (r + S < r1) { q1.i = row_at(r + S); hop1(q1); hop2(q1); }
+  if (r + 2 * S < r1) { q2.i = row_at(r + 2 * S); hop1(q2); }
+  if (r + 3 * S < r1) q3.i = row_at(r + 3 * S);
   for (; r < r1; r += S) {
     TplRow2 q4{};
     q4.len = 0;
     q4.k0 = q4.k1 = -1;
-    if (r + 4 * S < r1) q4.i = A.rows[r + 4 * S];
+    if (r + 4 * S < r1) q4.i = row_at(r + 4 * S);
     if (r + 3 * S < r1) hop1(q3);
     if (r + 2 * S < r1) hop2(q2);
     if (r + S < r1) hop3(q1);
@@ -963,8 +1049,9 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   }
 }
 
-void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t nsmax, const int64_t* rowoff,
-                    int32_t* ccol, double* cval, int num_cus, int64_t lds_per_cu, hipStream_t s) {
+void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t nsmax, int32_t tile_n,
+                    const int64_t* rowoff, int32_t* ccol, double* cval, int num_cus, int64_t lds_per_cu,
+                    hipStream_t s) {
   if (a.nrows <= 0) return;
   if (maxb <= kTplPairMax && a.bnnz >= 2) {
     // maxb: the widest A or B row of any template row (pairs of B values
@@ -985,11 +1072,11 @@ void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t ns
     const int64_t cap = (int64_t)num_cus * fit;
     if (blocks > cap) blocks = cap;
     if (w64)
-      hipLaunchKernelGGL(k_tpl_num2<true>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax, rowoff,
-                         ccol, cval);
+      hipLaunchKernelGGL(k_tpl_num2<true>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax, tile_n,
+                         rowoff, ccol, cval);
     else
-      hipLaunchKernelGGL(k_tpl_num2<false>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax, rowoff,
-                         ccol, cval);
+      hipLaunchKernelGGL(k_tpl_num2<false>, dim3((unsigned)blocks), dim3(R * kTplPairG), lds, s, a, T, nsmax,
+                         tile_n, rowoff, ccol, cval);
     return;
   }
   const int64_t gb = tpl_group_bytes(nsmax);

@@ -30,6 +30,9 @@ def engs():
 def cases():
     yield "laplace2d 64", ap.laplace2d(64), True
     yield "stencil27 g=14", ap.stencil27(14, 0.0, 5), True
+    # wide enough (56-column lines) that template rows are walked in tiles of
+    # fewer lines than a plane
+    yield "stencil27 g=56", ap.stencil27(56, 0.0, 6), True
     yield "stencil27 g=14 drop", ap.stencil27(14, 0.1, 5), False
     yield "banded full 3000 bw 8", ap.banded(3000, 8, 1.0, 1), True
     yield "banded 0.7", ap.banded(3000, 8, 0.7, 2), False
