@@ -185,7 +185,7 @@ void launch_batch_tblock(int32_t K, int32_t N, int32_t max_nt, int64_t nnz_t, co
                          hipStream_t s) {
   if (K <= 0) return;
   const size_t lds = sizeof(int32_t) * ((size_t)max_nt + 1);
-  (void)hipFuncSetAttribute((const void*)k_batch_tblock, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  set_dyn_lds((const void*)k_batch_tblock, lds);
   hipLaunchKernelGGL(k_batch_tblock, dim3((unsigned)K), dim3(1024), lds, s, K, blk_row, blk_nnz, blk_op, toff, rid,
                      col, val, tp, bcol, bval);
   launch_fill_i32(tp + N, 1, (int32_t)nnz_t, s);

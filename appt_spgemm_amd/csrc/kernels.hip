@@ -11,6 +11,8 @@
 // heavy-row bins let the waves of a workgroup run ahead of each other, so
 // their sums may be associated differently (bounded by the per-entry
 // tolerance, not bit-equal).
+#include <unordered_map>
+
 #include "kernels.hpp"
 #include "device_common.hpp"
 
@@ -50,6 +52,24 @@ __global__ void k_fill3_i32(int32_t* a, int64_t na, int32_t* b, int64_t nb, int3
     else c[t - na - nb] = 0;
   }
 }
+void set_dyn_lds(const void* fn, size_t bytes) {
+  struct Key {
+    const void* fn;
+    int dev;
+    bool operator==(const Key& o) const { return fn == o.fn && dev == o.dev; }
+  };
+  struct KeyHash {
+    size_t operator()(const Key& k) const { return std::hash<const void*>()(k.fn) ^ (size_t)k.dev; }
+  };
+  thread_local std::unordered_map<Key, size_t, KeyHash> cap;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  auto it = cap.find(Key{fn, dev});
+  if (it != cap.end() && it->second >= bytes) return;
+  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  cap[Key{fn, dev}] = bytes;
+}
+
 void launch_fill3_i32(int32_t* a, int64_t na, int32_t* b, int64_t nb, int32_t* c, int64_t nc, hipStream_t s) {
   const int64_t tot = na + nb + nc;
   if (tot <= 0) return;
@@ -1966,7 +1986,7 @@ static void sym_sort_launch(const RowArgs& a, int32_t na_max, int32_t* rowcnt, h
   int R; unsigned blocks;
   sort_launch_shape<G>(a, rb, &R, &blocks);
   const size_t lds = (size_t)R * rb;
-  (void)hipFuncSetAttribute((const void*)k_sym_sort<G, E>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  set_dyn_lds((const void*)k_sym_sort<G, E>, lds);
   hipLaunchKernelGGL((k_sym_sort<G, E>), dim3(blocks), dim3(R * G), lds, s, a, na_max, rowcnt);
 }
 template <int G, int E, bool UPPER, typename K>
@@ -1976,8 +1996,7 @@ static void num_sort_launch_k(const RowArgs& a, int32_t na_max, const int64_t* r
   int R; unsigned blocks;
   sort_launch_shape<G>(a, rb, &R, &blocks);
   const size_t lds = (size_t)R * rb;
-  (void)hipFuncSetAttribute((const void*)k_num_sort<G, E, UPPER, K>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds);
+  set_dyn_lds((const void*)k_num_sort<G, E, UPPER, K>, lds);
   hipLaunchKernelGGL((k_num_sort<G, E, UPPER, K>), dim3(blocks), dim3(R * G), lds, s, a, na_max, rowoff, ccol,
                      cval);
 }
@@ -2896,7 +2915,7 @@ static void sym_lds(const RowArgs& a, int32_t words_max, int32_t* rowcnt, hipStr
   const int R = G > kWave ? 1 : rows_per_block(G, rb, a.lds_budget);
   const size_t lds = (size_t)R * rb;
   const unsigned blocks = (unsigned)((a.nrows + R - 1) / R);
-  (void)hipFuncSetAttribute((const void*)k_symbolic_lds<G, DENSE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  set_dyn_lds((const void*)k_symbolic_lds<G, DENSE>, lds);
   hipLaunchKernelGGL((k_symbolic_lds<G, DENSE>), dim3(blocks), dim3(R * G), lds, s, a, row_words, rowcnt);
 }
 
@@ -2907,7 +2926,7 @@ static void num_lds(const RowArgs& a, int32_t words_max, int32_t c_max, const in
   const int R = G > kWave ? 1 : rows_per_block(G, rb, a.lds_budget);
   const size_t lds = (size_t)R * rb;
   const unsigned blocks = (unsigned)((a.nrows + R - 1) / R);
-  (void)hipFuncSetAttribute((const void*)k_numeric_lds<G, DENSE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  set_dyn_lds((const void*)k_numeric_lds<G, DENSE>, lds);
   hipLaunchKernelGGL((k_numeric_lds<G, DENSE>), dim3(blocks), dim3(R * G), lds, s, a, words_max, c_max, rowoff, ccol,
                      cval);
 }
@@ -2943,7 +2962,7 @@ static void sym_window_launch(const RowArgs& a, int32_t W, int32_t* rowcnt, hipS
   int R; unsigned blocks;
   window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks);
   const size_t lds = (size_t)R * gb;
-  (void)hipFuncSetAttribute((const void*)k_sym_window<G, UPPER, W64>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  set_dyn_lds((const void*)k_sym_window<G, UPPER, W64>, lds);
   hipLaunchKernelGGL((k_sym_window<G, UPPER, W64>), dim3(blocks), dim3(R * G), lds, s, a, W, rowcnt);
 }
 template <int G, bool UPPER, bool W64, bool LIST>
@@ -2953,8 +2972,7 @@ static void num_window_launch(const RowArgs& a, int32_t W, int32_t Cm, const int
   int R; unsigned blocks;
   window_shape(G, gb, a.nrows, a.lds_budget, &R, &blocks);
   const size_t lds = (size_t)R * gb;
-  (void)hipFuncSetAttribute((const void*)k_num_window<G, UPPER, W64, LIST>,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  set_dyn_lds((const void*)k_num_window<G, UPPER, W64, LIST>, lds);
   hipLaunchKernelGGL((k_num_window<G, UPPER, W64, LIST>), dim3(blocks), dim3(R * G), lds, s, a, W, Cm,
                      rowoff, ccol, cval);
 }
@@ -2991,11 +3009,11 @@ static void heavy_launch(bool vals, const RowArgs& a, const HeavyArgs& h, int32_
   const unsigned blocks = (unsigned)(a.nrows < g_num_cus ? a.nrows : g_num_cus);
   launch_fill_i32(work, 1, 0, s);
   if (vals) {
-    (void)hipFuncSetAttribute((const void*)k_heavy_num, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    set_dyn_lds((const void*)k_heavy_num, lds);
     hipLaunchKernelGGL(k_heavy_num, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_num, h.vcap, h.dcols, work,
                        h.bflag, h.cur, rowoff, ccol, cval, h.fault);
   } else {
-    (void)hipFuncSetAttribute((const void*)k_heavy_sym, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    set_dyn_lds((const void*)k_heavy_sym, lds);
     hipLaunchKernelGGL(k_heavy_sym, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_sym, work, h.bflag, h.cur, rowcnt);
   }
 }

@@ -69,6 +69,11 @@ inline int64_t num_slots(int64_t c) { return np2_host((c * 3) / 2 + 1); }
 // ---- launchers (all asynchronous on `s`) ----
 // gate (optional) on a launcher: its kernels run only when *gate != 0
 void launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s, const int32_t* gate = nullptr);
+// Raise a kernel's dynamic-LDS cap to at least `bytes` on the current device.
+// The cap only has to cover each launch, so it is set once per thread,
+// device and kernel for the largest size asked so far: between short
+// kernels, the host call otherwise costs more than the kernels do.
+void set_dyn_lds(const void* fn, size_t bytes);
 // B-row metadata {start, end, min column, max column} (min = INT32_MAX,
 // max = -1 for an empty row).
 // Row k is [ptr[k], ptr[k+1]) of col, or [off2 + ptr2[k], off2 + ptr2[k+1])

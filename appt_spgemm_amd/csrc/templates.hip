@@ -646,9 +646,8 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
                        int4* bmeta, const int32_t* bc, bool a_is_b, const int32_t* bptr, int32_t* bflag,
                        hipStream_t s) {
   // tables empty (a representative is written by the row that inserts its key)
-  launch_fill_i32(reinterpret_cast<int32_t*>(T.ckey), 2 * kClassSlots, 0, s);
-  launch_fill_i32(reinterpret_cast<int32_t*>(T.tkey), 2 * kTplSlots, 0, s);
-  launch_fill_i32(T.matched, 4, 0, s);
+  launch_fill3_i32(reinterpret_cast<int32_t*>(T.ckey), 2 * kClassSlots, reinterpret_cast<int32_t*>(T.tkey),
+                   2 * kTplSlots, T.matched, 4, s);
   auto grid = [](int64_t rows) {
     int64_t b = (rows * kClsLanes + 255) / 256;
     return (unsigned)(b < 1 ? 1 : (b > kClsMaxBlocks ? kClsMaxBlocks : b));
@@ -1062,12 +1061,22 @@ void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t ns
     const size_t lds = (size_t)((int64_t)R * gb);
     const bool w64 = a.bnnz >= ((int64_t)1 << 29);
     const void* fn = w64 ? (const void*)k_tpl_num2<true> : (const void*)k_tpl_num2<false>;
-    (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    set_dyn_lds(fn, lds);
     // persistent blocks: as many as are resident at once (rows are split
     // among the blocks up front, so a block that waits for a slot is a tail)
-    int fit = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, fn, R * kTplPairG, lds) != hipSuccess || fit < 1) fit = 4;
-    if (fit > 8) fit = 8;
+    // (asked once per thread, device and LDS size)
+    thread_local int fit_dev = -1, fit = 0;
+    thread_local size_t fit_lds = 0;
+    thread_local const void* fit_fn = nullptr;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (fit_dev != dev || fit_lds != lds || fit_fn != fn) {
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, fn, R * kTplPairG, lds) != hipSuccess || fit < 1) fit = 4;
+      if (fit > 8) fit = 8;
+      fit_dev = dev;
+      fit_lds = lds;
+      fit_fn = fn;
+    }
     int64_t blocks = ((int64_t)a.nrows + R - 1) / R;
     const int64_t cap = (int64_t)num_cus * fit;
     if (blocks > cap) blocks = cap;
@@ -1090,7 +1099,7 @@ void launch_tpl_num(const RowArgs& a, const TplArgs& T, int32_t maxb, int32_t ns
   const bool w64 = a.bnnz >= ((int64_t)1 << 29);
   const void* fn = G == 32 ? (w64 ? (const void*)k_tpl_num<32, true> : (const void*)k_tpl_num<32, false>)
                            : (w64 ? (const void*)k_tpl_num<64, true> : (const void*)k_tpl_num<64, false>);
-  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  set_dyn_lds(fn, lds);
   const dim3 grid((unsigned)blocks), block((unsigned)(R * G));
   if (G == 32) {
     if (w64) hipLaunchKernelGGL((k_tpl_num<32, true>), grid, block, lds, s, a, T, nsmax, rowoff, ccol, cval);
