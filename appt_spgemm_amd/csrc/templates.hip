@@ -857,10 +857,15 @@ __device__ __forceinline__ void store_i4(int32_t* p, int4 v) {
   *(i32x4u*)p = i32x4u{v.x, v.y, v.z, v.w};
 }
 
+// A row in the pair kernel's pipeline.  Every hop stores what it loaded as it
+// is and derives nothing from it: a value computed from a load at once would
+// make the hop wait for that load, and the next rows' loads are meant to be
+// in flight while this row's products are formed.
 struct TplRow2 {
-  int i, t, s, len;
-  int k0, k1;        // B rows of entries lane and lane + G (-1: none); after hop3 their lengths
+  int i, t, s, e;    // row, template, first entry, end
+  int k0, k1;        // B rows of entries lane and lane + G (-1: none)
   int bs0, bs1;      // B-row starts
+  int be0, be1;      // B-row ends
   double a0, a1;     // a_iu of those entries
   int64_t off;       // first output position of the row
   int nS;            // the template's slot count
@@ -904,17 +909,19 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   const int64_t tiles_n = TL > 0 ? (int64_t)((Ly + TY - 1) / TY) * Zs * TY * TL : 0;
   const bool tiled = TL > 0 && tiles_n < INT32_MAX;
   const RowWalk walk = row_walk(tiled ? (int)tiles_n : A.nrows, R, gi);
+  // a tile position's row, or -1 off the grid; whether the row has a
+  // template is known once hop1 has read it (rows without are skipped)
   auto row_at = [&](int rr) -> int {
     if (!tiled) return A.rows[rr];
-    const int x = rr % TL;
-    int t = rr / TL;
-    const int yl = t % TY;
-    t /= TY;
-    const int z = t % Zs, y = (t / Zs) * TY + yl;
-    if (y >= Ly || y * TL + x >= TP) return -1;
+    const uint32_t ur = (uint32_t)rr;
+    const uint32_t x = ur % (uint32_t)TL;
+    uint32_t t = ur / (uint32_t)TL;
+    const uint32_t yl = t % (uint32_t)TY;
+    t /= (uint32_t)TY;
+    const uint32_t z = t % (uint32_t)Zs, y = (t / (uint32_t)Zs) * (uint32_t)TY + yl;
+    if (y >= (uint32_t)Ly || y * (uint32_t)TL + x >= (uint32_t)TP) return -1;
     const int64_t i = (int64_t)z * TP + (int64_t)y * TL + x;
-    if (i >= tile_n) return -1;
-    return T.tplid[i] >= 0 ? (int)i : -1;
+    return i < tile_n ? (int)i : -1;
   };
   const int S = walk.step, r1 = walk.end;
   const uint32_t bnnz = (uint32_t)A.bnnz;
@@ -922,29 +929,30 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   gsync<G>();
 
   auto hop1 = [&](TplRow2& q) {
-    if (q.i < 0) {   // a tile position with no template row
-      q.t = 0;
-      q.s = q.len = 0;
+    if (q.i < 0) {   // a tile position off the grid
+      q.t = -1;
+      q.s = q.e = 0;
       return;
     }
-    q.t = T.tplid[q.i];
+    q.t = T.tplid[q.i];   // -1: not a template row (tiled walk), skipped
     q.s = A.ap[q.i];
-    q.len = A.ap[q.i + 1] - q.s;
+    q.e = A.ap[q.i + 1];
   };
   auto hop2 = [&](TplRow2& q) {
     q.k0 = q.k1 = -1;
-    if (lane < q.len) {
-      q.a0 = A.av[q.s + lane];
-      q.k0 = A.ac[q.s + lane];
-    }
-    if (lane + G < q.len) {
-      q.a1 = A.av[q.s + lane + G];
-      q.k1 = A.ac[q.s + lane + G];
-    }
-    if (q.i < 0) {
+    if (q.t < 0) {
+      q.s = q.e = 0;
       q.off = 0;
       q.nS = 0;
       return;
+    }
+    if (q.s + lane < q.e) {
+      q.a0 = A.av[q.s + lane];
+      q.k0 = A.ac[q.s + lane];
+    }
+    if (q.s + lane + G < q.e) {
+      q.a1 = A.av[q.s + lane + G];
+      q.k1 = A.ac[q.s + lane + G];
     }
     q.off = rowoff[q.i];
     q.nS = T.info[q.t].z;
@@ -953,17 +961,17 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
     if (q.k0 >= 0) {
       const int4 b = A.bmeta[q.k0];
       q.bs0 = b.x;
-      q.k0 = b.y - b.x;   // from here on: the B row's length
+      q.be0 = b.y;
     }
     if (q.k1 >= 0) {
       const int4 b = A.bmeta[q.k1];
       q.bs1 = b.x;
-      q.k1 = b.y - b.x;
+      q.be1 = b.y;
     }
   };
   int r = walk.first;
   TplRow2 q0{}, q1{}, q2{}, q3{};
-  q0.len = q1.len = q2.len = q3.len = 0;
+  q0.s = q0.e = q1.s = q1.e = q2.s = q2.e = q3.s = q3.e = 0;
   q0.k0 = q1.k0 = q2.k0 = q3.k0 = -1;
   q0.k1 = q1.k1 = q2.k1 = q3.k1 = -1;
   if (r < r1) { q0.i = row_at(r); hop1(q0); hop2(q0); hop3(q0); }
@@ -972,20 +980,20 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
   if (r + 3 * S < r1) q3.i = row_at(r + 3 * S);
   for (; r < r1; r += S) {
     TplRow2 q4{};
-    q4.len = 0;
+    q4.s = q4.e = 0;
     q4.k0 = q4.k1 = -1;
     if (r + 4 * S < r1) q4.i = row_at(r + 4 * S);
     if (r + 3 * S < r1) hop1(q3);
     if (r + 2 * S < r1) hop2(q2);
     if (r + S < r1) hop3(q1);
-    const int t = q0.t, m = q0.len, nS = q0.nS;
+    const int t = q0.t < 0 ? 0 : q0.t, m = q0.e - q0.s, nS = q0.nS;   // skipped rows: m = nS = 0
     if (lane < m) {
       sa[lane] = q0.a0;
-      se[lane] = make_int2(q0.bs0, q0.k0);
+      se[lane] = make_int2(q0.bs0, q0.be0 - q0.bs0);
     }
     if (lane + G < m) {
       sa[lane + G] = q0.a1;
-      se[lane + G] = make_int2(q0.bs1, q0.k1);
+      se[lane + G] = make_int2(q0.bs1, q0.be1 - q0.bs1);
     }
     gsync<G>();
     const uint32_t* p2 = T.pos2 + (size_t)t * (2 * kTplMaxLen * kTplPairs);
@@ -995,13 +1003,21 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
     for (int u0 = 0; u0 < m; u0 += kTplFast) {
       double pv0[kTplFast], pv1[kTplFast];
       uint32_t sl[kTplFast];
+      int2 xs[kTplFast];
+      double as[kTplFast];
+      // the batch's staged entries first (one LDS wait for all of them)
 #pragma unroll
       for (int d = 0; d < kTplFast; d++) {
-        const bool in = u0 + d < m;
-        const int u = in ? u0 + d : u0;
-        const int2 x = se[u];
+        const int u = u0 + d < m ? u0 + d : u0;
+        xs[d] = se[u];
+        as[d] = sa[u];
+      }
+#pragma unroll
+      for (int d = 0; d < kTplFast; d++) {
+        const int u = u0 + d < m ? u0 + d : u0;
+        const int2 x = xs[d];
         const int par = x.x & 1;
-        const bool live = in && 2 * lane < x.y + par;
+        const bool live = (u0 + d < m) & (2 * lane < x.y + par);
         const uint32_t q = live ? (uint32_t)((x.x & ~1) + 2 * lane) : 0u;   // even: the pair (q, q + 1)
         // the last entry of an odd-length B: load the pair ending there
         const bool tail = q + 1 >= bnnz;
@@ -1009,9 +1025,8 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
         sl[d] = live ? v : 0xffffffffu;
         const dpair raw = load_pair<W64>(A.bv, tail ? q - 1 : q);
         const double bx = tail ? raw.y : raw.x, by = tail ? 0.0 : raw.y;
-        const double au = sa[u];
-        pv0[d] = au * bx;
-        pv1[d] = au * by;
+        pv0[d] = as[d] * bx;
+        pv1[d] = as[d] * by;
       }
 #pragma unroll
       for (int d = 0; d < kTplFast; d++) {
