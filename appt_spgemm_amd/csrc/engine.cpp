@@ -432,7 +432,7 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
     T.matched = as<int32_t>(ensure(W.tmatched, sizeof(int32_t) * 4));
     T.row0 = O.row0;
     launch_tpl_detect(T, n, O.ap, O.ac, O.nb, bmeta, O.bc, a_is_b, meta_in_classes ? O.bptr : nullptr,
-                      as<int32_t>(W.ctl.p) + 2, s);
+                      as<int32_t>(W.ctl.p) + 2, as<int32_t>(W.ctl.p) + kCtlLongRows, s);
   }
   launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part,
                  W.tpl_on ? &W.tpl : nullptr, rowcnt, rowwords, s);

@@ -167,10 +167,12 @@ struct Result {  // what appt_csr_result::_priv points to
 // [0..1] heavy-row claim counters (symbolic, numeric); [2] B rows unsorted;
 // [3] the symmetric-pattern transpose gather failed; [4] A rows not strictly
 // increasing; [5..7] structure check failures (ends, row pointer order,
-// columns); [8] a heavy numeric window could not advance; [12..13] AAtS as
+// columns); [8] a heavy numeric window could not advance; [10] a B row longer than the
+// template class pass describes; [12..13] AAtS as
 // an upper-triangle Gram: the full A*A^T product count (u64).
 constexpr int kCtlWords = 16;
 constexpr int kCtlHeavyFault = 8;
+constexpr int kCtlLongRows = 10;   // some B row has more than kTplMaxLen entries (template class pass)
 constexpr int kCtlGramProducts = 12;
 
 struct HostStats {  // pinned staging for the small device->host reads; the

@@ -109,7 +109,9 @@ void launch_copy_i64(const int64_t* in, int64_t n, int64_t* out, hipStream_t s) 
 // ---------------------------------------------------------------------------
 __global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ ptr2,
                            const uint8_t* __restrict__ sel, int64_t off2, const int32_t* __restrict__ col,
-                           int4* __restrict__ meta, int32_t* __restrict__ flag, int32_t min_len) {
+                           int4* __restrict__ meta, int32_t* __restrict__ flag, int32_t min_len,
+                           const int32_t* gate) {
+  if (gated_off(gate)) return;
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
@@ -139,12 +141,13 @@ __global__ void k_rowstats(int32_t n, const int32_t* __restrict__ ptr, const int
   if (__ballot(unsorted) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
 }
 void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
-                     const int32_t* col, int4* meta, int32_t* flag, hipStream_t s, int32_t min_len) {
+                     const int32_t* col, int4* meta, int32_t* flag, hipStream_t s, int32_t min_len,
+                     const int32_t* gate) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   hipLaunchKernelGGL(k_rowstats, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, ptr2, sel, off2, col, meta, flag,
-                     min_len);
+                     min_len, gate);
 }
 
 // ---------------------------------------------------------------------------

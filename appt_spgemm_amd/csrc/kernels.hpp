@@ -79,9 +79,11 @@ void set_dyn_lds(const void* fn, size_t bytes);
 // Row k is [ptr[k], ptr[k+1]) of col, or [off2 + ptr2[k], off2 + ptr2[k+1])
 // where sel (optional) is non-zero.  flag[0] is set to 1 when some row's
 // columns are not in nondecreasing order (flag must be zero beforehand).
-// (rows of at most min_len entries are skipped: another kernel describes them)
+// (rows of at most min_len entries are skipped: another kernel describes them;
+// gate (optional): the kernel does nothing while *gate is zero)
 void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const uint8_t* sel, int64_t off2,
-                     const int32_t* col, int4* meta, int32_t* flag, hipStream_t s, int32_t min_len = -1);
+                     const int32_t* col, int4* meta, int32_t* flag, hipStream_t s, int32_t min_len = -1,
+                     const int32_t* gate = nullptr);
 void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t s, const int32_t* gate = nullptr);
 // Structure check of a square CSR (flags zero beforehand): flag[0] = 1 when
 // ptr[0] != 0 or ptr[n] != nnz, flag[1] = 1 when row pointers decrease,
@@ -270,10 +272,11 @@ __host__ __device__ inline int64_t tpl_pair_group_bytes(int64_t nsmax) {
 // representative) and the template tables.
 // bptr (optional): B's row pointers, when bmeta is not built yet -- the
 // class pass then writes B's row metadata too (rows over kTplMaxLen entries
-// by a pass of their own) and raises *bflag on a descending B row.
+// by a pass of their own, launched only when *longrows was raised: longrows
+// must be zero beforehand) and raises *bflag on a descending B row.
 void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb,
                        int4* bmeta, const int32_t* bc, bool a_is_b, const int32_t* bptr, int32_t* bflag,
-                       hipStream_t s);
+                       int32_t* longrows, hipStream_t s);
 // Numeric pass of the template bin (maxb: longest B row, nsmax: most slots
 // of a row of the bin).
 // tile_n > 0: the bin holds every template row of an n = tile_n row matrix

@@ -224,7 +224,8 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
                                                       unsigned long long* __restrict__ keys,
                                                       int32_t* __restrict__ rep, int32_t* __restrict__ cls,
                                                       int4* __restrict__ meta, int32_t* __restrict__ flag,
-                                                      const int32_t* __restrict__ matched) {
+                                                      const int32_t* __restrict__ matched,
+                                                      int32_t* __restrict__ longrows) {
   constexpr int G = kClsLanes;
   __shared__ KeyCache kc;
   cache_init(kc);
@@ -271,6 +272,8 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
       mx = gmax<G>(mx);
       if (lane == 0 && r < nrows && m <= kTplMaxLen) meta[r] = make_int4(se.x, se.y, mn, mx);
       if (__ballot(desc) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
+      // rows past kTplMaxLen are left to the metadata pass, run only then
+      if (__ballot(r < nrows && m > kTplMaxLen) != 0ull && threadIdx.x % kWave == 0) longrows[0] = 1;
     }
     if (off) {
       if (lane == 0 && r < nrows) cls[r] = -1;
@@ -644,7 +647,7 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
 
 void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb,
                        int4* bmeta, const int32_t* bc, bool a_is_b, const int32_t* bptr, int32_t* bflag,
-                       hipStream_t s) {
+                       int32_t* longrows, hipStream_t s) {
   // tables empty (a representative is written by the row that inserts its key)
   launch_fill3_i32(reinterpret_cast<int32_t*>(T.ckey), 2 * kClassSlots, reinterpret_cast<int32_t*>(T.tkey),
                    2 * kTplSlots, T.matched, 4, s);
@@ -661,11 +664,11 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
       // entries) and a pass of its own for the longer rows
       const Rows RP{bptr, nullptr, 0};
       hipLaunchKernelGGL(k_class_insert<true>, dim3(grid(nb)), dim3(256), 0, s, nb, RP, bc, T.ckey, T.crep,
-                         T.bclass, bmeta, bflag, (const int32_t*)T.matched);
-      launch_rowstats(nb, bptr, nullptr, nullptr, 0, bc, bmeta, bflag, s, kTplMaxLen);
+                         T.bclass, bmeta, bflag, (const int32_t*)T.matched, longrows);
+      launch_rowstats(nb, bptr, nullptr, nullptr, 0, bc, bmeta, bflag, s, kTplMaxLen, longrows);
     } else {
       hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep,
-                         T.bclass, nullptr, nullptr, (const int32_t*)T.matched);
+                         T.bclass, nullptr, nullptr, (const int32_t*)T.matched, nullptr);
     }
     // A*A: the check runs inside the template pass below
     if (!a_is_b)
@@ -686,7 +689,7 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
     launch_fill_i32(reinterpret_cast<int32_t*>(T.akey), 2 * kClassSlots, 0, s);
     const Rows RA{ap, nullptr, T.row0};
     hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.akey, T.arep, T.aclass,
-                       nullptr, nullptr, (const int32_t*)T.matched);
+                       nullptr, nullptr, (const int32_t*)T.matched, nullptr);
     hipLaunchKernelGGL(k_class_verify, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.arep, T.aclass, T.matched + 1,
                        (const int32_t*)T.matched);
   }

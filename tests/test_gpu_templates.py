@@ -37,6 +37,10 @@ def cases():
     yield "banded full 3000 bw 8", ap.banded(3000, 8, 1.0, 1), True
     yield "banded 0.7", ap.banded(3000, 8, 0.7, 2), False
     yield "erdos-renyi", ap.erdos_renyi(4000, 6.0, 3), False
+    # hub rows of more than 64 entries: their B-row metadata comes from the
+    # pass the class pass launches for long rows (values accumulate in
+    # workgroup bins: within tolerance, not bit for bit)
+    yield "rmat 12 long rows", ap.rmat(12, 16, seed=3), False
     # a stencil with one row's columns shuffled (same set, other storage order):
     # it and the rows that reference it leave the template
     m = ap.stencil27(10, 0.0, 7)
@@ -54,14 +58,15 @@ def test_templates_match_oracle(engs, op):
         got = on.multiply(m, op)
         rows = on.bin_report()["numeric"].get("template", (0, 0))[0]
         say("   template rows", rows, "of", m.n)
-        assert_product(got, ref(m, op), exact=name != "stencil27 one reversed row" or op == "AA")
+        exact = (name != "stencil27 one reversed row" or op == "AA") and not name.startswith("rmat")
+        assert_product(got, ref(m, op), exact=exact)
         if expect:
             assert rows > m.n // 4
         plain = off.multiply(m, op)
         assert "template" not in off.bin_report()["numeric"]
         np.testing.assert_array_equal(got.rowptr, plain.rowptr)
         np.testing.assert_array_equal(got.col, plain.col)
-        if op != "AAtS":
+        if op != "AAtS" and exact:
             np.testing.assert_array_equal(got.val, plain.val)
 
 
