@@ -359,7 +359,10 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
     O.bv = tv;
     bptr = tp;
   }
-  if (op == APPT_OP_AATS) launch_rows_strict(n, A.rowptr, A.col, ctl + 4, s);
+  // strictly increasing rows are implied when the symmetric-pattern gather
+  // succeeded (it checks every row for descents and duplicates), so the
+  // check runs on the device only when that transpose failed
+  if (op == APPT_OP_AATS) launch_rows_strict(n, A.rowptr, A.col, ctl + 4, s, ctl + 3);
   O.bptr = bptr;
   // when row templates will be looked for, their class pass over B's rows
   // also writes the B-row metadata (a Gram needs it at once)
@@ -432,7 +435,7 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
     T.matched = as<int32_t>(ensure(W.tmatched, sizeof(int32_t) * 4));
     T.row0 = O.row0;
     launch_tpl_detect(T, n, O.ap, O.ac, O.nb, bmeta, O.bc, a_is_b, meta_in_classes ? O.bptr : nullptr,
-                      as<int32_t>(W.ctl.p) + 2, as<int32_t>(W.ctl.p) + kCtlLongRows, s);
+                      as<int32_t>(W.ctl.p) + 2, as<int32_t>(W.ctl.p) + kCtlLongRows, as<int32_t>(W.ctl.p) + 3, s);
   }
   launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part,
                  W.tpl_on ? &W.tpl : nullptr, rowcnt, rowwords, s);

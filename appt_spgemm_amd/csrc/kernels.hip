@@ -156,7 +156,8 @@ void launch_rowstats(int32_t n, const int32_t* ptr, const int32_t* ptr2, const u
 // B row holds each column once (see the value-order note at the top).
 // ---------------------------------------------------------------------------
 __global__ void k_rows_strict(int32_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                              int32_t* __restrict__ flag) {
+                              int32_t* __restrict__ flag, const int32_t* gate) {
+  if (gated_off(gate)) return;
   constexpr int G = 16;
   const int lane = threadIdx.x % G;
   const int64_t gpb = blockDim.x / G;
@@ -165,11 +166,12 @@ __global__ void k_rows_strict(int32_t n, const int32_t* __restrict__ ptr, const 
     for (int p = ptr[i] + lane; p + 1 < ptr[i + 1]; p += G) bad |= col[p] >= col[p + 1];
   if (__ballot(bad) != 0ull && threadIdx.x % kWave == 0) flag[0] = 1;
 }
-void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s) {
+void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s,
+                        const int32_t* gate) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 4096) blocks = 4096;
-  hipLaunchKernelGGL(k_rows_strict, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, flag);
+  hipLaunchKernelGGL(k_rows_strict, dim3((unsigned)blocks), dim3(256), 0, s, n, ptr, col, flag, gate);
 }
 
 __global__ void k_csr_check(int32_t n, int64_t nnz, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,

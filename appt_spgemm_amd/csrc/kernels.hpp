@@ -92,7 +92,9 @@ void launch_colcount(int64_t nnz, const int32_t* col, int32_t* cnt, hipStream_t 
 void launch_csr_check(int32_t n, int64_t nnz, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);
 // flag[0] = 1 (zero beforehand) when some row of (ptr, col) is not strictly
 // increasing
-void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s);
+// gate (optional): nothing runs while *gate is zero
+void launch_rows_strict(int32_t n, const int32_t* ptr, const int32_t* col, int32_t* flag, hipStream_t s,
+                        const int32_t* gate = nullptr);
 // Radix transpose: digits of at most kRadixMaxBits bits (the fewest passes
 // that cover the column range, bits split evenly), one 256-thread block per
 // tile of kRadixTile entries (kRadixChunks chunks of 64 per wave).
@@ -274,9 +276,12 @@ __host__ __device__ inline int64_t tpl_pair_group_bytes(int64_t nsmax) {
 // class pass then writes B's row metadata too (rows over kTplMaxLen entries
 // by a pass of their own, launched only when *longrows was raised: longrows
 // must be zero beforehand) and raises *bflag on a descending B row.
+// apat: zero on the device when A's rows have B's patterns (A*A; A*A^T when
+// the symmetric-pattern transpose gather succeeded): the A-side rows then
+// take B's classes.
 void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb,
                        int4* bmeta, const int32_t* bc, bool a_is_b, const int32_t* bptr, int32_t* bflag,
-                       int32_t* longrows, hipStream_t s);
+                       int32_t* longrows, const int32_t* apat, hipStream_t s);
 // Numeric pass of the template bin (maxb: longest B row, nsmax: most slots
 // of a row of the bin).
 // tile_n > 0: the bin holds every template row of an n = tile_n row matrix

@@ -225,8 +225,10 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
                                                       int32_t* __restrict__ rep, int32_t* __restrict__ cls,
                                                       int4* __restrict__ meta, int32_t* __restrict__ flag,
                                                       const int32_t* __restrict__ matched,
-                                                      int32_t* __restrict__ longrows) {
+                                                      int32_t* __restrict__ longrows,
+                                                      const int32_t* __restrict__ gate) {
   constexpr int G = kClsLanes;
+  if (gate && *gate == 0) return;   // A-side rows whose classes are copied from B's
   __shared__ KeyCache kc;
   cache_init(kc);
   // sampled patterns mostly distinct: no classes (the metadata still)
@@ -296,9 +298,11 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
 // counted into *matched (one atomic per block).
 __global__ void __launch_bounds__(256) k_class_verify(int32_t nrows, Rows R, const int32_t* __restrict__ col,
                                                       const int32_t* __restrict__ rep, int32_t* __restrict__ cls,
-                                                      int32_t* __restrict__ matched, const int32_t* __restrict__ gate) {
+                                                      int32_t* __restrict__ matched, const int32_t* __restrict__ gate,
+                                                      const int32_t* __restrict__ apat) {
   constexpr int G = kClsLanes;
   if (patterns_distinct(gate)) return;   // every row is classless already
+  if (apat && *apat == 0) return;        // A-side rows whose classes are copied from B's
   __shared__ int32_t block_matched;
   if (threadIdx.x == 0) block_matched = 0;
   __syncthreads();
@@ -645,9 +649,20 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
 
 }  // namespace
 
+// A-side rows that are B's rows by pattern (A*A, and A*A^T of a symmetric
+// pattern, which *apat == 0 reports: the transpose gather succeeded): their
+// classes are B's classes of the same global rows, copied instead of hashed
+// and checked again.
+__global__ void k_class_copy(int32_t n, int32_t row0, const int32_t* __restrict__ bclass,
+                             int32_t* __restrict__ aclass, const int32_t* __restrict__ apat) {
+  if (*apat != 0) return;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    aclass[i] = bclass[(int64_t)row0 + i];
+}
+
 void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int32_t* ac, int32_t nb,
                        int4* bmeta, const int32_t* bc, bool a_is_b, const int32_t* bptr, int32_t* bflag,
-                       int32_t* longrows, hipStream_t s) {
+                       int32_t* longrows, const int32_t* apat, hipStream_t s) {
   // tables empty (a representative is written by the row that inserts its key)
   launch_fill3_i32(reinterpret_cast<int32_t*>(T.ckey), 2 * kClassSlots, reinterpret_cast<int32_t*>(T.tkey),
                    2 * kTplSlots, T.matched, 4, s);
@@ -664,16 +679,16 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
       // entries) and a pass of its own for the longer rows
       const Rows RP{bptr, nullptr, 0};
       hipLaunchKernelGGL(k_class_insert<true>, dim3(grid(nb)), dim3(256), 0, s, nb, RP, bc, T.ckey, T.crep,
-                         T.bclass, bmeta, bflag, (const int32_t*)T.matched, longrows);
+                         T.bclass, bmeta, bflag, (const int32_t*)T.matched, longrows, nullptr);
       launch_rowstats(nb, bptr, nullptr, nullptr, 0, bc, bmeta, bflag, s, kTplMaxLen, longrows);
     } else {
       hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep,
-                         T.bclass, nullptr, nullptr, (const int32_t*)T.matched, nullptr);
+                         T.bclass, nullptr, nullptr, (const int32_t*)T.matched, nullptr, nullptr);
     }
     // A*A: the check runs inside the template pass below
     if (!a_is_b)
       hipLaunchKernelGGL(k_class_verify, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.crep, T.bclass, T.matched,
-                         (const int32_t*)T.matched);
+                         (const int32_t*)T.matched, nullptr);
   }
   if (n <= 0) return;
   if (a_is_b) {
@@ -686,12 +701,18 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
   if (!a_is_b) {
     // A-side rows (a row panel, or A next to B = A^T): a table of their own,
     // whose representatives are A-side rows
+    // (rows whose pattern is B's take B's classes instead, decided on the
+    // device by *apat)
     launch_fill_i32(reinterpret_cast<int32_t*>(T.akey), 2 * kClassSlots, 0, s);
     const Rows RA{ap, nullptr, T.row0};
+    int64_t cb = ((int64_t)n + 255) / 256;
+    if (cb > 4096) cb = 4096;
+    hipLaunchKernelGGL(k_class_copy, dim3((unsigned)cb), dim3(256), 0, s, n, T.row0, (const int32_t*)T.bclass,
+                       T.aclass, apat);
     hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.akey, T.arep, T.aclass,
-                       nullptr, nullptr, (const int32_t*)T.matched, nullptr);
+                       nullptr, nullptr, (const int32_t*)T.matched, nullptr, apat);
     hipLaunchKernelGGL(k_class_verify, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.arep, T.aclass, T.matched + 1,
-                       (const int32_t*)T.matched);
+                       (const int32_t*)T.matched, apat);
   }
   hipLaunchKernelGGL(k_tpl_insert, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.aclass, T.bclass, T.tkey, T.trep,
                      T.tplid, T.matched, nb);
