@@ -902,6 +902,7 @@ int64_t stats_part_elems() { return (int64_t)kNumBins * kStatFields * kStatsMaxB
 // Structural pre-pass: products f_i (k_flops of the reference), output span
 // [lo, hi] from the B rows' min/max columns, symbolic bin, bin statistics.
 constexpr int kPrepassLanes = 4, kPrepassLong = 64;
+constexpr int kPrepassCheckU = 8;   // template-check entries per lane per step (32 per row)
 __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32_t* __restrict__ ac,
                           const int4* __restrict__ bmeta, int upper, BinParams P,
                           int64_t* __restrict__ rowf, int32_t* __restrict__ rowlo, int32_t* __restrict__ rowhi,
@@ -961,39 +962,55 @@ __global__ void k_prepass(int32_t n, const int32_t* __restrict__ ap, const int32
     const int64_t i = i0 + (wi - wave_first);
     const bool live = i < n;
     int64_t f = 0;
-    int lo = INT32_MAX, hi = -1, s = 0, e = 0;
+    int lo = INT32_MAX, hi = -1, s = 0, e = 0, tpl = -1, acl = -1;
     if (live) {
       s = ap[i];
       e = ap[i + 1];
+      if (tpl_on) {
+        tpl = T.tplid[i];
+        acl = T.aclass[i];
+      }
     }
     const bool lng = live && e - s > kPrepassLong;
+    if (lng || acl < 0) tpl = -1;
     // a template candidate (template rows have at most kTplMaxLen <= kPrepassLong
-    // entries): valid template, same A class as its representative, and the
-    // same B class at every entry (classes set: a class check may clear a
-    // class after the template pass read it)
-    int tpl = (tpl_on && live && !lng) ? T.tplid[i] : -1;
-    if (tpl >= 0 && (T.info[tpl].x == 0 || T.aclass[i] < 0 || T.aclass[i] != T.aclass[T.trep[tpl]])) tpl = -1;
-    const int32_t* tup = tpl >= 0 ? T.tuple + (size_t)tpl * kTplMaxLen - s : nullptr;
+    // entries): valid template of this row's length, same A class as its
+    // representative, and the same B class at every entry (classes set: a
+    // class check may clear a class after the template pass read it).  The
+    // checks' loads depend on the row's template and entries only, so they
+    // go out together: three dependent rounds (row, template and entries,
+    // B classes and the representative's class).  The products are the
+    // template's, the span unused.
     int same = 1;
+    int64_t tf = 0;
     if (tpl >= 0) {
-      // a template candidate is checked on its entries' B classes alone; its
-      // products are the template's, its span unused
-      for (int p0 = s + lane; p0 < e; p0 += 4 * G) {
-        int k[4];
+      const int m = e - s;
+      const int4 inf = T.info[tpl];
+      const int rep = T.trep[tpl];
+      tf = T.posoff[(size_t)tpl * (kTplMaxLen + 1) + m];
+      const int32_t* tup = T.tuple + (size_t)tpl * kTplMaxLen;
+      constexpr int U = kPrepassCheckU;
+      for (int u0 = lane; u0 < m; u0 += U * G) {
+        int k[U], tv[U];
 #pragma unroll
-        for (int u = 0; u < 4; u++) k[u] = p0 + u * G < e ? ac[p0 + u * G] : -1;
+        for (int u = 0; u < U; u++) {
+          const bool in = u0 + u * G < m;
+          k[u] = in ? ac[s + u0 + u * G] : -1;
+          tv[u] = in ? tup[u0 + u * G] : -1;
+        }
+        int bcl[U];
 #pragma unroll
-        for (int u = 0; u < 4; u++)
-          if (k[u] >= 0) {
-            const int bcl = T.bclass[k[u]];
-            same &= (bcl >= 0 && bcl == tup[p0 + u * G]) ? 1 : 0;
-          }
+        for (int u = 0; u < U; u++) bcl[u] = k[u] >= 0 ? T.bclass[k[u]] : -1;
+#pragma unroll
+        for (int u = 0; u < U; u++)
+          if (k[u] >= 0) same &= (bcl[u] >= 0 && bcl[u] == tv[u]) ? 1 : 0;
       }
+      same &= (inf.x != 0 && inf.y == m && T.aclass[rep] == acl) ? 1 : 0;
     }
     same = gmin<G>(same);
     if (!same) tpl = -1;
     if (tpl >= 0) {
-      if (lane == 0) f = T.posoff[(size_t)tpl * (kTplMaxLen + 1) + T.info[tpl].y];
+      if (lane == 0) f = tf;
     } else if (live && !lng) {
       // 4 entries per lane per step: their column and B-row loads in flight together
       for (int p0 = s + lane; p0 < e; p0 += 4 * G) {

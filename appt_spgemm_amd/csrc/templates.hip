@@ -435,39 +435,31 @@ __global__ void __launch_bounds__(256) k_verify_tpl(int32_t n, const int32_t* __
   for (int64_t w0 = wave_first; w0 < n; w0 += stride) {
     const int64_t r = w0 + (threadIdx.x % kWave) / G;
     const int c = r < n ? cls[r] : -1;
-    const int q = c >= 0 ? crep[c] : -1;
     const int s = r < n ? ap[r] : 0, e = r < n ? ap[r + 1] : 0;
     const int m = e - s;
-    int same = 1;
-    if (c >= 0 && q != (int)r) {
-      const int sq = ap[q];
-      if (m != ap[q + 1] - sq) {
-        same = 0;
-      } else {
-        for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
-          int x0[kClsUnroll], x1[kClsUnroll];
-#pragma unroll
-          for (int x = 0; x < kClsUnroll; x++) {
-            const bool in = u0 + x * G < m;
-            x0[x] = in ? ac[s + u0 + x * G] : 0;
-            x1[x] = in ? ac[sq + u0 + x * G] : 0;
-          }
-#pragma unroll
-          for (int x = 0; x < kClsUnroll; x++)
-            if (u0 + x * G < m && (int64_t)x0[x] - r != (int64_t)x1[x] - q) same = 0;
-        }
-      }
+    const int q = c >= 0 ? crep[c] : -1;
+    // compared with the representative (rows of other lengths differ)
+    const bool cmp = c >= 0 && q != (int)r;
+    int same = 1, sq = 0;
+    if (cmp) {
+      sq = ap[q];
+      same = ap[q + 1] - sq == m ? 1 : 0;
     }
-    same = gmin<G>(same);
-    if (lane == 0 && c >= 0 && !same) cls[r] = -1;
-    const int a = (c >= 0 && same) ? c : -1;
+    // one pass over the row's entries: the comparison with the
+    // representative's entries and the template hash of the entries' B
+    // classes, their loads in flight together.  The hash is used only if the
+    // row keeps its class.
     unsigned long long h = 0;
-    int ok = a >= 0 ? 1 : 0;
-    if (a >= 0) {
+    int ok = c >= 0 ? 1 : 0;
+    if (c >= 0 && same) {
       for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
-        int k[kClsUnroll];
+        int k[kClsUnroll], x1[kClsUnroll];
 #pragma unroll
-        for (int x = 0; x < kClsUnroll; x++) k[x] = u0 + x * G < m ? ac[s + u0 + x * G] : -1;
+        for (int x = 0; x < kClsUnroll; x++) {
+          const bool in = u0 + x * G < m;
+          k[x] = in ? ac[s + u0 + x * G] : -1;
+          x1[x] = (in && cmp) ? ac[sq + u0 + x * G] : 0;
+        }
         int bc[kClsUnroll];
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++) bc[x] = k[x] >= 0 ? __hip_atomic_load(&cls[k[x]], __ATOMIC_RELAXED,
@@ -475,11 +467,16 @@ __global__ void __launch_bounds__(256) k_verify_tpl(int32_t n, const int32_t* __
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++)
           if (k[x] >= 0) {
+            if (cmp && (int64_t)k[x] - r != (int64_t)x1[x] - q) same = 0;
             ok &= bc[x] >= 0 ? 1 : 0;
             h += tmix(((unsigned long long)(u0 + x * G + 1) << 32) ^ (unsigned long long)(uint32_t)bc[x]);
           }
       }
     }
+    same = gmin<G>(same);
+    if (lane == 0 && c >= 0 && !same) cls[r] = -1;
+    const int a = (c >= 0 && same) ? c : -1;
+    if (a < 0) ok = 0;
     h = gsum<G, unsigned long long>(h, nullptr);
     ok = gmin<G>(ok);
     h = (h + tmix(0x7a3ull ^ ((unsigned long long)(uint32_t)a << 8))) | 1ull;
