@@ -542,6 +542,8 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
   __shared__ int32_t scan[kBuildT / kWave + 1];
   __shared__ int32_t maxb_s;
   __shared__ int32_t dsort[kTplMaxLen];
+  __shared__ int32_t cols[kTplMaxLen];
+  __shared__ int32_t blen[kTplMaxLen];
   const int t = blockIdx.x, tid = threadIdx.x;
   if (keys[t] == 0ull) {
     if (tid == 0) {
@@ -553,18 +555,26 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
   const int r = rep[t];
   const int s = ap[r], m = ap[r + 1] - s;
   const int64_t gi = (int64_t)row0 + r;
+  // the representative's entries and their B rows, one thread each (m <=
+  // kTplMaxLen <= kBuildT: template rows are at most kTplMaxLen long)
+  if (tid < m) {
+    const int k = ac[s + tid];
+    const int4 b = bmeta[k];
+    cols[tid] = k;
+    bstart[tid] = b.x;
+    blen[tid] = b.y - b.x;
+  }
+  __syncthreads();
   if (tid == 0) {
     int run = 0, maxb = 0;
     for (int u = 0; u < m; u++) {
-      const int4 b = bmeta[ac[s + u]];
       off[u] = run;
-      bstart[u] = b.x;
-      run += b.y - b.x;
-      maxb = b.y - b.x > maxb ? b.y - b.x : maxb;
+      run += blen[u];
+      maxb = blen[u] > maxb ? blen[u] : maxb;
     }
     off[m] = run;
     maxb_s = maxb;
-    grid[t] = grid_strides(ac + s, m, gi, dsort);
+    grid[t] = grid_strides(cols, m, gi, dsort);
   }
   __syncthreads();
   const int P = off[m];
@@ -576,7 +586,7 @@ __global__ void __launch_bounds__(kBuildT) k_tpl_build(const int32_t* __restrict
   for (int u = tid; u <= m; u += kBuildT) po[u] = off[u];
   // the representative's B classes, entry by entry: the pre-pass compares
   // every candidate row against them
-  for (int u = tid; u < m; u += kBuildT) tuple[(size_t)t * kTplMaxLen + u] = bclass[ac[s + u]];
+  for (int u = tid; u < m; u += kBuildT) tuple[(size_t)t * kTplMaxLen + u] = bclass[cols[u]];
   // keys: (offset + 2^31) << 32 | product index; padding sorts last
   int np2 = 1;
   while (np2 < P) np2 <<= 1;
