@@ -1036,20 +1036,23 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
       const int64_t row = (int64_t)T.row0 + q0.i;
       const int64_t off = q0.off;
       const int32_t* so = T.soff + (size_t)t * kTplMaxProducts;
-      for (int u0 = 0; u0 < m; u0 += kTplFast) {
-        double pv0[kTplFast], pv1[kTplFast];
-        uint32_t sl[kTplFast];
-        int2 xs[kTplFast];
-        double as[kTplFast];
+      // one batch of W entries from u0: their gathers issued together, no
+      // branch around them (dead lanes and entries past m load entry 0)
+      auto batch = [&](int u0, auto wc) {
+        constexpr int W = decltype(wc)::value;
+        double pv0[W], pv1[W];
+        uint32_t sl[W];
+        int2 xs[W];
+        double as[W];
         // the batch's staged entries first (one LDS wait for all of them)
-  #pragma unroll
-        for (int d = 0; d < kTplFast; d++) {
+#pragma unroll
+        for (int d = 0; d < W; d++) {
           const int u = u0 + d < m ? u0 + d : u0;
           xs[d] = se[u];
           as[d] = sa[u];
         }
-  #pragma unroll
-        for (int d = 0; d < kTplFast; d++) {
+#pragma unroll
+        for (int d = 0; d < W; d++) {
           const int u = u0 + d < m ? u0 + d : u0;
           const int2 x = xs[d];
           const int par = x.x & 1;
@@ -1064,13 +1067,21 @@ __global__ void __launch_bounds__(256) k_tpl_num2(RowArgs A, TplArgs T, int32_t 
           pv0[d] = as[d] * bx;
           pv1[d] = as[d] * by;
         }
-  #pragma unroll
-        for (int d = 0; d < kTplFast; d++) {
+#pragma unroll
+        for (int d = 0; d < W; d++) {
           const uint32_t s0 = sl[d] & 0xffffu, s1 = sl[d] >> 16;
           if (s0 != 0xffffu) atomicAdd(&vals[s0], pv0[d]);
           if (s1 != 0xffffu) atomicAdd(&vals[s1], pv1[d]);
         }
-      }
+      };
+      // full batches, then the rest in a batch of kTplFast or kTplFast / 2
+      // entries: a 27-entry row ends in 3 entries instead of 3 live and 3
+      // dead ones (the gathers' address processing bounds the kernel, and
+      // dead entries cost it as much as live ones)
+      int u0 = 0;
+      for (; u0 + kTplFast <= m; u0 += kTplFast) batch(u0, std::integral_constant<int, kTplFast>{});
+      if (m - u0 > kTplFast / 2) batch(u0, std::integral_constant<int, kTplFast>{});
+      else if (u0 < m) batch(u0, std::integral_constant<int, kTplFast / 2>{});
       gsync<G>();
       // write-out: four consecutive slots per lane (16-byte stores, so a
       // row takes a third of the store instructions), clearing them
