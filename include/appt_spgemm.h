@@ -1,8 +1,10 @@
 /*
  * appt_spgemm.h -- C ABI of the MI355X (gfx950) SpGEMM engine.
  *
- * Computes C = A*A, C = A*A^T, or C = A*A^T via the symmetric upper-triangle
- * Gram kernel, for square CSR matrices with int32 indices and fp64 values,
+ * Computes C = A*A, C = A*A^T, or the symmetric C = A*A^T of the reference's
+ * Gram task (full rows whose values are bit-symmetric by default; optionally
+ * only one triangle's products, mirrored), for square CSR matrices with int32
+ * indices and fp64 values,
  * using structural semantics: an entry whose contributions cancel to 0.0 is
  * kept, columns of every output row are strictly ascending.
  *
