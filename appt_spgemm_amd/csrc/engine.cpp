@@ -353,8 +353,16 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
       rx.hoff = as<int64_t>(ensure(W.rhoff, sizeof(int64_t) * (size_t)(hn + 1)));
       rx.htmp = as<int64_t>(ensure(W.rtmp, sizeof(int64_t) * (size_t)scan_tmp_elems(std::max<int64_t>(hn, 1))));
     }
-    launch_transpose(n, nnz, A.rowptr, A.col, A.val, tp, tc, tv, cnt, off64, tmp, slist, radix ? &rx : nullptr,
-                     asym, s);
+    // the gather's outcome is read back before the general transpose is
+    // launched: a synchronisation costs less than the general path's
+    // launches (up to 17 for the radix sort), which would otherwise be
+    // queued and find their gate closed on the device
+    launch_transpose_gather(n, A.rowptr, A.col, A.val, tp, tc, tv, asym, s);
+    APPT_HIP(hipMemcpyAsync(&hs_->ctl[3], asym, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    wait_stream();
+    if (hs_->ctl[3] != 0)
+      launch_transpose_general(n, nnz, A.rowptr, A.col, A.val, tp, tc, tv, cnt, off64, tmp, slist,
+                               radix ? &rx : nullptr, asym, s);
     O.bc = tc;
     O.bv = tv;
     bptr = tp;

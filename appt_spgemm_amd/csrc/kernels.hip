@@ -228,29 +228,57 @@ __global__ void k_transpose_gather(int32_t n, const int32_t* __restrict__ ap, co
     const int s = ap[k], e0 = ap[k + 1];
     const int e = stop != 0 ? s : e0;
     bool miss = false;
-    for (int p = s + lane; p < e; p += G) {
-      if (p - s >= 16 * G && (p - s) % (16 * G) == lane &&
+    // two entries per lane per step (p and p + G), their dependent loads
+    // issued together; the guessed mirror entry's value is loaded with its
+    // column, so a row of up to 2G entries takes four dependent rounds
+    for (int p0 = s + lane; p0 < e; p0 += 2 * G) {
+      if (p0 - s >= 16 * G && (p0 - s) % (16 * G) == lane &&
           __hip_atomic_load(asym, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
         break;
-      const int i = ac[p];
-      if (p > s && ac[p - 1] >= i) miss = true;   // descent or duplicate
-      int lo = ap[i], hi = ap[i + 1];
-      const int end = hi;
-      const int g = hi - 1 - (p - s);
-      if (g >= lo && g < hi && ac[g] == (int)k) {
-        lo = g;
-      } else {
-        while (lo < hi) {
-          const int mid = (int)(((unsigned)lo + (unsigned)hi) >> 1);
-          if (ac[mid] < (int)k) lo = mid + 1;
-          else hi = mid;
-        }
+      int pp[2], i[2], prev[2], lo[2], hi[2], cg[2];
+      bool in[2], gok[2];
+      double vg[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        pp[u] = p0 + u * G;
+        in[u] = pp[u] < e;
+        i[u] = in[u] ? ac[pp[u]] : 0;
+        prev[u] = (in[u] && pp[u] > s) ? ac[pp[u] - 1] : INT32_MIN;
       }
-      if (lo < end && ac[lo] == (int)k) {
-        bc[p] = i;
-        bv[p] = av[lo];
-      } else {
-        miss = true;
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        lo[u] = in[u] ? ap[i[u]] : 0;
+        hi[u] = in[u] ? ap[i[u] + 1] : 0;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        // entry d of row k is expected at index len_i - 1 - d of row i
+        const int g = hi[u] - 1 - (pp[u] - s);
+        gok[u] = in[u] && g >= lo[u] && g < hi[u];
+        cg[u] = gok[u] ? ac[g] : -1;
+        vg[u] = gok[u] ? av[g] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        if (!in[u]) continue;
+        if (pp[u] > s && prev[u] >= i[u]) miss = true;   // descent or duplicate
+        if (gok[u] && cg[u] == (int)k) {
+          bc[pp[u]] = i[u];
+          bv[pp[u]] = vg[u];
+          continue;
+        }
+        int l = lo[u], h = hi[u];
+        while (l < h) {
+          const int mid = (int)(((unsigned)l + (unsigned)h) >> 1);
+          if (ac[mid] < (int)k) l = mid + 1;
+          else h = mid;
+        }
+        if (l < hi[u] && ac[l] == (int)k) {
+          bc[pp[u]] = i[u];
+          bv[pp[u]] = av[l];
+        } else {
+          miss = true;
+        }
       }
     }
     if (miss) *asym = 1;   // every writer stores 1
@@ -630,9 +658,8 @@ void launch_radix_pairs(int32_t n, int64_t count, const RadixScratch& rx, int32_
                    rx.htmp, nullptr, s);
 }
 
-void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
-                      int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
-                      int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s) {
+void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, int32_t* tp,
+                             int32_t* bc, double* bv, int32_t* asym, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n * 16 + 255) / 256;
   if (blocks > 16384) blocks = 16384;
@@ -640,6 +667,14 @@ void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* 
   int64_t b0 = ((int64_t)n + 256) / 256;
   if (b0 > 4096) b0 = 4096;
   hipLaunchKernelGGL(k_rowptr_if_symmetric, dim3((unsigned)b0), dim3(256), 0, s, n, ap, tp, asym);
+}
+
+void launch_transpose_general(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
+                              int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
+                              int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = ((int64_t)n * 16 + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
   if (rx) {
     // asymmetric, large: radix passes; the last one writes (keys2, bc, bv).
     // The row ids go to the rows buffer the first pass does not write.
@@ -660,6 +695,13 @@ void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* 
   launch_i64_to_i32(off64, n, cnt, s, asym);
   launch_transpose_scatter(n, ap, ac, av, cnt, bc, bv, asym, s);
   launch_sort_rows(n, tp, bc, bv, slist, asym, s);
+}
+
+void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
+                      int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
+                      int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s) {
+  launch_transpose_gather(n, ap, ac, av, tp, bc, bv, asym, s);
+  launch_transpose_general(n, nnz, ap, ac, av, tp, bc, bv, cnt, off64, tmp, slist, rx, asym, s);
 }
 
 // ---------------------------------------------------------------------------

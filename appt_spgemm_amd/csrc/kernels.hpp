@@ -124,6 +124,14 @@ struct RadixScratch {
 void launch_transpose(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
                       int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
                       int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s);
+// The same in two steps: the symmetric-pattern gather (sets *asym when it
+// fails), then the general transpose, which runs on the device only when
+// *asym is set -- a caller that has read *asym back skips it when zero.
+void launch_transpose_gather(int32_t n, const int32_t* ap, const int32_t* ac, const double* av, int32_t* tp,
+                             int32_t* bc, double* bv, int32_t* asym, hipStream_t s);
+void launch_transpose_general(int32_t n, int64_t nnz, const int32_t* ap, const int32_t* ac, const double* av,
+                              int32_t* tp, int32_t* bc, double* bv, int32_t* cnt, int64_t* off64, int64_t* tmp,
+                              int32_t* slist, const RadixScratch* rx, int32_t* asym, hipStream_t s);
 // AAtS mirror by radix sort: per-row counts of entries left / right of the
 // diagonal, then (with their exclusive scans lo_off / up_off and the upper
 // total `count`) the strictly-upper entries sorted by column into the
