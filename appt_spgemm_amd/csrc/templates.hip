@@ -255,8 +255,18 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++) c[x] = u0 + x * G < m ? col[se.x + u0 + x * G] : 0;
         if (META) {
+          // each entry's successor: the next lane's entry of the same step
+          // (a quad rotation: G = 4 lanes per row); lane G - 1 takes lane 0's
+          // entry of the next step, and the last step's from memory
+          static_assert(G == 4, "successors come from a quad rotation");
+          const int last = u0 + (kClsUnroll - 1) * G + 1;   // lane G - 1's last successor, own index
+          const int tail = col[se.x + (last < m ? last : m - 1)];
 #pragma unroll
-          for (int x = 0; x < kClsUnroll; x++) nx[x] = u0 + x * G + 1 < m ? col[se.x + u0 + x * G + 1] : INT32_MAX;
+          for (int x = 0; x < kClsUnroll; x++) {
+            const int r0 = __builtin_amdgcn_update_dpp(0, c[x], 0x39, 0xf, 0xf, false);
+            const int r1 = x + 1 < kClsUnroll ? __builtin_amdgcn_update_dpp(0, c[x + 1], 0x39, 0xf, 0xf, false) : tail;
+            nx[x] = u0 + x * G + 1 < m ? (lane < G - 1 ? r0 : r1) : INT32_MAX;
+          }
         }
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++)
