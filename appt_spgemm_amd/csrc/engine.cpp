@@ -447,8 +447,7 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
   }
   launch_prepass(n, O.ap, O.ac, bmeta, O.upper, cfg.sym, rowf, rowlo, rowhi, symbin, dstats, dprod, part,
                  W.tpl_on ? &W.tpl : nullptr, rowcnt, rowwords, s);
-  launch_bin_cursors(dstats, dcursor, nullptr, nullptr, s);
-  launch_bin_scatter(n, symbin, dcursor, symrows, s);
+  launch_bin_scatter(n, symbin, dstats, dcursor, symrows, nullptr, nullptr, s);
 }
 
 void Engine::read_sym(Workspace& W, Plan& P, bool want_ctl) {
@@ -573,9 +572,10 @@ void Engine::stage_offsets(Workspace& W, const Operands& O, const Config& cfg, i
   launch_numbin(n, O.ap, as<int64_t>(W.rowf.p), as<int32_t>(W.rowcnt.p), as<int32_t>(W.rowwords.p),
                 as<int32_t>(W.rowlo.p), as<int32_t>(W.rowhi.p), cfg.num, numbin, dnum, as<int32_t>(W.statpart.p),
                 W.tpl_on ? &W.tpl : nullptr, s);
-  // the cursor kernel also parks nnz(C) next to the products count
-  launch_bin_cursors(dnum, dcursor, rowoff + n, reinterpret_cast<int64_t*>(dstats + stat_words() * 2 + 2), s);
-  launch_bin_scatter(n, numbin, dcursor, numrows, s);
+  // the scatter also parks nnz(C) next to the products count (for n = 0 the
+  // zeroed statistics hold it already)
+  launch_bin_scatter(n, numbin, dnum, dcursor, numrows, rowoff + n,
+                     reinterpret_cast<int64_t*>(dstats + stat_words() * 2 + 2), s);
 }
 
 void Engine::read_num(Workspace& W, const Operands& O, Plan& P) {

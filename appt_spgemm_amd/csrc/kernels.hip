@@ -1178,11 +1178,28 @@ void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int3
 // it in four chunks of 64; ranks come from wave ballots per distinct bin; one
 // global atomic per (tile, bin) reserves the tile's range.
 __global__ void __launch_bounds__(256) k_bin_scatter(int32_t n, const uint8_t* __restrict__ bin,
-                                                     int32_t* __restrict__ cursor, int32_t* __restrict__ rows) {
+                                                     const int32_t* __restrict__ stats, int32_t* __restrict__ taken,
+                                                     int32_t* __restrict__ rows, const int64_t* __restrict__ total_src,
+                                                     int64_t* __restrict__ total_dst) {
   constexpr int kItems = 4, kWaves = 4;
   __shared__ int32_t cnt[kWaves][kNumBins];
   __shared__ int32_t woff[kWaves][kNumBins];
+  __shared__ int32_t start[kNumBins];
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  // first slot of each bin in the row list: exclusive scan of the bin
+  // counts over bins 1.. (bin 0 is not listed), in every block
+  for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) start[b] = b > 0 ? stats[b * kStatFields] : 0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (blockIdx.x == 0 && total_src) *total_dst = *total_src;
+    int32_t run = 0;
+    for (int b = 0; b < kNumBins; b++) {
+      const int32_t c = start[b];
+      start[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
   for (int64_t tile = (int64_t)blockIdx.x * 1024; tile < n; tile += (int64_t)gridDim.x * 1024) {
     for (int t = lane; t < kNumBins; t += kWave) cnt[w][t] = 0;
     __syncthreads();
@@ -1211,7 +1228,7 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int32_t n, const uint8_t* _
     for (int b = threadIdx.x; b < kNumBins; b += blockDim.x) {
       int tot = 0;
       for (int v = 0; v < kWaves; v++) tot += cnt[v][b];
-      int base = (b > 0 && tot) ? atomicAdd(&cursor[b], tot) : 0;
+      int base = (b > 0 && tot) ? start[b] + atomicAdd(&taken[b], tot) : 0;
       for (int v = 0; v < kWaves; v++) { woff[v][b] = base; base += cnt[v][b]; }
     }
     __syncthreads();
@@ -1223,30 +1240,13 @@ __global__ void __launch_bounds__(256) k_bin_scatter(int32_t n, const uint8_t* _
     __syncthreads();
   }
 }
-// cursor[b] = first slot of bin b in the row list: exclusive scan of the
-// bin counts stats[b * kStatFields] over bins 1.. (bin 0 is not listed).
-__global__ void k_bin_cursors(const int32_t* __restrict__ stats, int32_t* __restrict__ cursor,
-                              const int64_t* __restrict__ total_src, int64_t* __restrict__ total_dst) {
-  if (threadIdx.x == 0 && total_src) *total_dst = *total_src;
-  if (threadIdx.x == 0) {
-    int32_t run = 0;
-    cursor[0] = 0;
-    for (int b = 1; b < kNumBins; b++) {
-      cursor[b] = run;
-      run += stats[b * kStatFields];
-    }
-  }
-}
-void launch_bin_cursors(const int32_t* stats, int32_t* cursor, const int64_t* total_src, int64_t* total_dst,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_bin_cursors, dim3(1), dim3(64), 0, s, stats, cursor, total_src, total_dst);
-}
-
-void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s) {
+void launch_bin_scatter(int32_t n, const uint8_t* bin, const int32_t* stats, int32_t* taken, int32_t* rows,
+                        const int64_t* total_src, int64_t* total_dst, hipStream_t s) {
   if (n <= 0) return;
   int64_t blocks = ((int64_t)n + 1023) / 1024;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(256), 0, s, n, bin, cursor, rows);
+  hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)blocks), dim3(256), 0, s, n, bin, stats, taken, rows, total_src,
+                     total_dst);
 }
 
 // ---------------------------------------------------------------------------

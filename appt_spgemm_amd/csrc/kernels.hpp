@@ -166,14 +166,16 @@ void launch_prepass(int32_t n, const int32_t* ap, const int32_t* ac, const int4*
 void launch_numbin(int32_t n, const int32_t* ap, const int64_t* rowf, const int32_t* rowcnt,
                    const int32_t* rowwords, const int32_t* rowlo, const int32_t* rowhi, const BinParams& P,
                    uint8_t* bin, int32_t* stats, int32_t* part, const TplArgs* T, hipStream_t s);
-// cursor[b] = start of bin b in the row list (exclusive scan of the counts
-// in a stats array of kNumBins * kStatFields).
-// (total_src, optional: *total_dst = *total_src on the way)
-void launch_bin_cursors(const int32_t* stats, int32_t* cursor, const int64_t* total_src, int64_t* total_dst,
-                        hipStream_t s);
+
 // Zero three int32 arrays in one launch.
 void launch_fill3_i32(int32_t* a, int64_t na, int32_t* b, int64_t nb, int32_t* c, int64_t nc, hipStream_t s);
-void launch_bin_scatter(int32_t n, const uint8_t* bin, int32_t* cursor, int32_t* rows, hipStream_t s);
+// Row list by bin: bin b's rows start at the exclusive scan of the counts
+// stats[b * kStatFields] (bins 1..; bin 0 is not listed), in no particular
+// order within a bin; taken: kNumBins int32, zero beforehand (blocks claim
+// their slots there).  total_src (optional): *total_dst = *total_src on the
+// way.
+void launch_bin_scatter(int32_t n, const uint8_t* bin, const int32_t* stats, int32_t* taken, int32_t* rows,
+                        const int64_t* total_src, int64_t* total_dst, hipStream_t s);
 
 struct RowArgs {
   const int32_t* rows;   // row ids of this bin
