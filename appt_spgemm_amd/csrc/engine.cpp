@@ -425,6 +425,7 @@ void Engine::stage_prepass(Workspace& W, const Operands& O, const Config& cfg) {
     const size_t nb = (size_t)std::max(O.nb, 1);
     T.ckey = as<unsigned long long>(ensure(W.ckey, sizeof(unsigned long long) * kClassSlots));
     T.crep = as<int32_t>(ensure(W.crep, sizeof(int32_t) * kClassSlots));
+    T.cpat = as<int32_t>(ensure(W.cpat, sizeof(int32_t) * (size_t)kClassSlots * (kTplMaxLen + 1)));
     T.bclass = as<int32_t>(ensure(W.bclass, sizeof(int32_t) * nb));
     const bool a_is_b = O.a_is_b && O.row0 == 0 && O.n == O.nb;
     T.aclass = a_is_b ? T.bclass : as<int32_t>(ensure(W.aclass, sizeof(int32_t) * nn));

@@ -98,7 +98,7 @@ struct Workspace {
   Scratch bptr, bcol, bval, cursor, bmeta, rowf, rowlo, rowhi, symbin, numbin, rowcnt, rowoff, symrows,
       numrows, scan_tmp, stats, rowwords, wlist, statpart, ctl, cur, sortlist, rkeys, rrows, rvals, rkeys2,
       rhist, rhoff, rtmp, rvals2, mlow, mup, mlo_off, mup_off, wgbits, wgslot, plancnt, planwords, digest,
-      ckey, crep, akey, arep, bclass, aclass, tkey, trep, tplid, tinfo, tposoff, tpos, tsoff, ttuple, tpos2, tmatched, tgrid,
+      ckey, crep, cpat, akey, arep, bclass, aclass, tkey, trep, tplid, tinfo, tposoff, tpos, tsoff, ttuple, tpos2, tmatched, tgrid,
       vmeta, vac, ucol, uval, rrows2, goff, gtot;
   // words per row of the stored workgroup-bin bitmaps of the current call
   // (0: none stored)
@@ -112,7 +112,7 @@ struct Workspace {
                       &rowcnt, &rowoff,  &symrows, &numrows,  &scan_tmp, &stats, &rowwords, &wlist, &statpart, &ctl,
                       &cur,    &sortlist, &rkeys,  &rrows,    &rvals, &rkeys2, &rhist,    &rhoff, &rtmp,
                       &rvals2, &mlow,     &mup,    &mlo_off,  &mup_off, &wgbits, &wgslot, &plancnt, &planwords,
-                      &digest, &ckey,    &crep,  &akey,   &arep,   &bclass,  &aclass, &tkey,  &trep,   &tplid,
+                      &digest, &ckey,    &crep,  &cpat,  &akey,   &arep,   &bclass,  &aclass, &tkey,  &trep,   &tplid,
                       &tinfo,  &tposoff, &tpos,  &tsoff,  &ttuple, &tpos2, &tmatched, &tgrid, &vmeta, &vac,
                       &ucol,   &uval,    &rrows2, &goff,  &gtot};
     for (Scratch* s : all) f(*s);

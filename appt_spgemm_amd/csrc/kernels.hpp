@@ -250,6 +250,7 @@ constexpr int kTplMaxProducts = kTplMaxLen * kTplMaxLen;
 struct TplArgs {
   unsigned long long* ckey;  // B-row class table: kClassSlots hash keys (0 = empty)
   int32_t* crep;             // class representative (a B row)
+  int32_t* cpat;             // per B-row class: its length, then its representative's kTplMaxLen relative offsets
   unsigned long long* akey;  // A-side class table (unused when A's rows are B's rows)
   int32_t* arep;             // class representative (an A-side row)
   int32_t* bclass;           // per B row: class slot, or -1

@@ -226,7 +226,7 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
                                                       int4* __restrict__ meta, int32_t* __restrict__ flag,
                                                       const int32_t* __restrict__ matched,
                                                       int32_t* __restrict__ longrows,
-                                                      const int32_t* __restrict__ gate) {
+                                                      const int32_t* __restrict__ gate, int32_t* __restrict__ pat) {
   constexpr int G = kClsLanes;
   if (gate && *gate == 0) return;   // A-side rows whose classes are copied from B's
   __shared__ KeyCache kc;
@@ -298,6 +298,18 @@ __global__ void __launch_bounds__(256) k_class_insert(int32_t nrows, Rows R, con
     if (lane == 0 && r < nrows) {
       if (mine) rep[slot] = (int32_t)r;
       cls[r] = ok ? slot : -1;
+    }
+    if (pat) {
+      // the inserting row records the class's pattern (length, then its
+      // relative offsets) for the checks that follow
+      const int leader = (threadIdx.x % kWave) & ~(G - 1);
+      const int gslot = __shfl(slot, leader, kWave);
+      const bool gmine = __shfl(mine ? 1 : 0, leader, kWave) != 0;
+      if (gmine) {
+        int32_t* pp = pat + (size_t)gslot * (kTplMaxLen + 1);
+        if (lane == 0) pp[0] = m;
+        for (int u = lane; u < m; u += G) pp[1 + u] = (int32_t)((int64_t)col[se.x + u] - b);
+      }
     }
   }
 }
@@ -418,15 +430,17 @@ __global__ void __launch_bounds__(256) k_tpl_insert(int32_t n, const int32_t* __
 }
 
 // A*A (A's rows are B's rows): the class check of each row and its template
-// candidate in one pass.  Row r is compared with its class representative
-// (as k_class_verify) and, as an output row, hashed with its own class and
+// candidate in one pass.  Row r is compared entry by entry with its class's
+// pattern, which the class pass recorded from the representative (so the
+// check reads it by class id instead of through the representative's row
+// bounds), and, as an output row, hashed with its own class and
 // its entries' B classes (as k_tpl_insert).  The B classes are read while
 // other rows' checks may still clear theirs: a candidate whose B class is
 // cleared afterwards fails the pre-pass check, which requires every class
 // of the row to be set and equal to the template's.  Templates go ahead
 // unless the sampled patterns were mostly distinct.
 __global__ void __launch_bounds__(256) k_verify_tpl(int32_t n, const int32_t* __restrict__ ap,
-                                                    const int32_t* __restrict__ ac, const int32_t* __restrict__ crep,
+                                                    const int32_t* __restrict__ ac, const int32_t* __restrict__ cpat,
                                                     int32_t* cls, unsigned long long* __restrict__ tkeys,
                                                     int32_t* __restrict__ trep, int32_t* __restrict__ tplid,
                                                     const int32_t* __restrict__ gate) {
@@ -447,28 +461,24 @@ __global__ void __launch_bounds__(256) k_verify_tpl(int32_t n, const int32_t* __
     const int c = r < n ? cls[r] : -1;
     const int s = r < n ? ap[r] : 0, e = r < n ? ap[r + 1] : 0;
     const int m = e - s;
-    const int q = c >= 0 ? crep[c] : -1;
-    // compared with the representative (rows of other lengths differ)
-    const bool cmp = c >= 0 && q != (int)r;
-    int same = 1, sq = 0;
-    if (cmp) {
-      sq = ap[q];
-      same = ap[q + 1] - sq == m ? 1 : 0;
-    }
-    // one pass over the row's entries: the comparison with the
-    // representative's entries and the template hash of the entries' B
-    // classes, their loads in flight together.  The hash is used only if the
-    // row keeps its class.
+    // the class's pattern as its inserting row recorded it: length, then
+    // relative offsets (a classed row has at most kTplMaxLen entries)
+    const int32_t* cp = cpat + (size_t)(c >= 0 ? c : 0) * (kTplMaxLen + 1);
+    const int cm = c >= 0 ? cp[0] : 0;
+    // one pass over the row's entries: the comparison with the class's
+    // pattern and the template hash of the entries' B classes, their loads
+    // in flight together.  The hash is used only if the row keeps its class.
+    int same = 1;
     unsigned long long h = 0;
     int ok = c >= 0 ? 1 : 0;
-    if (c >= 0 && same) {
+    if (c >= 0) {
       for (int u0 = lane; u0 < m; u0 += G * kClsUnroll) {
         int k[kClsUnroll], x1[kClsUnroll];
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++) {
           const bool in = u0 + x * G < m;
           k[x] = in ? ac[s + u0 + x * G] : -1;
-          x1[x] = (in && cmp) ? ac[sq + u0 + x * G] : 0;
+          x1[x] = in ? cp[1 + u0 + x * G] : 0;
         }
         int bc[kClsUnroll];
 #pragma unroll
@@ -477,12 +487,13 @@ __global__ void __launch_bounds__(256) k_verify_tpl(int32_t n, const int32_t* __
 #pragma unroll
         for (int x = 0; x < kClsUnroll; x++)
           if (k[x] >= 0) {
-            if (cmp && (int64_t)k[x] - r != (int64_t)x1[x] - q) same = 0;
+            if ((int64_t)k[x] - r != (int64_t)x1[x]) same = 0;
             ok &= bc[x] >= 0 ? 1 : 0;
             h += tmix(((unsigned long long)(u0 + x * G + 1) << 32) ^ (unsigned long long)(uint32_t)bc[x]);
           }
       }
     }
+    if (cm != m) same = 0;
     same = gmin<G>(same);
     if (lane == 0 && c >= 0 && !same) cls[r] = -1;
     const int a = (c >= 0 && same) ? c : -1;
@@ -696,11 +707,11 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
       // entries) and a pass of its own for the longer rows
       const Rows RP{bptr, nullptr, 0};
       hipLaunchKernelGGL(k_class_insert<true>, dim3(grid(nb)), dim3(256), 0, s, nb, RP, bc, T.ckey, T.crep,
-                         T.bclass, bmeta, bflag, (const int32_t*)T.matched, longrows, nullptr);
+                         T.bclass, bmeta, bflag, (const int32_t*)T.matched, longrows, nullptr, T.cpat);
       launch_rowstats(nb, bptr, nullptr, nullptr, 0, bc, bmeta, bflag, s, kTplMaxLen, longrows);
     } else {
       hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(nb)), dim3(256), 0, s, nb, RB, bc, T.ckey, T.crep,
-                         T.bclass, nullptr, nullptr, (const int32_t*)T.matched, nullptr, nullptr);
+                         T.bclass, nullptr, nullptr, (const int32_t*)T.matched, nullptr, nullptr, T.cpat);
     }
     // A*A: the check runs inside the template pass below
     if (!a_is_b)
@@ -709,8 +720,8 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
   }
   if (n <= 0) return;
   if (a_is_b) {
-    hipLaunchKernelGGL(k_verify_tpl, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, T.crep, T.bclass, T.tkey, T.trep,
-                       T.tplid, (const int32_t*)T.matched);
+    hipLaunchKernelGGL(k_verify_tpl, dim3(grid(n)), dim3(256), 0, s, n, ap, ac, (const int32_t*)T.cpat, T.bclass,
+                       T.tkey, T.trep, T.tplid, (const int32_t*)T.matched);
     hipLaunchKernelGGL(k_tpl_build, dim3(kTplSlots), dim3(kBuildT), 0, s, ap, ac, bmeta, bc, T.row0, T.tkey, T.trep,
                        T.bclass, T.info, T.posoff, T.pos, T.soff, T.tuple, T.pos2, T.grid);
     return;
@@ -727,7 +738,7 @@ void launch_tpl_detect(const TplArgs& T, int32_t n, const int32_t* ap, const int
     hipLaunchKernelGGL(k_class_copy, dim3((unsigned)cb), dim3(256), 0, s, n, T.row0, (const int32_t*)T.bclass,
                        T.aclass, apat);
     hipLaunchKernelGGL(k_class_insert<false>, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.akey, T.arep, T.aclass,
-                       nullptr, nullptr, (const int32_t*)T.matched, nullptr, apat);
+                       nullptr, nullptr, (const int32_t*)T.matched, nullptr, apat, nullptr);
     hipLaunchKernelGGL(k_class_verify, dim3(grid(n)), dim3(256), 0, s, n, RA, ac, T.arep, T.aclass, T.matched + 1,
                        (const int32_t*)T.matched, apat);
   }
