@@ -363,6 +363,11 @@ void Engine::stage_operands(Workspace& W, const appt_csr_view& A, appt_op op, Op
     if (hs_->ctl[3] != 0)
       launch_transpose_general(n, nnz, A.rowptr, A.col, A.val, tp, tc, tv, cnt, off64, tmp, slist,
                                radix ? &rx : nullptr, asym, s);
+    // a symmetric pattern: B has A's row pointers and column array, so
+    // template detection takes the A*A path (one fused class and template
+    // pass; templates concern patterns only, the values stay B's)
+    else
+      O.a_is_b = true;
     O.bc = tc;
     O.bv = tv;
     bptr = tp;

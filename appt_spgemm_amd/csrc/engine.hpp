@@ -135,7 +135,7 @@ struct Operands {
   int upper = 0;  // AAtS: values formed for j >= i, lower part mirrored
   int32_t ncols = 0;  // bound on output column indices (a batch's are local to each block)
   int32_t row0 = 0;   // global row index of A-side row 0 (row panels)
-  bool a_is_b = false;  // the A-side rows are B's rows (A*A over all rows)
+  bool a_is_b = false;  // the A-side rows have B's rows' structure (A*A, or A*A^T of a symmetric pattern)
   bool gram = false;    // AAtS as a triangle-only Gram: B rows are per-entry column ranges (gram.hip)
   bool gram_lower = false;  // the Gram's triangle: j <= i (else j >= i)
   // B's row pointers; bmeta_pending: the B-row metadata is left to the
