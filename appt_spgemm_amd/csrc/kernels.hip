@@ -777,6 +777,9 @@ __device__ __forceinline__ int sort_bin_of(const BinParams& P, int64_t f, int64_
 // receives the LDS sizing statistic: span words (dense) or key bound (hash).
 // Rows whose span does not fit a dense window go to a register-sort bin
 // when they have at most kSortMax products, else to the word hash.
+constexpr int64_t kWgMaxProducts = 65536;      // workgroup symbolic rows: at most this many products,
+constexpr int64_t kWgWideWords = 8192;         // and over spans of more than this many words
+constexpr int64_t kWgWideMaxProducts = 4096;   // at most this many
 __device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t words, int64_t nA, int64_t* stat) {
   *stat = 0;
   if (f <= 0 || words <= 0) return 0;
@@ -803,8 +806,15 @@ __device__ __forceinline__ int sym_bin_of(const BinParams& P, int64_t f, int64_t
     *stat = words;
     return 1 + (lc * 2 + 0) * 4 + 3;
   }
-  if (words * 4 <= P.wg_bytes) { *stat = words; return kBinWgBitmap; }
-  if (hash_bytes <= P.wg_bytes) { *stat = keys; return kBinWgHash; }
+  // rows of many products take the heavy kernel even when their span fits a
+  // workgroup: its products are dealt over 1024 lanes, several in flight
+  // each, where a workgroup row walks each short B row with one lane; for
+  // wide spans the workgroup bitmap's clear adds to that (R-MAT 2^20 A*A
+  // row-pointer plan 581 -> 101 ms, 2^16 A*A 7.80 -> 7.38 ms)
+  if (f <= kWgMaxProducts && (words <= kWgWideWords || f <= kWgWideMaxProducts)) {
+    if (words * 4 <= P.wg_bytes) { *stat = words; return kBinWgBitmap; }
+    if (hash_bytes <= P.wg_bytes) { *stat = keys; return kBinWgHash; }
+  }
   *stat = words;
   return kBinHeavy;
 }
