@@ -480,6 +480,7 @@ HeavyArgs Engine::heavy_args(Workspace& W, const Operands& O, const Config& cfg)
   const int64_t dyn = lds_max_ - 24 * 1024;
   h.wc_sym = heavy_window_cols(dyn, cfg.heavy_cap);
   heavy_num_shape(dyn, cfg.heavy_cap, &h.wc_num, &h.vcap, &h.dcols);
+  h.adapt = cfg.heavy_cap == 0;
   h.work = as<int32_t>(W.ctl.p);
   h.bflag = h.work + 2;
   h.fault = h.work + kCtlHeavyFault;

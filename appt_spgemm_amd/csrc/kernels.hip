@@ -2297,10 +2297,17 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_sym(RowArgs A, int32_t wc, in
 // searches, scans and barriers).  Measured on R-MAT rows: rows spanning more
 // than kHeavySparseSpan dense windows run faster sparse (scale 21: 1.87 s
 // against 3.22 s), narrower ones dense (scale 16: 5.6 ms against 6.3 ms).
-// Cursors carry over between windows as in k_heavy_sym.  LDS: bits (wcols / 32
-// words), then either prefix counts + vcap slots (sparse) or dcols slots
-// (dense).
+// With lbytes > 0 (the launch's dynamic LDS, no tuning cap) each sparse
+// window takes its width and slots from the row's remaining outputs and
+// columns (heavy_row_shape): a scattered row (some 10^4 outputs over 2^21
+// columns) gets a few windows of several 10^5 columns instead of 16 of
+// 131,072, and a dense row windows whose structure pass ends near the cut.
+// A cut caps the next window at 3/2 of the columns it took.
+// Cursors carry over between windows as in k_heavy_sym.  LDS: bits (window
+// columns / 32 words), then either prefix counts + value slots (sparse) or
+// dcols slots (dense).
 __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols, int32_t vcap, int32_t dcols,
+                                                       int64_t lbytes,
                                                        int32_t* __restrict__ work, const int32_t* __restrict__ bflag,
                                                        int32_t* __restrict__ cur, const int64_t* __restrict__ rowoff,
                                                        int32_t* __restrict__ ccol, double* __restrict__ cval,
@@ -2309,8 +2316,6 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
   __shared__ HeavyStage hs;
   __shared__ int32_t cut_word;
   uint32_t* bits = (uint32_t*)lds_raw;
-  uint32_t* pre = bits + wcols / 32;                     // sparse windows
-  double* svals = (double*)(lds_raw + (size_t)(wcols / 32) * 8);
   double* dvals = (double*)(lds_raw + (size_t)(wcols / 32) * 4);  // dense windows
   const int tid = threadIdx.x;
   const bool sorted = bflag[0] == 0;
@@ -2327,12 +2332,17 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
     const int64_t end = rowoff[i + 1];
     // window shape of this row (uniform across the block)
     const bool sparse = (int64_t)hi - lo + 1 > kHeavySparseSpan * dcols;
-    double* vals = sparse ? svals : dvals;
-    const int vlim = sparse ? vcap : dcols;
-    int64_t w0 = lo;
+    int64_t w0 = lo, wlim = 0;
     bool first = true;
     while (w0 <= hi) {
-      const int64_t wspan = sparse ? wcols : dcols;
+      // sparse windows: sized per window from the row's remaining outputs
+      // and columns (heavy_row_shape) unless a tuning cap fixed the shape
+      int32_t wcr = wcols, vcr = vcap;
+      if (sparse && lbytes > 0) heavy_row_shape(lbytes, end - out, (int64_t)hi + 1 - w0, wlim, &wcr, &vcr);
+      uint32_t* pre = bits + wcr / 32;
+      double* vals = sparse ? (double*)(lds_raw + (size_t)(wcr / 32) * 8) : dvals;
+      const int vlim = sparse ? vcr : dcols;
+      const int64_t wspan = sparse ? wcr : dcols;
       const int64_t w1 = (w0 + wspan) < ((int64_t)hi + 1) ? (w0 + wspan) : ((int64_t)hi + 1);
       const int words = (int)((w1 - w0 + 31) >> 5);
       for (int t = tid; t < words; t += kHeavyT) bits[t] = 0u;
@@ -2352,7 +2362,7 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
           const int64_t p0 = run + block_scan_excl<int64_t>((int64_t)pc, hs.red, &tot);
           if (w < words) {
             pre[w] = (uint32_t)(p0 < (int64_t)0xffffffffu ? p0 : 0xffffffffu);
-            if (p0 <= vcap && p0 + pc > vcap) cut_word = w;  // unique: counts are monotone
+            if (p0 <= vcr && p0 + pc > vcr) cut_word = w;  // unique: counts are monotone
           }
           run += tot;
         }
@@ -2366,7 +2376,7 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
           break;
         }
         if (cw < words) wcut = w0 + (int64_t)cw * 32;
-        const int nvals = cw < words ? (int)pre[cw] : (int)(run < vcap ? run : vcap);
+        const int nvals = cw < words ? (int)pre[cw] : (int)(run < vcr ? run : vcr);
         for (int t = tid; t < nvals; t += kHeavyT) vals[t] = 0.0;
         __syncthreads();
         // values of [w0, wcut); cursors move to the cut
@@ -2410,6 +2420,7 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
       if (sparse) run = cw < words ? (int64_t)pre[cw] : (int64_t)pre[words - 1] + __popc(bits[words - 1]);
       out += run;
       __syncthreads();
+      if (sparse) wlim = heavy_next_wlim(cw < words, wcut - w0, wlim);
       w0 = wcut;
       first = false;
     }
@@ -3084,7 +3095,8 @@ static void heavy_launch(bool vals, const RowArgs& a, const HeavyArgs& h, int32_
   launch_fill_i32(work, 1, 0, s);
   if (vals) {
     set_dyn_lds((const void*)k_heavy_num, lds);
-    hipLaunchKernelGGL(k_heavy_num, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_num, h.vcap, h.dcols, work,
+    hipLaunchKernelGGL(k_heavy_num, dim3(blocks), dim3(kHeavyT), lds, s, a, h.wc_num, h.vcap, h.dcols,
+                       h.adapt ? (int64_t)lds : (int64_t)0, work,
                        h.bflag, h.cur, rowoff, ccol, cval, h.fault);
   } else {
     set_dyn_lds((const void*)k_heavy_sym, lds);

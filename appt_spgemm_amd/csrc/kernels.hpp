@@ -205,6 +205,7 @@ constexpr int kWordListCap = 64;  // word-list slots per output row (stride of R
 // numeric kernel sets (to 1) if a sparse window cannot advance.
 struct HeavyArgs {
   int32_t wc_sym, wc_num, vcap, dcols;  // window columns (symbolic / numeric sparse), slots, dense width
+  bool adapt;                           // numeric sparse windows shaped per row (no tuning cap)
   int32_t* work;
   const int32_t* bflag;
   int32_t* cur;

@@ -2,9 +2,11 @@
 //
 // Restates the window loop of k_heavy_num (appt_spgemm_amd/csrc/kernels.hip)
 // over a row's output columns, using the product's own shape rules
-// (heavy_shape.hpp), and checks that every window advances, that a sparse
-// window never holds more than vcap outputs, and that the windows together
-// emit every output exactly once.  Random rows: span up to 2^22 columns,
+// (heavy_shape.hpp: the fixed shape under a tuning cap, else each sparse
+// window sized from the row's remaining outputs and columns), and checks that
+// every window advances, that a sparse window never holds more than its value
+// slots, that a window's bits, prefix counts and slots fit the launch's LDS,
+// and that the windows together emit every output exactly once.  Random rows: span up to 2^22 columns,
 // densities from one column in 10^4 to fully dense, clustered or uniform.
 // Prints one line per configuration and "OK <rows>" at the end; exits 1 on
 // the first violated invariant.
@@ -62,15 +64,23 @@ int main() {
       if (vcap < kHeavyMinSlots || dcols < kHeavyMinSlots || wcols < kHeavyMinCols || (wcols & 31) || (dcols & 31))
         return fail("shape floor", vcap, dcols);
       if (cap == 0 && heavy_lds_bytes(true, wcols, vcap, dcols) > dyn) return fail("shape exceeds LDS", wcols, dyn);
-      std::printf("lds %lld cap %d: wcols %d vcap %d dcols %d\n", (long long)lds, cap, wcols, vcap, dcols);
+      // the launch's dynamic LDS, which the per-row shapes divide
+      const int64_t lbytes = heavy_lds_bytes(true, wcols, vcap, dcols);
+      int64_t nwin = 0;
       for (const Row& r : rows) {
         const int64_t lo = r.lo, hi = r.hi, span = hi - lo + 1;
         auto outputs = [&](int64_t a, int64_t b) { return (int64_t)r.pre[(size_t)(b - lo)] - r.pre[(size_t)(a - lo)]; };
         const int64_t total = r.pre[(size_t)span];
         const bool sparse = hi - lo + 1 > kHeavySparseSpan * dcols;
-        int64_t w0 = lo, emitted = 0, windows = 0;
+        int64_t w0 = lo, emitted = 0, windows = 0, wlim = 0;
         while (w0 <= hi) {
-          const int64_t wspan = sparse ? wcols : dcols;
+          int32_t wcr = wcols, vcr = vcap;
+          if (sparse && cap == 0) {
+            heavy_row_shape(lbytes, total - emitted, hi + 1 - w0, wlim, &wcr, &vcr);
+            if (wcr < kHeavyMinCols || (wcr & 31) || vcr < kHeavyMinSlots) return fail("row shape floor", wcr, vcr);
+            if ((int64_t)(wcr / 32) * 8 + (int64_t)vcr * 8 > lbytes) return fail("row shape exceeds LDS", wcr, vcr);
+          }
+          const int64_t wspan = sparse ? wcr : dcols;
           const int64_t w1 = (w0 + wspan) < (hi + 1) ? (w0 + wspan) : (hi + 1);
           const int64_t words = (w1 - w0 + 31) >> 5;
           int64_t wcut = w1;
@@ -80,7 +90,7 @@ int main() {
             for (int64_t w = 0; w < words; w++) {
               const int64_t a = w0 + w * 32, b = (a + 32) < w1 ? (a + 32) : w1;
               const int64_t pc = outputs(a, b);
-              if (run <= vcap && run + pc > vcap) { cut = w; break; }
+              if (run <= vcr && run + pc > vcr) { cut = w; break; }
               run += pc;
             }
             if (cut == 0) return fail("sparse window cannot advance", w0, vcap);
@@ -88,15 +98,19 @@ int main() {
           }
           if (wcut <= w0) return fail("window does not advance", w0, wcut);
           const int64_t in = outputs(w0, wcut);
-          if (sparse && in > vcap) return fail("sparse window over capacity", in, vcap);
+          if (sparse && in > vcr) return fail("sparse window over capacity", in, vcr);
           if (!sparse && wcut - w0 > dcols) return fail("dense window over width", wcut - w0, dcols);
           emitted += in;
+          if (sparse) wlim = heavy_next_wlim(wcut < w1, wcut - w0, wlim);
           w0 = wcut;
           if (++windows > span / 32 + 2) return fail("too many windows", windows, span);
         }
         if (emitted != total) return fail("outputs lost or repeated", emitted, total);
+        nwin += windows;
         checked++;
       }
+      std::printf("lds %lld cap %d: wcols %d vcap %d dcols %d, %lld windows over %zu rows\n", (long long)lds, cap,
+                  wcols, vcap, dcols, (long long)nwin, rows.size());
     }
   }
   std::printf("OK %lld\n", (long long)checked);
