@@ -2105,7 +2105,18 @@ struct HeavyStage {
   int32_t qs[kHeavyT];   // segment start (B position)
   double a[kHeavyT];     // a_ik
   int64_t red[17];
+  int64_t tot;           // products of the last staged chunk
   int32_t row;
+};
+
+// A heavy row's entries kept in LDS across its windows (rows of at most
+// kHeavyT entries over sorted B rows): cursor, B-row end and B-row maximum
+// column per entry, and a_ik in HeavyStage::a, loaded once per row instead of
+// an A-column, B-metadata and cursor load chain per window and pass.
+struct HeavyRes {
+  int32_t* cur;          // nullptr: not resident (cursors in global memory)
+  int32_t* be;
+  int32_t* bw;
 };
 
 // B ranges of a chunk of m A entries (from position cb) inside the window
@@ -2115,7 +2126,7 @@ struct HeavyStage {
 // qe the first B position at or past wend; keep moves the cursor there.
 template <bool VALS>
 __device__ __forceinline__ void heavy_stage_ranges(const RowArgs& A, HeavyStage& hs, int cb, int m, int64_t wend,
-                                                   bool carry, bool first, bool keep, int32_t* cur) {
+                                                   bool carry, bool first, bool keep, int32_t* cur, HeavyRes res) {
   const int tid = threadIdx.x;
   // each entry's B range inside the window, T lanes per entry (T = 1024 /
   // m rounded down to a power of two, at most 64): the window end is found
@@ -2127,10 +2138,21 @@ __device__ __forceinline__ void heavy_stage_ranges(const RowArgs& A, HeavyStage&
   const int ent = tid >> lg, sub = tid & (T - 1);
   if (ent < m) {
     const int p = cb + ent;
-    const int4 bm = A.bmeta[A.ac[p]];
-    const int qs = (carry && !first) ? cur[p] : bm.x;
-    int l = qs, h = bm.y;
-    if (carry && l < h && (int64_t)bm.w >= wend) {
+    int qs, h;
+    int64_t bmax;
+    if (res.cur) {
+      // resident: the group's lanes read the cursor before its lane 0 moves it
+      qs = res.cur[ent];
+      h = res.be[ent];
+      bmax = res.bw[ent];
+    } else {
+      const int4 bm = A.bmeta[A.ac[p]];
+      qs = (carry && !first) ? cur[p] : bm.x;
+      h = bm.y;
+      bmax = bm.w;
+    }
+    int l = qs;
+    if (carry && l < h && bmax >= wend) {
       // first position of the B row at or past the window end; l, h are
       // uniform over the group, whose lanes are active together
       const int base = (tid & (kWave - 1)) & ~(T - 1);
@@ -2158,8 +2180,11 @@ __device__ __forceinline__ void heavy_stage_ranges(const RowArgs& A, HeavyStage&
     if (sub == 0) {
       hs.qs[ent] = qs;
       hs.off[ent] = l;
-      if (carry && keep) cur[p] = l;
-      if (VALS) hs.a[ent] = A.av[p];
+      if (carry && keep) {
+        if (res.cur) res.cur[ent] = l;
+        else cur[p] = l;
+      }
+      if (VALS && !res.cur) hs.a[ent] = A.av[p];
     }
   }
 }
@@ -2170,19 +2195,31 @@ constexpr int kHeavyU = 4;  // products in flight per thread in heavy_products
 template <int MODE>
 __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs, int i, int s, int e, int64_t w0,
                                                int64_t wend, bool carry, bool first, bool keep, int32_t* cur,
-                                               uint32_t* bits, const uint32_t* pre, double* vals) {
-  // carry: B rows sorted and cursors kept across windows (heavy_stage_ranges)
+                                               uint32_t* bits, const uint32_t* pre, double* vals,
+                                               HeavyRes res = HeavyRes{nullptr, nullptr, nullptr},
+                                               bool reuse = false) {
+  // carry: B rows sorted and cursors kept across windows (heavy_stage_ranges).
+  // reuse (resident rows, one chunk): hs still holds this window's ranges and
+  // their scan from the previous call, whose window ended at the same column.
   const int tid = threadIdx.x;
   for (int cb = s; cb < e; cb += kHeavyT) {
     const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
-    heavy_stage_ranges<MODE != kHeavyOr>(A, hs, cb, m, wend, carry, first, keep, cur);
-    __syncthreads();
-    const int qs = tid < m ? hs.qs[tid] : 0;
-    const int qe = tid < m ? (int)hs.off[tid] : 0;
     int64_t tot;
-    const int64_t ex = block_scan_excl<int64_t>((int64_t)(qe - qs), hs.red, &tot);
-    hs.off[tid] = ex;
-    __syncthreads();
+    if (!reuse) {
+      heavy_stage_ranges<MODE != kHeavyOr>(A, hs, cb, m, wend, carry, first, keep, cur, res);
+      __syncthreads();
+      const int qs = tid < m ? hs.qs[tid] : 0;
+      const int qe = tid < m ? (int)hs.off[tid] : 0;
+      const int64_t ex = block_scan_excl<int64_t>((int64_t)(qe - qs), hs.red, &tot);
+      hs.off[tid] = ex;
+      if (tid == 0) hs.tot = tot;
+      __syncthreads();
+    } else {
+      tot = hs.tot;
+      // cursors move past the window: start + segment length
+      if (keep && tid < m)
+        res.cur[tid] = hs.qs[tid] + (int)((tid + 1 < m ? hs.off[tid + 1] : tot) - hs.off[tid]);
+    }
     // kHeavyU products per thread per step (t, t + 1024, ...): their owner
     // searches, gathers and LDS updates are issued together.  The owner of
     // t is the last entry whose segment starts at or before t, found by a
@@ -2332,13 +2369,33 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
     const int64_t end = rowoff[i + 1];
     // window shape of this row (uniform across the block)
     const bool sparse = (int64_t)hi - lo + 1 > kHeavySparseSpan * dcols;
+    // sparse rows of one chunk over sorted B rows keep their entries in LDS
+    // (HeavyRes) behind the windows, whose shapes take the rest
+    const int mrow = e - s;
+    const bool resident = sparse && lbytes > 0 && sorted && mrow > 0 && mrow <= kHeavyT;
+    const int64_t rbytes = resident ? (((int64_t)12 * mrow + 15) & ~(int64_t)15) : 0;
+    HeavyRes res{nullptr, nullptr, nullptr};
+    if (resident) {
+      res.cur = (int32_t*)(lds_raw + (size_t)(lbytes - rbytes));
+      res.be = res.cur + mrow;
+      res.bw = res.be + mrow;
+      if (tid < mrow) {
+        const int4 bm = A.bmeta[A.ac[s + tid]];
+        res.cur[tid] = bm.x;
+        res.be[tid] = bm.y;
+        res.bw[tid] = bm.w;
+        hs.a[tid] = A.av[s + tid];
+      }
+      __syncthreads();
+    }
     int64_t w0 = lo, wlim = 0;
     bool first = true;
     while (w0 <= hi) {
       // sparse windows: sized per window from the row's remaining outputs
       // and columns (heavy_row_shape) unless a tuning cap fixed the shape
       int32_t wcr = wcols, vcr = vcap;
-      if (sparse && lbytes > 0) heavy_row_shape(lbytes, end - out, (int64_t)hi + 1 - w0, wlim, &wcr, &vcr);
+      if (sparse && lbytes > 0)
+        heavy_row_shape(lbytes - rbytes, end - out, (int64_t)hi + 1 - w0, wlim, &wcr, &vcr);
       uint32_t* pre = bits + wcr / 32;
       double* vals = sparse ? (double*)(lds_raw + (size_t)(wcr / 32) * 8) : dvals;
       const int vlim = sparse ? vcr : dcols;
@@ -2351,20 +2408,23 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
       if (sparse) {
         if (tid == 0) cut_word = words;
         // structure of [w0, w1); cursors are not advanced
-        heavy_products<kHeavyOr>(A, hs, i, s, e, w0, w1, sorted, first, false, cur, bits, pre, vals);
+        heavy_products<kHeavyOr>(A, hs, i, s, e, w0, w1, sorted, first, false, cur, bits, pre, vals, res);
         // slots: exclusive prefix of the words' popcounts; the cut is the
-        // first word whose entries would not all fit in vcap
-        int64_t run = 0;
-        for (int wb = 0; wb < words; wb += kHeavyT) {
-          const int w = wb + tid;
-          const int pc = w < words ? __popc(bits[w]) : 0;
-          int64_t tot;
-          const int64_t p0 = run + block_scan_excl<int64_t>((int64_t)pc, hs.red, &tot);
-          if (w < words) {
-            pre[w] = (uint32_t)(p0 < (int64_t)0xffffffffu ? p0 : 0xffffffffu);
-            if (p0 <= vcr && p0 + pc > vcr) cut_word = w;  // unique: counts are monotone
-          }
-          run += tot;
+        // first word whose entries would not all fit in vcap.  Each thread
+        // takes a run of consecutive words, so a window of any width costs
+        // one block scan
+        const int per = (words + kHeavyT - 1) / kHeavyT;
+        const int wa = tid * per < words ? tid * per : words;
+        const int wz = wa + per < words ? wa + per : words;
+        int64_t loc = 0;
+        for (int w = wa; w < wz; w++) loc += __popc(bits[w]);
+        int64_t run;
+        int64_t p0 = block_scan_excl<int64_t>(loc, hs.red, &run);
+        for (int w = wa; w < wz; w++) {
+          const int pc = __popc(bits[w]);
+          pre[w] = (uint32_t)(p0 < (int64_t)0xffffffffu ? p0 : 0xffffffffu);
+          if (p0 <= vcr && p0 + pc > vcr) cut_word = w;  // unique: counts are monotone
+          p0 += pc;
         }
         __syncthreads();
         cw = cut_word;
@@ -2379,8 +2439,10 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
         const int nvals = cw < words ? (int)pre[cw] : (int)(run < vcr ? run : vcr);
         for (int t = tid; t < nvals; t += kHeavyT) vals[t] = 0.0;
         __syncthreads();
-        // values of [w0, wcut); cursors move to the cut
-        heavy_products<kHeavySlot>(A, hs, i, s, e, w0, wcut, sorted, first, true, cur, bits, pre, vals);
+        // values of [w0, wcut); cursors move to the cut.  A resident row's
+        // uncut window reuses the structure pass's ranges and their scan
+        heavy_products<kHeavySlot>(A, hs, i, s, e, w0, wcut, sorted, first, true, cur, bits, pre, vals, res,
+                                   resident && cw == words);
       } else {
         for (int t = tid; t < (int)(w1 - w0); t += kHeavyT) vals[t] = 0.0;
         heavy_products<kHeavyDense>(A, hs, i, s, e, w0, w1, sorted, first, true, cur, bits, pre, vals);
