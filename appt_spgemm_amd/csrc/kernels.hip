@@ -17,6 +17,7 @@
 #include "device_common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace appt {
 
@@ -2192,12 +2193,26 @@ __device__ __forceinline__ void heavy_stage_ranges(const RowArgs& A, HeavyStage&
 enum HeavyMode { kHeavyOr = 0, kHeavySlot = 1, kHeavyDense = 2 };
 constexpr int kHeavyU = 4;  // products in flight per thread in heavy_products
 
+// A resident row's structure pass over a window of at most kHoldSteps steps
+// of the product loop (kHoldSteps * kHeavyU * kHeavyT products) also forms
+// the products' values and keeps them, with their columns and owners, in the
+// thread's registers: once the window's slots (and its cut) are known they
+// are added from there, with no value pass.
+constexpr int kHoldSteps = 2;                       // steps a window's held products may take
+constexpr int kHoldN = kHoldSteps * kHeavyU;         // products held per thread
+struct HeavyHold {
+  int32_t j[kHoldN];    // column, -1: none
+  double v[kHoldN];
+  int32_t l[kHoldN];    // owner (A entry of the chunk)
+  bool on;              // every product of the window is held
+};
+
 template <int MODE>
 __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs, int i, int s, int e, int64_t w0,
                                                int64_t wend, bool carry, bool first, bool keep, int32_t* cur,
                                                uint32_t* bits, const uint32_t* pre, double* vals,
                                                HeavyRes res = HeavyRes{nullptr, nullptr, nullptr},
-                                               bool reuse = false) {
+                                               bool reuse = false, HeavyHold* hold = nullptr) {
   // carry: B rows sorted and cursors kept across windows (heavy_stage_ranges).
   // reuse (resident rows, one chunk): hs still holds this window's ranges and
   // their scan from the previous call, whose window ended at the same column.
@@ -2220,13 +2235,21 @@ __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs,
       if (keep && tid < m)
         res.cur[tid] = hs.qs[tid] + (int)((tid + 1 < m ? hs.off[tid + 1] : tot) - hs.off[tid]);
     }
+    if (hold) {
+      hold->on = res.cur != nullptr && e - s <= kHeavyT && tot <= (int64_t)kHoldN * kHeavyT;
+#pragma unroll
+      for (int u = 0; u < kHoldN; u++) hold->j[u] = -1;
+    }
     // kHeavyU products per thread per step (t, t + 1024, ...): their owner
     // searches, gathers and LDS updates are issued together.  The owner of
     // t is the last entry whose segment starts at or before t, found by a
     // fixed-depth search (l = 0 holds: hs.off[0] == 0).
     int depth = 0;
     while ((1 << depth) < m) depth++;
-    for (int64_t t0 = tid; t0 < tot; t0 += (int64_t)kHeavyU * kHeavyT) {
+    // one step from t0; with VH the values are formed for the structure
+    // pass too and the step's columns, values and owners are returned
+    auto step = [&](int64_t t0, auto vh, int32_t* jo, double* vo, int32_t* lo) {
+      constexpr bool VH = decltype(vh)::value;
       int64_t tt[kHeavyU];
       int l[kHeavyU];
 #pragma unroll
@@ -2249,7 +2272,15 @@ __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs,
         const bool ok = tt[u] < tot;
         q[u] = hs.qs[l[u]] + (ok ? (int)(tt[u] - hs.off[l[u]]) : 0);
         j[u] = ok ? A.bc[q[u]] : -1;
-        if (MODE != kHeavyOr) pv[u] = ok ? hs.a[l[u]] * A.bv[q[u]] : 0.0;
+        if (MODE != kHeavyOr || VH) pv[u] = ok ? hs.a[l[u]] * A.bv[q[u]] : 0.0;
+      }
+      if constexpr (VH) {
+#pragma unroll
+        for (int u = 0; u < kHeavyU; u++) {
+          jo[u] = j[u];
+          vo[u] = pv[u];
+          lo[u] = l[u];
+        }
       }
 #pragma unroll
       for (int u = 0; u < kHeavyU; u++) {
@@ -2266,6 +2297,19 @@ __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs,
           atomicAdd(&vals[pos], pv[u]);
         }
       }
+    };
+    if (MODE == kHeavyOr && hold && hold->on) {
+      // every step of the window, unrolled so the held products stay in
+      // registers
+#pragma unroll
+      for (int it = 0; it < kHoldSteps; it++) {
+        const int64_t t0 = tid + (int64_t)it * kHeavyU * kHeavyT;
+        if (t0 < tot)
+          step(t0, std::true_type{}, hold->j + it * kHeavyU, hold->v + it * kHeavyU, hold->l + it * kHeavyU);
+      }
+    } else {
+      for (int64_t t0 = tid; t0 < tot; t0 += (int64_t)kHeavyU * kHeavyT)
+        step(t0, std::false_type{}, nullptr, nullptr, nullptr);
     }
     __syncthreads();
   }
@@ -2407,8 +2451,12 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
       int64_t wcut = w1;
       if (sparse) {
         if (tid == 0) cut_word = words;
-        // structure of [w0, w1); cursors are not advanced
-        heavy_products<kHeavyOr>(A, hs, i, s, e, w0, w1, sorted, first, false, cur, bits, pre, vals, res);
+        // structure of [w0, w1); cursors are not advanced.  A resident
+        // row's window of one product step keeps its values (HeavyHold)
+        HeavyHold hold;
+        hold.on = false;
+        heavy_products<kHeavyOr>(A, hs, i, s, e, w0, w1, sorted, first, false, cur, bits, pre, vals, res, false,
+                                 &hold);
         // slots: exclusive prefix of the words' popcounts; the cut is the
         // first word whose entries would not all fit in vcap.  Each thread
         // takes a run of consecutive words, so a window of any width costs
@@ -2439,47 +2487,104 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
         const int nvals = cw < words ? (int)pre[cw] : (int)(run < vcr ? run : vcr);
         for (int t = tid; t < nvals; t += kHeavyT) vals[t] = 0.0;
         __syncthreads();
-        // values of [w0, wcut); cursors move to the cut.  A resident row's
-        // uncut window reuses the structure pass's ranges and their scan
-        heavy_products<kHeavySlot>(A, hs, i, s, e, w0, wcut, sorted, first, true, cur, bits, pre, vals, res,
-                                   resident && cw == words);
+        if (hold.on) {
+          // held products: those before the cut go to their slots.  The
+          // cursors move past the window (start + segment length) or, for a
+          // cut window, past each entry's products before the cut, counted
+          // into hs.off, whose scan is not needed any more: B rows are
+          // sorted, so those products are a prefix of the entry's segment
+          const bool cut = cw < words;
+          int32_t* cnt = (int32_t*)hs.off;
+          if (cut) {
+            if (tid < mrow) cnt[tid] = 0;
+            __syncthreads();
+          }
+#pragma unroll
+          for (int u = 0; u < kHoldN; u++) {
+            const int64_t jj = hold.j[u];
+            if (jj >= w0 && jj < wcut) {
+              if (!A.upper || jj >= i) {
+                const uint32_t b = (uint32_t)(jj - w0);
+                const uint32_t w = b >> 5;
+                const uint32_t pos = pre[w] + __popc(bits[w] & ((1u << (b & 31u)) - 1u));
+                atomicAdd(&vals[pos], hold.v[u]);
+              }
+              if (cut) atomicAdd(&cnt[hold.l[u]], 1);
+            }
+          }
+          if (!cut && tid < mrow)
+            res.cur[tid] = hs.qs[tid] + (int)((tid + 1 < mrow ? hs.off[tid + 1] : hs.tot) - hs.off[tid]);
+          __syncthreads();
+          if (cut) {
+            if (tid < mrow) res.cur[tid] = hs.qs[tid] + cnt[tid];
+            __syncthreads();
+          }
+        } else {
+          // values of [w0, wcut); cursors move to the cut.  A resident row's
+          // uncut window reuses the structure pass's ranges and their scan
+          heavy_products<kHeavySlot>(A, hs, i, s, e, w0, wcut, sorted, first, true, cur, bits, pre, vals, res,
+                                     resident && cw == words);
+        }
       } else {
         for (int t = tid; t < (int)(w1 - w0); t += kHeavyT) vals[t] = 0.0;
         heavy_products<kHeavyDense>(A, hs, i, s, e, w0, w1, sorted, first, true, cur, bits, pre, vals);
       }
-      // columns and values of the words before the cut, in order; sparse
-      // windows have their word prefixes, dense ones scan the words first
+      // columns and values of the words before the cut, in order
       int64_t run = 0;
-      for (int wb = 0; wb < cw; wb += kHeavyT) {
-        const int w = wb + tid;
-        const uint32_t x = w < cw ? bits[w] : 0u;
-        int64_t tot, p0;
-        if (sparse) {
-          p0 = w < cw ? (int64_t)pre[w] : 0;
-          tot = 0;
-        } else {
-          p0 = run + block_scan_excl<int64_t>((int64_t)__popc(x), hs.red, &tot);
-        }
-        if (w < cw) {
-          uint32_t y = x;
-          int64_t o = out + p0;
-          int k = 0;
-          while (y) {
-            const int bpos = __builtin_ctz(y);
-            // sparse: slot p0 + k (the prefix of this word); dense: column offset
-            const int slot = sparse ? (int)p0 + k : w * 32 + bpos;
-            if (o < end && slot < vlim) {
-              ccol[o] = (int32_t)(w0 + (int64_t)w * 32 + bpos);
-              cval[o] = vals[slot];
+      if (sparse) {
+        // one thread per slot: slot k lies in the last word w before the cut
+        // with pre[w] <= k (a search over the prefixes) at the (k - pre[w])-th
+        // set bit of that word, so consecutive slots are stored to
+        // consecutive output positions, whatever the words' densities
+        run = cw < words ? (int64_t)pre[cw] : (int64_t)pre[words - 1] + __popc(bits[words - 1]);
+        int dep = 0;
+        while ((1 << dep) < cw) dep++;
+        for (int64_t k = tid; k < run; k += kHeavyT) {
+          int w = 0;
+          for (int d = dep - 1; d >= 0; d--) {
+            const int cand = w + (1 << d);
+            if (cand < cw && (int64_t)pre[cand] <= k) w = cand;
+          }
+          uint32_t x = bits[w];
+          int rk = (int)(k - (int64_t)pre[w]), bpos = 0;
+#pragma unroll
+          for (int sh = 16; sh > 0; sh >>= 1) {
+            const int c = __popc(x & ((1u << sh) - 1u));
+            if (rk >= c) {
+              rk -= c;
+              x >>= sh;
+              bpos += sh;
             }
-            o++;
-            k++;
-            y &= y - 1u;
+          }
+          const int64_t o = out + k;
+          if (o < end && k < vlim) {
+            ccol[o] = (int32_t)(w0 + (int64_t)w * 32 + bpos);
+            cval[o] = vals[k];
           }
         }
-        run += tot;
+      } else {
+        for (int wb = 0; wb < cw; wb += kHeavyT) {
+          const int w = wb + tid;
+          const uint32_t x = w < cw ? bits[w] : 0u;
+          int64_t tot;
+          const int64_t p0 = run + block_scan_excl<int64_t>((int64_t)__popc(x), hs.red, &tot);
+          if (w < cw) {
+            uint32_t y = x;
+            int64_t o = out + p0;
+            while (y) {
+              const int bpos = __builtin_ctz(y);
+              const int slot = w * 32 + bpos;   // dense: the column offset
+              if (o < end && slot < vlim) {
+                ccol[o] = (int32_t)(w0 + (int64_t)w * 32 + bpos);
+                cval[o] = vals[slot];
+              }
+              o++;
+              y &= y - 1u;
+            }
+          }
+          run += tot;
+        }
       }
-      if (sparse) run = cw < words ? (int64_t)pre[cw] : (int64_t)pre[words - 1] + __popc(bits[words - 1]);
       out += run;
       __syncthreads();
       if (sparse) wlim = heavy_next_wlim(cw < words, wcut - w0, wlim);
