@@ -2212,10 +2212,14 @@ __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs,
                                                int64_t wend, bool carry, bool first, bool keep, int32_t* cur,
                                                uint32_t* bits, const uint32_t* pre, double* vals,
                                                HeavyRes res = HeavyRes{nullptr, nullptr, nullptr},
-                                               bool reuse = false, HeavyHold* hold = nullptr) {
+                                               bool reuse = false, HeavyHold* hold = nullptr,
+                                               int32_t* ccol_w = nullptr, int64_t clim = 0) {
   // carry: B rows sorted and cursors kept across windows (heavy_stage_ranges).
   // reuse (resident rows, one chunk): hs still holds this window's ranges and
   // their scan from the previous call, whose window ended at the same column.
+  // ccol_w (slot mode): the window's first output column entry; every
+  // product stores its column at its slot there (slots below clim), so the
+  // write-out only copies values.
   const int tid = threadIdx.x;
   for (int cb = s; cb < e; cb += kHeavyT) {
     const int m = (e - cb) < kHeavyT ? (e - cb) : kHeavyT;
@@ -2291,10 +2295,11 @@ __device__ __forceinline__ void heavy_products(const RowArgs& A, HeavyStage& hs,
         } else if (MODE == kHeavyDense) {
           atomicOr(&bits[b >> 5], 1u << (b & 31u));
           if (!A.upper || j[u] >= i) atomicAdd(&vals[b], pv[u]);
-        } else if (!A.upper || j[u] >= i) {
+        } else {
           const uint32_t w = b >> 5;
           const uint32_t pos = pre[w] + __popc(bits[w] & ((1u << (b & 31u)) - 1u));
-          atomicAdd(&vals[pos], pv[u]);
+          if (ccol_w != nullptr && (int64_t)pos < clim) ccol_w[pos] = j[u];
+          if (!A.upper || j[u] >= i) atomicAdd(&vals[pos], pv[u]);
         }
       }
     };
@@ -2503,12 +2508,12 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
           for (int u = 0; u < kHoldN; u++) {
             const int64_t jj = hold.j[u];
             if (jj >= w0 && jj < wcut) {
-              if (!A.upper || jj >= i) {
-                const uint32_t b = (uint32_t)(jj - w0);
-                const uint32_t w = b >> 5;
-                const uint32_t pos = pre[w] + __popc(bits[w] & ((1u << (b & 31u)) - 1u));
-                atomicAdd(&vals[pos], hold.v[u]);
-              }
+              const uint32_t b = (uint32_t)(jj - w0);
+              const uint32_t w = b >> 5;
+              const uint32_t pos = pre[w] + __popc(bits[w] & ((1u << (b & 31u)) - 1u));
+              // the column at its slot (products of one column store the same value)
+              if (out + (int64_t)pos < end) ccol[out + pos] = (int32_t)jj;
+              if (!A.upper || jj >= i) atomicAdd(&vals[pos], hold.v[u]);
               if (cut) atomicAdd(&cnt[hold.l[u]], 1);
             }
           }
@@ -2523,7 +2528,7 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
           // values of [w0, wcut); cursors move to the cut.  A resident row's
           // uncut window reuses the structure pass's ranges and their scan
           heavy_products<kHeavySlot>(A, hs, i, s, e, w0, wcut, sorted, first, true, cur, bits, pre, vals, res,
-                                     resident && cw == words);
+                                     resident && cw == words, nullptr, ccol + out, end - out);
         }
       } else {
         for (int t = tid; t < (int)(w1 - w0); t += kHeavyT) vals[t] = 0.0;
@@ -2532,35 +2537,12 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
       // columns and values of the words before the cut, in order
       int64_t run = 0;
       if (sparse) {
-        // one thread per slot: slot k lies in the last word w before the cut
-        // with pre[w] <= k (a search over the prefixes) at the (k - pre[w])-th
-        // set bit of that word, so consecutive slots are stored to
-        // consecutive output positions, whatever the words' densities
+        // the columns are at their slots already (stored by the pass that
+        // added the values): the values are copied out in slot order
         run = cw < words ? (int64_t)pre[cw] : (int64_t)pre[words - 1] + __popc(bits[words - 1]);
-        int dep = 0;
-        while ((1 << dep) < cw) dep++;
         for (int64_t k = tid; k < run; k += kHeavyT) {
-          int w = 0;
-          for (int d = dep - 1; d >= 0; d--) {
-            const int cand = w + (1 << d);
-            if (cand < cw && (int64_t)pre[cand] <= k) w = cand;
-          }
-          uint32_t x = bits[w];
-          int rk = (int)(k - (int64_t)pre[w]), bpos = 0;
-#pragma unroll
-          for (int sh = 16; sh > 0; sh >>= 1) {
-            const int c = __popc(x & ((1u << sh) - 1u));
-            if (rk >= c) {
-              rk -= c;
-              x >>= sh;
-              bpos += sh;
-            }
-          }
           const int64_t o = out + k;
-          if (o < end && k < vlim) {
-            ccol[o] = (int32_t)(w0 + (int64_t)w * 32 + bpos);
-            cval[o] = vals[k];
-          }
+          if (o < end && k < vlim) cval[o] = vals[k];
         }
       } else {
         for (int wb = 0; wb < cw; wb += kHeavyT) {
