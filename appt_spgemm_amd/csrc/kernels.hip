@@ -67,7 +67,9 @@ void set_dyn_lds(const void* fn, size_t bytes) {
   (void)hipGetDevice(&dev);
   auto it = cap.find(Key{fn, dev});
   if (it != cap.end() && it->second >= bytes) return;
-  (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  // a refused cap is not remembered: the launch that needs it then fails and
+  // the stage's error check reports it
+  if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) return;
   cap[Key{fn, dev}] = bytes;
 }
 
