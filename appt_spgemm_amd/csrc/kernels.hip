@@ -2155,7 +2155,28 @@ __device__ __forceinline__ void heavy_stage_ranges(const RowArgs& A, HeavyStage&
       bmax = bm.w;
     }
     int l = qs;
-    if (carry && l < h && bmax >= wend) {
+    if (carry && l < h && bmax >= wend && T == 1) {
+      // one lane per entry (chunks of more than kHeavyT / 2 entries, such as
+      // the hub rows' ones): gallop from the cursor, since a window usually
+      // ends a few positions past it, then bisect the last step (about
+      // 2 log2 of the positions consumed instead of log2 of the row)
+      int64_t step = 1;
+      while (l < h) {
+        const int p = (int64_t)(h - l) > step ? l + (int)step - 1 : h - 1;
+        if ((int64_t)A.bc[p] < wend) {
+          l = p + 1;
+          step <<= 1;
+        } else {
+          h = p;
+          break;
+        }
+      }
+      while (l < h) {
+        const int mid = l + ((h - l) >> 1);
+        if ((int64_t)A.bc[mid] < wend) l = mid + 1;
+        else h = mid;
+      }
+    } else if (carry && l < h && bmax >= wend) {
       // first position of the B row at or past the window end; l, h are
       // uniform over the group, whose lanes are active together
       const int base = (tid & (kWave - 1)) & ~(T - 1);
