@@ -2439,11 +2439,17 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
     const int s = A.ap[i], e = A.ap[i + 1];
     int64_t out = rowoff[i];
     const int64_t end = rowoff[i + 1];
-    // window shape of this row (uniform across the block)
-    const bool sparse = (int64_t)hi - lo + 1 > kHeavySparseSpan * dcols;
+    // window shape of this row (uniform across the block).  Rows of more
+    // than one chunk (hubs) whose outputs cover at least 28 % of their span
+    // take dense windows however wide the span: a dense window makes one
+    // product pass over each chunk where a sparse one makes two, and at that
+    // density its span / dcols windows are at most twice the sparse ones
+    const int mrow = e - s;
+    const int64_t span = (int64_t)hi - lo + 1;
+    const bool sparse =
+        span > kHeavySparseSpan * dcols && !(lbytes > 0 && mrow > kHeavyT && 100 * (end - out) >= 28 * span);
     // sparse rows of one chunk over sorted B rows keep their entries in LDS
     // (HeavyRes) behind the windows, whose shapes take the rest
-    const int mrow = e - s;
     const bool resident = sparse && lbytes > 0 && sorted && mrow > 0 && mrow <= kHeavyT;
     const int64_t rbytes = resident ? (((int64_t)12 * mrow + 15) & ~(int64_t)15) : 0;
     HeavyRes res{nullptr, nullptr, nullptr};
