@@ -2440,14 +2440,16 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
     int64_t out = rowoff[i];
     const int64_t end = rowoff[i + 1];
     // window shape of this row (uniform across the block).  Rows of more
-    // than one chunk (hubs) whose outputs cover at least 28 % of their span
+    // than one chunk (hubs) whose outputs cover at least 12 % of their span
     // take dense windows however wide the span: a dense window makes one
-    // product pass over each chunk where a sparse one makes two, and at that
-    // density its span / dcols windows are at most twice the sparse ones
+    // product pass over each chunk where a sparse one makes two, and a hub
+    // row's chunk passes (window-end searches, scans, barriers) cost more
+    // than its extra windows (R-MAT 2^21: 28 % 0.838 s, 12 % 0.824-0.837 s, 4 %
+    // 0.841 s; for rows of one chunk sparse windows stay faster)
     const int mrow = e - s;
     const int64_t span = (int64_t)hi - lo + 1;
     const bool sparse =
-        span > kHeavySparseSpan * dcols && !(lbytes > 0 && mrow > kHeavyT && 100 * (end - out) >= 28 * span);
+        span > kHeavySparseSpan * dcols && !(lbytes > 0 && mrow > kHeavyT && 100 * (end - out) >= 12 * span);
     // sparse rows of one chunk over sorted B rows keep their entries in LDS
     // (HeavyRes) behind the windows, whose shapes take the rest
     const bool resident = sparse && lbytes > 0 && sorted && mrow > 0 && mrow <= kHeavyT;
