@@ -2575,6 +2575,25 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
           const int64_t o = out + k;
           if (o < end && k < vlim) cval[o] = vals[k];
         }
+      } else if (cw <= kHeavyT) {
+        // dense window of at most kHeavyT words: word prefixes (parked in
+        // hs.off, free after the product pass), then one thread per column,
+        // so consecutive set columns are stored to consecutive positions
+        const uint32_t x = tid < cw ? bits[tid] : 0u;
+        const int64_t p0 = block_scan_excl<int64_t>((int64_t)__popc(x), hs.red, &run);
+        hs.off[tid] = p0;
+        __syncthreads();
+        const int ncol = cw * 32;
+        for (int c = tid; c < ncol; c += kHeavyT) {
+          const uint32_t y = bits[c >> 5];
+          if ((y >> (c & 31)) & 1u) {
+            const int64_t o = out + hs.off[c >> 5] + __popc(y & ((1u << (c & 31)) - 1u));
+            if (o < end && c < vlim) {
+              ccol[o] = (int32_t)(w0 + c);
+              cval[o] = vals[c];
+            }
+          }
+        }
       } else {
         for (int wb = 0; wb < cw; wb += kHeavyT) {
           const int w = wb + tid;
