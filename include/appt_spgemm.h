@@ -88,7 +88,7 @@ typedef struct appt_tuning {
   int32_t lds_budget_bytes;   /* LDS bytes a workgroup may take */
   int32_t bitmap_words_factor;/* numeric dense window when words(span) <= factor * distinct words */
   int32_t group_for_len[4];   /* lanes per row for mean B-row length <= 8,16,32,inf (0 = default) */
-  int32_t heavy_window_cols;  /* column-window cap of the heavy-row bin (0 = as wide as LDS allows) */
+  int32_t heavy_window_cols;  /* fixed column window of the heavy-row bin (0 = shaped per row within the LDS) */
   int32_t sort_span_factor;   /* numeric pass: rows of at most 512 products whose columns are
                                  scattered (fewer than 4 columns per 3 distinct 32-column words) take
                                  the register-sort bin when their output span exceeds factor * products
