@@ -228,3 +228,36 @@ def test_c5_served_poisson_stream(eng):
     finally:
         for d in dev:
             d.free()
+
+
+@pytest.mark.parametrize("op", ["AA", "AAt"])
+def test_rmat18_heavy_windows_default_shapes(eng, op):
+    """R-MAT 2^18 in one call with the default heavy-row windows: rows wider
+    than eight dense windows take sparse windows shaped per row (held values,
+    cut windows, resident entries), the widest hubs (several 1024-entry
+    chunks) dense windows; the 64 heaviest rows and 128 random ones equal the
+    oracle's."""
+    m = ap.rmat(18, 16, seed=4)
+    rng = np.random.default_rng(9)
+    f = row_products(m, op)
+    heavy = np.argsort(f)[-64:]
+    sample = np.unique(np.concatenate([heavy, rng.choice(m.n, 128, replace=False), [0, m.n - 1]]))
+    b = (m.rowptr, m.col, m.val) if op == "AA" else oracle.transpose(m.n, m.rowptr, m.col, m.val)
+    want = oracle_rows(m, b, sample, op)
+    say(f"case rmat 2^18 {op}: max row {int(np.diff(m.rowptr).max())} entries, {len(sample)} sampled rows")
+    A = eng.upload(m)
+    try:
+        R = eng.compute(A, op)
+        try:
+            rows = eng.bin_report()["numeric"].get("heavy", (0, 0))[0]
+            say(f"  nnz(C) {R.nnz}, heavy rows {rows}")
+            assert rows > 1000
+            for i in sample:
+                h = R.download_rows(int(i), int(i) + 1)
+                o_rp, o_col, o_val, o_scale = want[i]
+                assert_structure(h.rowptr, h.col, o_rp, o_col)
+                assert_values(h.val, o_val, o_scale, exact=False)
+        finally:
+            R.release()
+    finally:
+        A.free()
