@@ -96,6 +96,15 @@ APPT_HD inline void heavy_row_shape(int64_t bytes, int64_t c, int64_t span, int6
   *vcap = (int32_t)((bytes - (w / 32) * 8) / 8);
 }
 
+// Dense window width in `bytes` of LDS (no tuning cap): a bitmap of the
+// window's own words and one f64 slot per column, 65 / 8 bytes per column,
+// a multiple of 64 columns (so the slots after the bitmap stay 8-byte
+// aligned), at least 64.
+APPT_HD inline int32_t heavy_dense_cols(int64_t bytes) {
+  const int64_t d = (bytes * 8 / 65) & ~(int64_t)63;
+  return (int32_t)(d < 64 ? 64 : d);
+}
+
 // Cap on the next sparse window after one that took `taken` columns (cut:
 // the window held more outputs than its slots), from the cap `wlim` the
 // window had: a cut window caps the next at 3/2 of the columns it took, an

@@ -2450,9 +2450,9 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
     const int64_t span = (int64_t)hi - lo + 1;
     const bool sparse =
         span > kHeavySparseSpan * dcols && !(lbytes > 0 && mrow > kHeavyT && 100 * (end - out) >= 12 * span);
-    // sparse rows of one chunk over sorted B rows keep their entries in LDS
+    // rows of one chunk over sorted B rows keep their entries in LDS
     // (HeavyRes) behind the windows, whose shapes take the rest
-    const bool resident = sparse && lbytes > 0 && sorted && mrow > 0 && mrow <= kHeavyT;
+    const bool resident = lbytes > 0 && sorted && mrow > 0 && mrow <= kHeavyT;
     const int64_t rbytes = resident ? (((int64_t)12 * mrow + 15) & ~(int64_t)15) : 0;
     HeavyRes res{nullptr, nullptr, nullptr};
     if (resident) {
@@ -2468,6 +2468,15 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
       }
       __syncthreads();
     }
+    // dense windows without a tuning cap: a bitmap of the window's own
+    // words, f64 slots, and the resident entries share the LDS (65 / 8 bytes
+    // per column), about 12 % wider than dcols
+    int32_t dcr = dcols;
+    double* dv = dvals;
+    if (!sparse && lbytes > 0) {
+      dcr = heavy_dense_cols(lbytes - rbytes);
+      dv = (double*)(lds_raw + (size_t)(dcr / 32) * 4);
+    }
     int64_t w0 = lo, wlim = 0;
     bool first = true;
     while (w0 <= hi) {
@@ -2477,9 +2486,9 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
       if (sparse && lbytes > 0)
         heavy_row_shape(lbytes - rbytes, end - out, (int64_t)hi + 1 - w0, wlim, &wcr, &vcr);
       uint32_t* pre = bits + wcr / 32;
-      double* vals = sparse ? (double*)(lds_raw + (size_t)(wcr / 32) * 8) : dvals;
-      const int vlim = sparse ? vcr : dcols;
-      const int64_t wspan = sparse ? wcr : dcols;
+      double* vals = sparse ? (double*)(lds_raw + (size_t)(wcr / 32) * 8) : dv;
+      const int vlim = sparse ? vcr : dcr;
+      const int64_t wspan = sparse ? wcr : dcr;
       const int64_t w1 = (w0 + wspan) < ((int64_t)hi + 1) ? (w0 + wspan) : ((int64_t)hi + 1);
       const int words = (int)((w1 - w0 + 31) >> 5);
       for (int t = tid; t < words; t += kHeavyT) bits[t] = 0u;
@@ -2563,7 +2572,7 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols,
         }
       } else {
         for (int t = tid; t < (int)(w1 - w0); t += kHeavyT) vals[t] = 0.0;
-        heavy_products<kHeavyDense>(A, hs, i, s, e, w0, w1, sorted, first, true, cur, bits, pre, vals);
+        heavy_products<kHeavyDense>(A, hs, i, s, e, w0, w1, sorted, first, true, cur, bits, pre, vals, res);
       }
       // columns and values of the words before the cut, in order
       int64_t run = 0;

@@ -80,7 +80,12 @@ int main() {
             if (wcr < kHeavyMinCols || (wcr & 31) || vcr < kHeavyMinSlots) return fail("row shape floor", wcr, vcr);
             if ((int64_t)(wcr / 32) * 8 + (int64_t)vcr * 8 > lbytes) return fail("row shape exceeds LDS", wcr, vcr);
           }
-          const int64_t wspan = sparse ? wcr : dcols;
+          // dense windows without a cap: the width the LDS left by a row of
+          // 1024 resident entries (12 KB) gives, as the kernel computes it
+          const int32_t dcr = cap == 0 ? heavy_dense_cols(lbytes - 12 * 1024) : dcols;
+          if (!sparse && cap == 0 && ((int64_t)(dcr / 32) * 4 + (int64_t)dcr * 8 > lbytes - 12 * 1024 || dcr < 64))
+            return fail("dense width exceeds LDS", dcr, lbytes);
+          const int64_t wspan = sparse ? wcr : dcr;
           const int64_t w1 = (w0 + wspan) < (hi + 1) ? (w0 + wspan) : (hi + 1);
           const int64_t words = (w1 - w0 + 31) >> 5;
           int64_t wcut = w1;
@@ -99,7 +104,7 @@ int main() {
           if (wcut <= w0) return fail("window does not advance", w0, wcut);
           const int64_t in = outputs(w0, wcut);
           if (sparse && in > vcr) return fail("sparse window over capacity", in, vcr);
-          if (!sparse && wcut - w0 > dcols) return fail("dense window over width", wcut - w0, dcols);
+          if (!sparse && wcut - w0 > dcr) return fail("dense window over width", wcut - w0, dcr);
           emitted += in;
           if (sparse) wlim = heavy_next_wlim(wcut < w1, wcut - w0, wlim);
           w0 = wcut;
