@@ -2392,9 +2392,10 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_sym(RowArgs A, int32_t wc, in
   }
 }
 // Numeric heavy rows.  Two window shapes, chosen per row:
-// * dense: a window of up to `vcap` columns, one f64 slot per column; one
-//   product pass sets the bits and accumulates, then the window's set columns
-//   are compacted into C.
+// * dense: a window of up to dcols columns (without a tuning cap the width
+//   heavy_dense_cols gives for the LDS left by the row's resident entries),
+//   one f64 slot per column; one product pass sets the bits and
+//   accumulates, then the window's set columns are compacted into C.
 // * sparse: a window of up to `wcols` columns (1 bit plus a 32-bit prefix
 //   count per word) holding at most `vcap` outputs: a structure pass sets the
 //   window's bits, prefix counts give each column its slot, the window is cut
@@ -2412,9 +2413,11 @@ __global__ void __launch_bounds__(kHeavyT) k_heavy_sym(RowArgs A, int32_t wc, in
 // columns) gets a few windows of several 10^5 columns instead of 16 of
 // 131,072, and a dense row windows whose structure pass ends near the cut.
 // A cut caps the next window at 3/2 of the columns it took.
-// Cursors carry over between windows as in k_heavy_sym.  LDS: bits (window
-// columns / 32 words), then either prefix counts + value slots (sparse) or
-// dcols slots (dense).
+// Cursors carry over between windows as in k_heavy_sym (in LDS for resident
+// rows).  LDS: bits (window columns / 32 words; with a tuning cap the dense
+// slots start after wcols / 32 words), then either prefix counts + value
+// slots (sparse) or one slot per column (dense), and the resident entries
+// at the end.
 __global__ void __launch_bounds__(kHeavyT) k_heavy_num(RowArgs A, int32_t wcols, int32_t vcap, int32_t dcols,
                                                        int64_t lbytes,
                                                        int32_t* __restrict__ work, const int32_t* __restrict__ bflag,
